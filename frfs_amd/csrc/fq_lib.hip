@@ -1,0 +1,975 @@
+// MI355X (gfx950) fast spectral collision operator: kernels and C ABI.
+//
+// Pipeline per batch ("chunk") of D DOFs, all volumes stored plane-major
+// [plane][I][J] (J fastest, plane = the z index K or vz):
+//   k_fwd_z    f (AoSoA, per source)  -> T0  : DFT- along z           (r2z + cuFFT fwd)
+//   k_fwd_xy   T0 -> F (FP64)                : DFT- along x, y, / n
+//   k_inv_z    F -> Z_i  (T)                 : e^{i th_z l_z} . F, DFT+ along z
+//   k_plane    Z -> P_g  (T)                 : e^{i(th_x l_x + th_y l_y)}, DFT+ x,y,
+//                                              products, sum in group, DFT- x,y
+//   k_accum    P, F -> Y (FP64)              : DFT- along z, H += W . Ghat (FP64),
+//                                              L += Lt . F, DFT+ along z of H and L
+//   k_out      Y -> Q (AoSoA, per target)    : DFT+ x,y; Q = Re gain - f Re loss
+//   k_proj_*   optional conservation projection (FP64)
+// T = double (FQ_PREC_FP64) or float (FQ_PREC_MIXED).
+//
+// Lane mapping: a "unit" (one plane or one row of N pencils) is LPP = 2*NP
+// lanes, NP = N rounded up to a power of two; lane = 2*J + half, the two
+// lanes of a pencil exchange through DPP (see fft.hpp).  Global loads and
+// stores walk J across lanes, so every access is a contiguous row segment.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/frfs_fq.h"
+#include "fft.hpp"
+#include "plan.hpp"
+
+using fq::cpx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return fail(FQ_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int pow2ceil(int n) { return n <= 8 ? 8 : n <= 16 ? 16 : n <= 32 ? 32 : 64; }
+
+template <int N> struct Geo {
+    static constexpr int E = N / 2;
+    static constexpr int NP = pow2ceil(N);
+    static constexpr int LPP = 2 * NP;        // lanes per unit
+    static constexpr int N2 = N * N;
+    static constexpr int VOL = N * N * N;
+    static constexpr int LDSROW = N + 1;      // padded LDS row (complex elements)
+};
+
+constexpr int kPencilThreads = 256;           // pencil kernels (no LDS)
+constexpr int kPlaneThreads = 128;            // plane kernels (LDS transpose per unit)
+
+template <typename T> __device__ __forceinline__ cpx<T> ld(const cpx<T> *p) { return *p; }
+
+template <typename To, typename Ti> __device__ __forceinline__ cpx<To> cvt(cpx<Ti> v) {
+    return {To(v.x), To(v.y)};
+}
+
+struct AosoaPtrs {
+    const double *f[4];
+    double *q[4];
+};
+
+struct Dof {
+    int64_t first;     // global DOF of chunk slot 0
+    int32_t nupts;
+    int32_t soasz;
+    int64_t ldim;
+    int64_t nvol;      // N^3
+    __device__ __forceinline__ int64_t base(int slot) const {
+        int64_t dof = first + slot;
+        int64_t elem = dof / nupts, upt = dof % nupts;
+        return upt * ldim + (elem / soasz) * nvol * soasz + elem % soasz;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Weight-table precompute (FP64), replaces precompute_bb (vhs-gll.mako:71-101)
+// and precompute_bc_{pq} (vss-gll.mako:53-95).  Table layout is plane-major
+// [K][I][J] with l = (l0[I], l0[J], l0[K]).
+// ---------------------------------------------------------------------------
+struct DevTerm {
+    int kind, x, y, pair;
+    double A, B, scale;
+};
+
+__device__ __forceinline__ double l0of(int idx, int N) { return double(idx < N / 2 ? idx : idx - N); }
+
+__device__ __forceinline__ double sinc_ref(double s) { return sin(s + 1e-15) / (s + 1e-15); }
+
+__global__ void k_tables(cpx<double> *tabs, const DevTerm *terms, const int *tbeg, int N,
+                         double L, double sw, double swpre, int M, int Mpre, int Nrho, int ns,
+                         const double *qz, const double *qw, const double *sz,
+                         const double *szpre, const double *masses, const double *eta,
+                         const double *gam) {
+    const int n = N * N * N;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tab = blockIdx.y;
+    if (idx >= n) return;
+    const int K = idx / (N * N), I = (idx / N) % N, J = idx % N;
+    const double lx = l0of(I, N), ly = l0of(J, N), lz = l0of(K, N);
+    const double cSqr = sqrt(lx * lx + ly * ly + lz * lz);
+    double acc_re = 0, acc_im = 0;
+    for (int t = tbeg[tab]; t < tbeg[tab + 1]; ++t) {
+        const DevTerm tm = terms[t];
+        if (tm.kind == 0) {
+            acc_re += tm.scale * (tm.A * sinc_ref(tm.B * cSqr));
+        } else {
+            const int p = tm.pair / ns, q = tm.pair % ns;
+            const double mpq = masses[q] / (masses[p] + masses[q]);
+            const double e = eta[tm.pair], g = gam[tm.pair];
+            const int x0 = tm.kind == 1 ? tm.x : 0, x1 = tm.kind == 1 ? tm.x + 1 : Nrho;
+            double c_re = 0, c_im = 0;
+            for (int x = x0; x < x1; ++x) {
+                const double fac = M_PI / L * qz[x];
+                const double fac_b = swpre * pow(qz[x], g + 2);
+                const double fac_c = qw[x] * sw * fac_b;
+                const int y0 = tm.kind == 1 ? tm.y : 0, y1 = tm.kind == 1 ? tm.y + 1 : M;
+                for (int y = y0; y < y1; ++y) {
+                    double s_re = 0, s_im = 0;
+                    for (int k = 0; k < Mpre; ++k) {
+                        const double c0 = szpre[3 * k], c1 = szpre[3 * k + 1], c2 = szpre[3 * k + 2];
+                        // the host verified 1 + sigma.sigma' >= 0 in exact order; clamp
+                        // contraction-rounded zeros (exact antipodes) to 0
+                        const double base = 1 + sz[3 * y] * c0 + sz[3 * y + 1] * c1 + sz[3 * y + 2] * c2;
+                        const double Bpq = pow(fmax(base, 0.0), e);
+                        const double Ld = fac * ((c0 * lx) + (c1 * ly) + (c2 * lz));
+                        if (tm.kind == 1) {
+                            s_re += Bpq * fac_b * cos(mpq * Ld);
+                            s_im += -Bpq * fac_b * sin(mpq * Ld);
+                        } else {
+                            s_re += Bpq * fac_c * cos(Ld);
+                            s_im += -Bpq * fac_c * sin(Ld);
+                        }
+                    }
+                    c_re += s_re;
+                    c_im += s_im;
+                }
+            }
+            acc_re += tm.scale * c_re;
+            acc_im += tm.scale * c_im;
+        }
+    }
+    tabs[int64_t(tab) * n + idx] = {acc_re, acc_im};
+}
+
+// ---------------------------------------------------------------------------
+// k_fwd_z: AoSoA gather + DFT- along z.  unit = (slot, src, I).
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(256) void k_fwd_z(AosoaPtrs ptr, Dof dof, cpx<double> *T0, int nsrc,
+                                               int64_t nunits) {
+    using G = Geo<N>;
+    constexpr int E = G::E;
+    const int lt = threadIdx.x % G::LPP;
+    const int64_t unit = int64_t(blockIdx.x) * (blockDim.x / G::LPP) + threadIdx.x / G::LPP;
+    if (unit >= nunits) return;   // whole unit exits together (no barriers here)
+    const int I = int(unit % N);
+    const int64_t ds = unit / N;
+    const int s = int(ds % nsrc);
+    const int slot = int(ds / nsrc);
+    const int J = lt >> 1, half = lt & 1;
+    const bool act = J < N;
+    const double *f = ptr.f[s];
+    const int64_t b = dof.base(slot);
+    cpx<double> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int K = half * E + e;
+        const int64_t vv = (int64_t(I) * N + J) * N + K;
+        v[e] = {act ? f[b + vv * dof.soasz] : 0.0, 0.0};
+    }
+    fq::fft2_dif<N, -1>(v, half);
+    cpx<double> *dst = T0 + (int64_t(slot) * nsrc + s) * G::VOL + int64_t(I) * N + J;
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) dst[int64_t(half + 2 * e) * G::N2] = v[e];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Shared plane helpers.  A plane unit has LPP lanes; `row` = lt >> 1 is the
+// column J in the column phase and the row (x index) in the row phase.
+// ---------------------------------------------------------------------------
+
+// k_fwd_xy: plane (slot, src, K'):  DFT- along x then y, divide by n.
+template <int N>
+__global__ __launch_bounds__(kPlaneThreads) void k_fwd_xy(const cpx<double> *T0, cpx<double> *F, int64_t nunits) {
+    using G = Geo<N>;
+    constexpr int E = G::E;
+    constexpr int UPB = kPlaneThreads / G::LPP;
+    __shared__ cpx<double> lds[UPB][N * G::LDSROW];
+    const int u = threadIdx.x / G::LPP;
+    const int lt = threadIdx.x % G::LPP;
+    const int64_t unit = int64_t(blockIdx.x) * UPB + u;
+    const bool valid = unit < nunits;
+    const int c = lt >> 1, half = lt & 1;
+    const bool act = valid && c < N;
+    const cpx<double> *src = T0 + unit * G::N2;
+    cpx<double> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = act ? src[(half * E + e) * N + c] : cpx<double>{0, 0};
+    fq::fft2_dif<N, -1>(v, half);                       // over I -> I' cyclic
+    cpx<double> *L = lds[u];
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) L[(half + 2 * e) * G::LDSROW + c] = v[e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = act ? L[c * G::LDSROW + half * E + e] : cpx<double>{0, 0};
+    fq::fft2_dif<N, -1>(v, half);                       // over J -> J' cyclic, row I' = c
+    const double nn = double(G::VOL);
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = {v[e].x / nn, v[e].y / nn};   // FTf /= vsize (vhs-gll.mako:120)
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) L[c * G::LDSROW + half + 2 * e] = v[e];
+    }
+    __syncthreads();
+    cpx<double> *dst = F + unit * G::N2;
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) dst[(half * E + e) * N + c] = L[(half * E + e) * G::LDSROW + c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_inv_z: unit = (slot, item, I).  Z_i = DFT+_z( e^{i th_z l_z} F_src ).
+// ---------------------------------------------------------------------------
+template <int N, typename T>
+__global__ __launch_bounds__(256) void k_inv_z(const cpx<double> *F, cpx<T> *Z, const cpx<T> *ph,
+                                               const int *item_src, int nitems, int nsrc,
+                                               int64_t nunits) {
+    using G = Geo<N>;
+    constexpr int E = G::E;
+    const int lt = threadIdx.x % G::LPP;
+    const int64_t unit = int64_t(blockIdx.x) * (blockDim.x / G::LPP) + threadIdx.x / G::LPP;
+    if (unit >= nunits) return;
+    const int I = int(unit % N);
+    const int64_t di = unit / N;
+    const int it = int(di % nitems);
+    const int slot = int(di / nitems);
+    const int J = lt >> 1, half = lt & 1;
+    const bool act = J < N;
+    const int s = item_src[it];
+    const cpx<double> *src = F + (int64_t(slot) * nsrc + s) * G::VOL + I * N + J;
+    const cpx<T> *phz = ph + (int64_t(it) * 3 + 2) * N;
+    cpx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int K = half * E + e;
+        cpx<T> x = act ? cvt<T>(src[int64_t(K) * G::N2]) : cpx<T>{0, 0};
+        v[e] = fq::cmul(x, phz[K]);
+    }
+    fq::fft2_dif<N, 1>(v, half);
+    cpx<T> *dst = Z + (int64_t(slot) * nitems + it) * G::VOL + I * N + J;
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) dst[int64_t(half + 2 * e) * G::N2] = v[e];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_plane: unit = (slot, group, vz).  For each product of the group:
+//   E_i = DFT+_{x,y}( e^{i(th_x l_x + th_y l_y)} Z_i(:,:,vz) ),  acc += coef E1 E2
+// then P_g(:,:,vz) = DFT-_{x,y}(acc).
+// ---------------------------------------------------------------------------
+struct DevProd { int i1, i2; double coef; };
+
+template <int N, typename T>
+__device__ __forceinline__ void plane_inv(const cpx<T> *src, const cpx<T> *ph, cpx<T> *L, cpx<T> *v,
+                                          int c, int half, bool act) {
+    using G = Geo<N>;
+    constexpr int E = G::E;
+    // column phase: lane (J=c, half) over I (block)
+    const cpx<T> py = act ? ph[N + c] : cpx<T>{1, 0};
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int I = half * E + e;
+        cpx<T> x = act ? src[I * N + c] : cpx<T>{0, 0};
+        v[e] = fq::cmul(x, fq::cmul(ph[I], py));
+    }
+    fq::fft2_dif<N, 1>(v, half);                 // -> vx = half + 2e
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) L[(half + 2 * e) * G::LDSROW + c] = v[e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = act ? L[c * G::LDSROW + half * E + e] : cpx<T>{0, 0};
+    fq::fft2_dif<N, 1>(v, half);                 // row vx = c: -> vy = half + 2e
+}
+
+template <int N, typename T, bool MULTI>
+__global__ __launch_bounds__(kPlaneThreads) void k_plane(const cpx<T> *Z, cpx<T> *P, const cpx<T> *ph,
+                                               const DevProd *prods, const int *g_p0, const int *g_np,
+                                               int nitems, int ngroups, int64_t nunits) {
+    using G = Geo<N>;
+    constexpr int E = G::E;
+    constexpr int UPB = kPlaneThreads / G::LPP;
+    __shared__ cpx<T> lds[UPB][N * G::LDSROW];
+    const int u = threadIdx.x / G::LPP;
+    const int lt = threadIdx.x % G::LPP;
+    const int64_t unit = int64_t(blockIdx.x) * UPB + u;
+    const bool valid = unit < nunits;
+    const int vz = int(unit % N);
+    const int64_t dg = unit / N;
+    const int g = valid ? int(dg % ngroups) : 0;
+    const int slot = int(dg / ngroups);
+    const int c = lt >> 1, half = lt & 1;
+    const bool act = valid && c < N;
+    cpx<T> *L = lds[u];
+    const int p0 = g_p0[g], np = g_np[g];
+    const cpx<T> *zs = Z + int64_t(slot) * nitems * G::VOL + int64_t(vz) * G::N2;
+    cpx<T> acc[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) acc[e] = {0, 0};
+    // plane_inv contains barriers: every lane of the block must run the same
+    // number of product iterations, so loop to the block-wide maximum and
+    // mask the iterations a unit does not own.
+    __shared__ int s_np[UPB];
+    if (lt == 0) s_np[u] = valid ? np : 0;
+    __syncthreads();
+    int npmax = 0;
+    for (int k = 0; k < UPB; ++k) npmax = max(npmax, s_np[k]);
+    for (int k = 0; k < npmax; ++k) {
+        const bool on = act && k < np;
+        DevProd pr = prods[p0 + (k < np ? k : 0)];
+        cpx<T> e1[E], e2[E];
+        plane_inv<N, T>(zs + int64_t(pr.i1) * G::VOL, ph + int64_t(pr.i1) * 3 * N, L, e1, c, half, on);
+        plane_inv<N, T>(zs + int64_t(pr.i2) * G::VOL, ph + int64_t(pr.i2) * 3 * N, L, e2, c, half, on);
+        const T cf = on ? T(pr.coef) : T(0);
+        if (MULTI) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                cpx<T> pe = fq::cmul(e1[e], e2[e]);
+                acc[e] = {acc[e].x + cf * pe.x, acc[e].y + cf * pe.y};
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) acc[e] = fq::cscale(fq::cmul(e1[e], e2[e]), cf);
+        }
+    }
+    // forward: acc is cyclic over vy (row vx = c)
+    fq::fft2_dit<N, -1>(acc, half);               // -> J' = half*E + e
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) L[c * G::LDSROW + half * E + e] = acc[e];
+    }
+    __syncthreads();
+    // column phase: lane (J' = c, half), input over vx cyclic
+#pragma unroll
+    for (int e = 0; e < E; ++e) acc[e] = act ? L[(half + 2 * e) * G::LDSROW + c] : cpx<T>{0, 0};
+    fq::fft2_dit<N, -1>(acc, half);               // -> I' = half*E + e
+    cpx<T> *dst = P + (int64_t(slot) * ngroups + g) * G::VOL + int64_t(vz) * G::N2;
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) dst[(half * E + e) * N + c] = acc[e];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_accum: unit = (slot, target, I').  H = sum_c W_c . DFT-_z(P_g(c)),
+// L = sum_j Lt_j . F_src(j); Y = DFT+_z of H and L.  FP64 accumulation.
+// ---------------------------------------------------------------------------
+template <int N, typename T>
+__global__ __launch_bounds__(256) void k_accum(const cpx<T> *P, const cpx<double> *F,
+                                               const cpx<double> *W, const int *c_group,
+                                               const int *t_c0, const int *t_cn, const int *l_src,
+                                               const int *t_l0, const int *t_ln, int ncontribs,
+                                               cpx<double> *Y, int ngroups, int nsrc, int ntgt,
+                                               int64_t nunits) {
+    using G = Geo<N>;
+    constexpr int E = G::E;
+    const int lt = threadIdx.x % G::LPP;
+    const int64_t unit = int64_t(blockIdx.x) * (blockDim.x / G::LPP) + threadIdx.x / G::LPP;
+    if (unit >= nunits) return;
+    const int I = int(unit % N);
+    const int64_t dt = unit / N;
+    const int t = int(dt % ntgt);
+    const int slot = int(dt / ntgt);
+    const int J = lt >> 1, half = lt & 1;
+    const bool act = J < N;
+    const int64_t off = int64_t(I) * N + J;
+    cpx<double> H[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) H[e] = {0, 0};
+    const int c0 = t_c0[t], cn = t_cn[t];
+    for (int ci = c0; ci < c0 + cn; ++ci) {
+        const int g = c_group[ci];
+        const cpx<T> *src = P + (int64_t(slot) * ngroups + g) * G::VOL + off;
+        cpx<T> v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = act ? src[int64_t(half + 2 * e) * G::N2] : cpx<T>{0, 0};
+        fq::fft2_dit<N, -1>(v, half);             // -> K' = half*E + e
+        const cpx<double> *w = W + int64_t(ci) * G::VOL + off;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const cpx<double> we = act ? w[int64_t(half * E + e) * G::N2] : cpx<double>{0, 0};
+            const cpx<double> pe = cvt<double>(v[e]);
+            H[e] = H[e] + fq::cmul(we, pe);
+        }
+    }
+    fq::fft2_dif<N, 1>(H, half);                  // -> vz = half + 2e
+    cpx<double> *yg = Y + ((int64_t(slot) * ntgt + t) * 2 + 0) * G::VOL + off;
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) yg[int64_t(half + 2 * e) * G::N2] = H[e];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) H[e] = {0, 0};
+    const int l0 = t_l0[t], ln = t_ln[t];
+    for (int li = l0; li < l0 + ln; ++li) {
+        const int s = l_src[li];
+        const cpx<double> *fs = F + (int64_t(slot) * nsrc + s) * G::VOL + off;
+        const cpx<double> *w = W + int64_t(ncontribs + li) * G::VOL + off;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int64_t k = int64_t(half * E + e) * G::N2;
+            if (act) H[e] = H[e] + fq::cmul(w[k], fs[k]);
+        }
+    }
+    fq::fft2_dif<N, 1>(H, half);
+    cpx<double> *yl = Y + ((int64_t(slot) * ntgt + t) * 2 + 1) * G::VOL + off;
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) yl[int64_t(half + 2 * e) * G::N2] = H[e];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_out: unit = (slot, target, vz).  gain = DFT+_{x,y}(Yg), loss = DFT+_{x,y}(Yl)
+// Q[v] = Re gain - f[v] Re loss, scattered into the AoSoA matrix
+// (output kernel, vhs-gll.mako:246-266; vss-gll.mako:275-300).
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ void plane_inv64(const cpx<double> *src, cpx<double> *L, cpx<double> *v, int c,
+                                            int half, bool act) {
+    using G = Geo<N>;
+    constexpr int E = G::E;
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = act ? src[(half * E + e) * N + c] : cpx<double>{0, 0};
+    fq::fft2_dif<N, 1>(v, half);
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) L[(half + 2 * e) * G::LDSROW + c] = v[e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = act ? L[c * G::LDSROW + half * E + e] : cpx<double>{0, 0};
+    fq::fft2_dif<N, 1>(v, half);
+}
+
+template <int N>
+__global__ __launch_bounds__(kPlaneThreads) void k_out(const cpx<double> *Y, AosoaPtrs ptr, Dof dof, int ntgt,
+                                             int64_t nunits) {
+    using G = Geo<N>;
+    constexpr int E = G::E;
+    constexpr int UPB = kPlaneThreads / G::LPP;
+    __shared__ cpx<double> lds[UPB][N * G::LDSROW];
+    const int u = threadIdx.x / G::LPP;
+    const int lt = threadIdx.x % G::LPP;
+    const int64_t unit = int64_t(blockIdx.x) * UPB + u;
+    const bool valid = unit < nunits;
+    const int vz = int(unit % N);
+    const int64_t dt = unit / N;
+    const int t = int(dt % ntgt);
+    const int slot = int(dt / ntgt);
+    const int c = lt >> 1, half = lt & 1;
+    const bool act = valid && c < N;
+    cpx<double> *L = lds[u];
+    const cpx<double> *yg = Y + ((int64_t(slot) * ntgt + t) * 2 + 0) * G::VOL + int64_t(vz) * G::N2;
+    cpx<double> gv[E], lv[E];
+    plane_inv64<N>(yg, L, gv, c, half, act);
+    plane_inv64<N>(yg + G::VOL, L, lv, c, half, act);
+    if (act) {
+        const double *f = ptr.f[t];
+        double *q = ptr.q[t];
+        const int64_t b = dof.base(slot);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int vy = half + 2 * e;
+            const int64_t vv = (int64_t(c) * N + vy) * N + vz;
+            const int64_t a = b + vv * dof.soasz;
+            q[a] = gv[e].x - f[a] * lv[e].x;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Conservation projection (FP64): Q <- Q - sum_k lambda_k phi_k with
+// phi = {1, v_x, v_y, v_z, |v|^2}, lambda = Gram^{-1} (sum_v Q phi cw):
+// the least-squares correction that zeroes the discrete mass, momentum and
+// energy moments (invariants of frfs/plugins/dgfsmomwriterstd.py:142-214).
+// One block per (slot, target).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void phis(int64_t v, int N, double L, double *ph) {
+    const int I = int(v / (int64_t(N) * N)), J = int((v / N) % N), K = int(v % N);
+    const double h = 2.0 * L / N;
+    const double cx = -L + L / N + h * I, cy = -L + L / N + h * J, cz = -L + L / N + h * K;
+    ph[0] = 1; ph[1] = cx; ph[2] = cy; ph[3] = cz; ph[4] = cx * cx + cy * cy + cz * cz;
+}
+
+__global__ __launch_bounds__(256) void k_project(AosoaPtrs ptr, Dof dof, int ntgt, int N, double L,
+                                                 const double *ginv) {
+    const int slot = blockIdx.x / ntgt, t = blockIdx.x % ntgt;
+    double *q = ptr.q[t];
+    const int64_t b = dof.base(slot);
+    const int64_t n = int64_t(N) * N * N;
+    double m[5] = {0, 0, 0, 0, 0};
+    for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
+        double ph[5];
+        phis(v, N, L, ph);
+        const double qv = q[b + v * dof.soasz];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) m[k] += qv * ph[k];
+    }
+    __shared__ double red[5][256];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = m[k];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    double lam[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        double s = 0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) s += ginv[i * 5 + k] * red[k][0];
+        lam[i] = s;
+    }
+    for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
+        double ph[5];
+        phis(v, N, L, ph);
+        double corr = 0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) corr += lam[k] * ph[k];
+        q[b + v * dof.soasz] -= corr;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+template <typename X> int upload(X **dst, const std::vector<X> &v) {
+    *dst = nullptr;
+    if (v.empty()) return FQ_SUCCESS;
+    HIPCHK(hipMalloc(dst, v.size() * sizeof(X)));
+    HIPCHK(hipMemcpy(*dst, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice));
+    return FQ_SUCCESS;
+}
+
+bool n_supported(int N) { return N == 8 || N == 16 || N == 24 || N == 32; }
+
+}  // namespace
+
+struct fq_plan {
+    fq::Plan P;
+};
+
+struct fq_handle {
+    fq::Plan P;
+    int precision = 0;
+    int flags = 0;
+    int device = 0;
+    int64_t chunk = 0;
+    // device tables
+    void *ph = nullptr;          // [items][3][N] cpx<T>
+    int *item_src = nullptr;
+    DevProd *prods = nullptr;
+    int *g_p0 = nullptr, *g_np = nullptr;
+    int *c_group = nullptr, *t_c0 = nullptr, *t_cn = nullptr;
+    int *l_src = nullptr, *t_l0 = nullptr, *t_ln = nullptr;
+    cpx<double> *W = nullptr;    // [contribs + loss][N^3]
+    double *ginv = nullptr;      // [25]
+    // scratch
+    cpx<double> *F = nullptr;
+    void *Z = nullptr;           // also T0 and Y
+    void *Pg = nullptr;
+    size_t scratch = 0;
+};
+
+namespace {
+
+template <typename T> int upload_phases(fq_handle *h) {
+    const fq::Plan &P = h->P;
+    const int N = P.N;
+    std::vector<cpx<T>> ph(P.items.size() * 3 * N);
+    for (size_t i = 0; i < P.items.size(); ++i)
+        for (int d = 0; d < 3; ++d)
+            for (int k = 0; k < N; ++k) {
+                const double l = double(k < N / 2 ? k : k - N);
+                const double a = P.items[i].th[d] * l;
+                ph[(i * 3 + d) * N + k] = {T(std::cos(a)), T(std::sin(a))};
+            }
+    cpx<T> *d = nullptr;
+    int rc = upload(&d, ph);
+    h->ph = d;
+    return rc;
+}
+
+int build_tables(fq_handle *h) {
+    const fq::Plan &P = h->P;
+    const int N = P.N;
+    const int64_t n = int64_t(N) * N * N;
+    const int ntab = P.n_tables();
+    std::vector<DevTerm> terms;
+    std::vector<int> tbeg(ntab + 1, 0);
+    for (auto &t : P.terms) tbeg[t.table + 1]++;
+    for (int i = 0; i < ntab; ++i) tbeg[i + 1] += tbeg[i];
+    terms.resize(P.terms.size());
+    std::vector<int> fill(ntab, 0);
+    for (auto &t : P.terms) terms[tbeg[t.table] + fill[t.table]++] = {t.kind, t.x, t.y, t.pair, t.A, t.B, t.scale};
+    DevTerm *d_terms = nullptr;
+    int *d_tbeg = nullptr;
+    double *d_qz = nullptr, *d_qw = nullptr, *d_sz = nullptr, *d_szpre = nullptr, *d_m = nullptr,
+           *d_eta = nullptr, *d_gam = nullptr;
+    int rc = FQ_SUCCESS;
+    auto chk = [&](int r) { if (rc == FQ_SUCCESS) rc = r; };
+    chk(upload(&d_terms, terms));
+    chk(upload(&d_tbeg, tbeg));
+    chk(upload(&d_qz, P.qz));
+    chk(upload(&d_qw, P.qw));
+    chk(upload(&d_sz, P.sz));
+    std::vector<double> one = {1.0};
+    chk(upload(&d_szpre, P.szpre.empty() ? one : P.szpre));
+    chk(upload(&d_m, P.masses.empty() ? one : P.masses));
+    chk(upload(&d_eta, P.eta.empty() ? one : P.eta));
+    chk(upload(&d_gam, P.gam.empty() ? one : P.gam));
+    if (rc == FQ_SUCCESS) {
+        hipError_t e = hipMalloc(&h->W, size_t(ntab) * n * sizeof(cpx<double>));
+        if (e != hipSuccess) rc = fail(FQ_ERR_ALLOC, std::string("hipMalloc tables: ") + hipGetErrorString(e));
+    }
+    if (rc == FQ_SUCCESS && ntab > 0) {
+        dim3 grid(unsigned((n + 255) / 256), unsigned(ntab));
+        hipLaunchKernelGGL(k_tables, grid, dim3(256), 0, 0, h->W, d_terms, d_tbeg, N, P.L, P.sw, P.swpre, P.M,
+                           P.Mpre, P.Nrho, P.nspcs, d_qz, d_qw, d_sz, d_szpre, d_m, d_eta, d_gam);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e != hipSuccess) rc = fail(FQ_ERR_HIP, std::string("k_tables: ") + hipGetErrorString(e));
+    }
+    for (void *p : {(void *)d_terms, (void *)d_tbeg, (void *)d_qz, (void *)d_qw, (void *)d_sz, (void *)d_szpre,
+                    (void *)d_m, (void *)d_eta, (void *)d_gam})
+        if (p) (void)hipFree(p);
+    return rc;
+}
+
+int build_projection(fq_handle *h) {
+    // Gram matrix of {1, v, |v|^2} over the velocity grid (host FP64), inverted.
+    const fq::Plan &P = h->P;
+    const int N = P.N;
+    const double L = P.L;
+    double Gm[25] = {0};
+    const double h_ = 2.0 * L / N;
+    for (int I = 0; I < N; ++I)
+        for (int J = 0; J < N; ++J)
+            for (int K = 0; K < N; ++K) {
+                const double cx = -L + L / N + h_ * I, cy = -L + L / N + h_ * J, cz = -L + L / N + h_ * K;
+                const double ph[5] = {1, cx, cy, cz, cx * cx + cy * cy + cz * cz};
+                for (int a = 0; a < 5; ++a)
+                    for (int b = 0; b < 5; ++b) Gm[a * 5 + b] += ph[a] * ph[b];
+            }
+    // Gauss-Jordan inverse
+    double A[5][10];
+    for (int a = 0; a < 5; ++a)
+        for (int b = 0; b < 10; ++b) A[a][b] = b < 5 ? Gm[a * 5 + b] : (b - 5 == a ? 1.0 : 0.0);
+    for (int c = 0; c < 5; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 5; ++r)
+            if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+        for (int b = 0; b < 10; ++b) std::swap(A[c][b], A[piv][b]);
+        const double d = A[c][c];
+        for (int b = 0; b < 10; ++b) A[c][b] /= d;
+        for (int r = 0; r < 5; ++r)
+            if (r != c) {
+                const double f = A[r][c];
+                for (int b = 0; b < 10; ++b) A[r][b] -= f * A[c][b];
+            }
+    }
+    std::vector<double> gi(25);
+    for (int a = 0; a < 5; ++a)
+        for (int b = 0; b < 5; ++b) gi[a * 5 + b] = A[a][b + 5];
+    return upload(&h->ginv, gi);
+}
+
+template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, const Dof &dof, int D,
+                                           hipStream_t st) {
+    using G = Geo<N>;
+    const fq::Plan &P = h->P;
+    const int nsrc = P.nsrc, ntgt = P.ntgt, nitems = int(P.items.size()), ngroups = P.ngroups;
+    constexpr int upb = kPencilThreads / G::LPP;
+    constexpr int upp = kPlaneThreads / G::LPP;
+    cpx<double> *T0 = reinterpret_cast<cpx<double> *>(h->Z);
+    cpx<T> *Z = reinterpret_cast<cpx<T> *>(h->Z);
+    cpx<T> *Pg = reinterpret_cast<cpx<T> *>(h->Pg);
+    cpx<double> *Y = reinterpret_cast<cpx<double> *>(h->Z);
+    const cpx<T> *ph = reinterpret_cast<const cpx<T> *>(h->ph);
+    auto blocks = [&](int64_t units) { return unsigned((units + upb - 1) / upb); };
+    auto pblocks = [&](int64_t units) { return unsigned((units + upp - 1) / upp); };
+    const dim3 bt(kPencilThreads), pt(kPlaneThreads);
+    int64_t u;
+    u = int64_t(D) * nsrc * N;
+    hipLaunchKernelGGL(k_fwd_z<N>, dim3(blocks(u)), bt, 0, st, ptr, dof, T0, nsrc, u);
+    hipLaunchKernelGGL(k_fwd_xy<N>, dim3(pblocks(u)), pt, 0, st, T0, h->F, u);
+    u = int64_t(D) * nitems * N;
+    hipLaunchKernelGGL((k_inv_z<N, T>), dim3(blocks(u)), bt, 0, st, h->F, Z, ph, h->item_src, nitems, nsrc, u);
+    u = int64_t(D) * ngroups * N;
+    bool multi = false;
+    for (int g = 0; g < ngroups; ++g) multi |= P.group_np[g] > 1;
+    if (multi)
+        hipLaunchKernelGGL((k_plane<N, T, true>), dim3(pblocks(u)), pt, 0, st, Z, Pg, ph, h->prods, h->g_p0,
+                           h->g_np, nitems, ngroups, u);
+    else
+        hipLaunchKernelGGL((k_plane<N, T, false>), dim3(pblocks(u)), pt, 0, st, Z, Pg, ph, h->prods, h->g_p0,
+                           h->g_np, nitems, ngroups, u);
+    u = int64_t(D) * ntgt * N;
+    hipLaunchKernelGGL((k_accum<N, T>), dim3(blocks(u)), bt, 0, st, Pg, h->F, h->W, h->c_group, h->t_c0,
+                       h->t_cn, h->l_src, h->t_l0, h->t_ln, int(P.contribs.size()), Y, ngroups, nsrc, ntgt, u);
+    hipLaunchKernelGGL(k_out<N>, dim3(pblocks(u)), pt, 0, st, Y, ptr, dof, ntgt, u);
+    if (h->flags & FQ_FLAG_PROJECT)
+        hipLaunchKernelGGL(k_project, dim3(unsigned(D * ntgt)), dim3(256), 0, st, ptr, dof, ntgt, N, P.L, h->ginv);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(FQ_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return FQ_SUCCESS;
+}
+
+template <int N> int dispatch_prec(fq_handle *h, const AosoaPtrs &ptr, const Dof &dof, int D, hipStream_t st) {
+    return h->precision == FQ_PREC_MIXED ? run_chunk<N, float>(h, ptr, dof, D, st)
+                                         : run_chunk<N, double>(h, ptr, dof, D, st);
+}
+
+int eval_impl(fq_handle *h, const AosoaPtrs &ptr, int32_t nupts, int64_t ldim, int32_t neles,
+              int64_t first_dof, int64_t ndof, void *stream) {
+    if (!h) return fail(FQ_ERR_INVALID, "null handle");
+    if (nupts <= 0 || neles <= 0 || ndof < 0 || first_dof < 0)
+        return fail(FQ_ERR_INVALID, "invalid DOF range or shape");
+    if (first_dof + ndof > int64_t(nupts) * neles) return fail(FQ_ERR_INVALID, "DOF range exceeds nupts*neles");
+    const int64_t n = int64_t(h->P.N) * h->P.N * h->P.N;
+    const int64_t nparr = (neles + 31) / 32 * 32;
+    if (ldim < n * nparr) return fail(FQ_ERR_INVALID, "ldim smaller than N^3 * padded neles");
+    if (ndof == 0) return FQ_SUCCESS;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != h->device) return fail(FQ_ERR_INVALID, "handle was created on another device");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    for (int64_t c0 = 0; c0 < ndof; c0 += h->chunk) {
+        const int D = int(std::min<int64_t>(h->chunk, ndof - c0));
+        Dof dof{first_dof + c0, nupts, 32, ldim, n};
+        int rc;
+        switch (h->P.N) {
+        case 8: rc = dispatch_prec<8>(h, ptr, dof, D, st); break;
+        case 16: rc = dispatch_prec<16>(h, ptr, dof, D, st); break;
+        case 24: rc = dispatch_prec<24>(h, ptr, dof, D, st); break;
+        case 32: rc = dispatch_prec<32>(h, ptr, dof, D, st); break;
+        default: rc = fail(FQ_ERR_UNSUPPORTED, "grid size not built");
+        }
+        if (rc != FQ_SUCCESS) return rc;
+    }
+    return FQ_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fq_last_error(void) { return g_err.c_str(); }
+
+int fq_plan_create(const fq_params *prm, fq_plan **out) {
+    if (!prm || !out) return fail(FQ_ERR_INVALID, "null argument");
+    *out = nullptr;
+    auto *p = new fq_plan();
+    std::string err;
+    if (!fq::plan_build(*prm, p->P, err)) {
+        delete p;
+        return fail(FQ_ERR_INVALID, err);
+    }
+    *out = p;
+    return FQ_SUCCESS;
+}
+
+int fq_plan_destroy(fq_plan *p) {
+    delete p;
+    return FQ_SUCCESS;
+}
+
+int fq_plan_get_info(const fq_plan *p, fq_plan_info *info) {
+    if (!p || !info) return fail(FQ_ERR_INVALID, "null argument");
+    fq::plan_info(p->P, *info);
+    return FQ_SUCCESS;
+}
+
+int fq_plan_export(const fq_plan *pl, int32_t *item_src, double *item_theta, int32_t *prod_i1, int32_t *prod_i2,
+                   double *prod_coef, int32_t *prod_group, int32_t *contrib_group, int32_t *contrib_target,
+                   int32_t *loss_src, int32_t *loss_target) {
+    if (!pl) return fail(FQ_ERR_INVALID, "null plan");
+    const fq::Plan &P = pl->P;
+    for (size_t i = 0; i < P.items.size(); ++i) {
+        if (item_src) item_src[i] = P.items[i].src;
+        if (item_theta)
+            for (int d = 0; d < 3; ++d) item_theta[3 * i + d] = P.items[i].th[d];
+    }
+    for (size_t k = 0; k < P.prods.size(); ++k) {
+        if (prod_i1) prod_i1[k] = P.prods[k].i1;
+        if (prod_i2) prod_i2[k] = P.prods[k].i2;
+        if (prod_coef) prod_coef[k] = P.prods[k].coef;
+        if (prod_group) prod_group[k] = P.prods[k].group;
+    }
+    for (size_t c = 0; c < P.contribs.size(); ++c) {
+        if (contrib_group) contrib_group[c] = P.contribs[c].group;
+        if (contrib_target) contrib_target[c] = P.contribs[c].tgt;
+    }
+    for (size_t l = 0; l < P.loss.size(); ++l) {
+        if (loss_src) loss_src[l] = P.loss[l].src;
+        if (loss_target) loss_target[l] = P.loss[l].tgt;
+    }
+    return FQ_SUCCESS;
+}
+
+int fq_create(const fq_params *prm, fq_handle **out) {
+    if (!prm || !out) return fail(FQ_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (prm->precision != FQ_PREC_FP64 && prm->precision != FQ_PREC_MIXED)
+        return fail(FQ_ERR_INVALID, "unknown precision");
+    if (!n_supported(prm->Nv)) return fail(FQ_ERR_UNSUPPORTED, "Nv must be one of 8, 16, 24, 32");
+    if (prm->soasz != 32) return fail(FQ_ERR_UNSUPPORTED, "soasz must be 32");
+    auto *h = new fq_handle();
+    std::string err;
+    if (!fq::plan_build(*prm, h->P, err)) {
+        delete h;
+        return fail(FQ_ERR_INVALID, err);
+    }
+    h->precision = prm->precision;
+    h->flags = prm->flags;
+    if (hipGetDevice(&h->device) != hipSuccess) {
+        delete h;
+        return fail(FQ_ERR_HIP, "no HIP device");
+    }
+    const fq::Plan &P = h->P;
+    const int N = P.N;
+    const int64_t n = int64_t(N) * N * N;
+    int rc = h->precision == FQ_PREC_MIXED ? upload_phases<float>(h) : upload_phases<double>(h);
+    std::vector<int> isrc;
+    for (auto &it : P.items) isrc.push_back(it.src);
+    std::vector<DevProd> dp;
+    for (auto &pr : P.prods) dp.push_back({pr.i1, pr.i2, pr.coef});
+    std::vector<int> cg, tc0(P.ntgt, 0), tcn(P.ntgt, 0), ls, tl0(P.ntgt, 0), tln(P.ntgt, 0);
+    for (auto &c : P.contribs) { cg.push_back(c.group); tcn[c.tgt]++; }
+    for (int t = 1; t < P.ntgt; ++t) tc0[t] = tc0[t - 1] + tcn[t - 1];
+    for (auto &l : P.loss) { ls.push_back(l.src); tln[l.tgt]++; }
+    for (int t = 1; t < P.ntgt; ++t) tl0[t] = tl0[t - 1] + tln[t - 1];
+    auto chk = [&](int r) { if (rc == FQ_SUCCESS) rc = r; };
+    chk(upload(&h->item_src, isrc));
+    chk(upload(&h->prods, dp));
+    chk(upload(&h->g_p0, P.group_p0));
+    chk(upload(&h->g_np, P.group_np));
+    chk(upload(&h->c_group, cg));
+    chk(upload(&h->t_c0, tc0));
+    chk(upload(&h->t_cn, tcn));
+    chk(upload(&h->l_src, ls));
+    chk(upload(&h->t_l0, tl0));
+    chk(upload(&h->t_ln, tln));
+    if (rc == FQ_SUCCESS) rc = build_tables(h);
+    if (rc == FQ_SUCCESS) rc = build_projection(h);
+    // scratch: F (FP64, nsrc) + max(Z (items, T), T0 (nsrc, FP64), Y (2 ntgt, FP64)) + P (groups, T)
+    const size_t tsz = h->precision == FQ_PREC_MIXED ? sizeof(cpx<float>) : sizeof(cpx<double>);
+    const size_t per_dof_F = size_t(P.nsrc) * n * sizeof(cpx<double>);
+    const size_t per_dof_Z = std::max({P.items.size() * n * tsz, size_t(P.nsrc) * n * sizeof(cpx<double>),
+                                       size_t(2 * P.ntgt) * n * sizeof(cpx<double>)});
+    const size_t per_dof_P = size_t(P.ngroups) * n * tsz;
+    const size_t per_dof = per_dof_F + per_dof_Z + per_dof_P;
+    int64_t chunk = prm->max_chunk;
+    if (chunk <= 0) {
+        const size_t budget = size_t(8) << 30;   // 8 GiB default scratch budget
+        chunk = std::max<int64_t>(1, int64_t(budget / per_dof));
+        chunk = std::min<int64_t>(chunk, 1024);
+    }
+    h->chunk = chunk;
+    if (rc == FQ_SUCCESS) {
+        hipError_t e1 = hipMalloc(&h->F, per_dof_F * chunk);
+        hipError_t e2 = hipMalloc(&h->Z, per_dof_Z * chunk);
+        hipError_t e3 = hipMalloc(&h->Pg, per_dof_P * chunk);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+            rc = fail(FQ_ERR_ALLOC, "hipMalloc scratch failed (reduce max_chunk)");
+        h->scratch = per_dof * chunk;
+    }
+    if (rc != FQ_SUCCESS) {
+        std::string keep = g_err;
+        fq_destroy(h);
+        g_err = keep;
+        return rc;
+    }
+    *out = h;
+    return FQ_SUCCESS;
+}
+
+int fq_destroy(fq_handle *h) {
+    if (!h) return FQ_SUCCESS;
+    for (void *p : {h->ph, (void *)h->item_src, (void *)h->prods, (void *)h->g_p0, (void *)h->g_np,
+                    (void *)h->c_group, (void *)h->t_c0, (void *)h->t_cn, (void *)h->l_src, (void *)h->t_l0,
+                    (void *)h->t_ln, (void *)h->W, (void *)h->ginv, (void *)h->F, h->Z, h->Pg})
+        if (p) (void)hipFree(p);
+    delete h;
+    return FQ_SUCCESS;
+}
+
+int fq_get_plan_info(const fq_handle *h, fq_plan_info *info) {
+    if (!h || !info) return fail(FQ_ERR_INVALID, "null argument");
+    fq::plan_info(h->P, *info);
+    return FQ_SUCCESS;
+}
+
+int fq_scratch_bytes(const fq_handle *h, int64_t *bytes) {
+    if (!h || !bytes) return fail(FQ_ERR_INVALID, "null argument");
+    *bytes = int64_t(h->scratch);
+    return FQ_SUCCESS;
+}
+
+int fq_eval(fq_handle *h, const double *f, double *q, int32_t nupts, int64_t ldim, int32_t neles,
+            int64_t first_dof, int64_t ndof, void *stream) {
+    if (!h) return fail(FQ_ERR_INVALID, "null handle");
+    if (h->P.nsrc != 1 || h->P.ntgt != 1)
+        return fail(FQ_ERR_INVALID, "fq_eval needs a single-species model; use fq_eval_multi/fq_eval_bilinear");
+    if (!f || !q) return fail(FQ_ERR_INVALID, "null f or q");
+    AosoaPtrs ptr{};
+    ptr.f[0] = f;
+    ptr.q[0] = q;
+    return eval_impl(h, ptr, nupts, ldim, neles, first_dof, ndof, stream);
+}
+
+int fq_eval_multi(fq_handle *h, const double *const *f, double *const *q, int32_t nupts, int64_t ldim,
+                  int32_t neles, int64_t first_dof, int64_t ndof, void *stream) {
+    if (!h || !f || !q) return fail(FQ_ERR_INVALID, "null argument");
+    if (h->P.model != FQ_MODEL_VSS) return fail(FQ_ERR_INVALID, "fq_eval_multi needs a vss-gll model");
+    AosoaPtrs ptr{};
+    for (int s = 0; s < h->P.nspcs; ++s) {
+        if (!f[s] || !q[s]) return fail(FQ_ERR_INVALID, "null species pointer");
+        ptr.f[s] = f[s];
+        ptr.q[s] = q[s];
+    }
+    return eval_impl(h, ptr, nupts, ldim, neles, first_dof, ndof, stream);
+}
+
+int fq_eval_bilinear(fq_handle *h, const double *f, const double *g, double *q, int32_t nupts, int64_t ldim,
+                     int32_t neles, int64_t first_dof, int64_t ndof, void *stream) {
+    if (!h || !f || !g || !q) return fail(FQ_ERR_INVALID, "null argument");
+    if (h->P.model != FQ_MODEL_VHS_NODAL) return fail(FQ_ERR_INVALID, "fq_eval_bilinear needs vhs-gll-nodal");
+    AosoaPtrs ptr{};
+    ptr.f[0] = f;
+    ptr.f[1] = g;
+    ptr.q[0] = q;
+    return eval_impl(h, ptr, nupts, ldim, neles, first_dof, ndof, stream);
+}
+
+}  // extern "C"

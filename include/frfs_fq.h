@@ -1,0 +1,177 @@
+/*
+ * frfs_fq.h -- C ABI of the MI355X fast spectral collision operator Q(f,f).
+ *
+ * Drop-in boundary for the reference's scattering-model plugin call.  The
+ * reference evaluates Q one spatial DOF at a time from Python:
+ *
+ *   DGFSVHSGLLScatteringModel.fs(d_arr_in, d_arr_out, elem, upt)
+ *       frfs/solvers/dgfs/scattering.py:227-286      (vhs-gll)
+ *   DGFSVHSGLLNodalScatteringModel.fs(f, g, Q, elem, upt)
+ *       frfs/solvers/dgfs/scattering.py:327-414      (vhs-gll-nodal)
+ *   DGFSBiVSSGLLScatteringModel.fs(elem, upt, *[f_0..f_S-1, Q_0..Q_S-1])
+ *       frfs/solvers/dgfsbi/scattering.py:323-330    (vss-gll)
+ *
+ * each issuing ~8 PyCUDA kernel launches and 6-7 cuFFT executions
+ * (frfs/solvers/dgfs/cufft.py:231-281) on the legacy default stream.  This
+ * library replaces the Mako/PyCUDA kernels, cuFFT and the per-DOF loop of
+ * DGFSSystem.collide (frfs/solvers/dgfs/system.py:199-218) with batched HIP
+ * kernels: one fq_eval call evaluates Q for a contiguous range of DOFs.
+ *
+ * Data boundary: f and Q are the solver's AoSoA matrices of ioshape
+ * (nupts, N^3, neles) (frfs/backends/base/types.py:26-47), addressed as
+ *   upt*ldim + ((elem/soasz)*N^3 + v)*soasz + elem%soasz
+ * (SOA_IX, frfs/backends/cuda/kernels/base.mako:5-6), with DOF index
+ * dof = elem*nupts + upt (the reference's ndrange(neles, nupts) order).
+ *
+ * Errors: functions return FQ_SUCCESS (0) or a negative FQ_ERR_* code and
+ * set a thread-local message readable with fq_last_error(); the Python shim
+ * raises from it, as frfs/solvers/dgfs/cufft.py:48-117 does for cuFFT.
+ *
+ * Ownership: the caller owns f and Q device memory; the handle owns the
+ * precomputed tables and the scratch.  One handle per stream: calls on one
+ * handle must not overlap (they share its scratch).
+ */
+#ifndef FRFS_FQ_H
+#define FRFS_FQ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FQ_ABI_VERSION 1
+
+enum fq_status {
+    FQ_SUCCESS = 0,
+    FQ_ERR_INVALID = -1,      /* bad argument or parameter */
+    FQ_ERR_UNSUPPORTED = -2,  /* valid but not built (e.g. grid size) */
+    FQ_ERR_HIP = -3,          /* HIP runtime error */
+    FQ_ERR_ALLOC = -4         /* device allocation failed */
+};
+
+enum fq_model {
+    FQ_MODEL_VHS = 0,         /* vhs-gll: half-sphere, alpha = 1 */
+    FQ_MODEL_VHS_NODAL = 1,   /* vhs-gll-nodal: bilinear Q(f,g) */
+    FQ_MODEL_VSS = 2          /* vss-gll: full sphere, nspcs species */
+};
+
+enum fq_precision {
+    FQ_PREC_FP64 = 0,         /* every stage in FP64 (the reference's arithmetic) */
+    FQ_PREC_MIXED = 1         /* FP32 transforms/products, FP64 accumulation */
+};
+
+enum fq_flags {
+    FQ_FLAG_PROJECT = 1       /* project Q onto the complement of the collision
+                                 invariants (mass/momentum/energy), in FP64 */
+};
+
+/*
+ * Model parameters.  The host builds the velocity mesh exactly as the
+ * reference does (frfs/solvers/dgfs/velocitymesh.py:25-135,
+ * frfs/solvers/dgfsbi/velocitymesh.py:27-151) and passes the results.
+ */
+typedef struct fq_params {
+    int32_t abi_version;      /* FQ_ABI_VERSION */
+    int32_t model;            /* enum fq_model */
+    int32_t precision;        /* enum fq_precision */
+    int32_t flags;            /* enum fq_flags */
+    int32_t Nv;               /* velocity points per axis (even) */
+    int32_t Nrho;             /* radial Gauss points */
+    int32_t M;                /* spherical directions used (half sphere for VHS) */
+    int32_t nspcs;            /* species (1 for VHS) */
+    int32_t Mpre;             /* VSS pre-integration design points */
+    int32_t soasz;            /* AoSoA block size (32 in the reference) */
+    int64_t max_chunk;        /* DOFs per internal batch; 0 = library default */
+    double L;                 /* velocity box half-width */
+    double sw;                /* spherical weight: 2 pi/M (VHS) or 4 pi/M (VSS) */
+    double cw;                /* velocity cell volume (2L/N)^3, for the projection */
+    const double *qz;         /* [Nrho] radial nodes on [0, R] */
+    const double *qw;         /* [Nrho] radial weights */
+    const double *sz;         /* [M*3] unit directions, row major */
+    /* VHS / VHS_NODAL */
+    double gamma;             /* 2(1 - omega) */
+    double prefac;            /* collision prefactor */
+    /* VSS: pair index pq = p*nspcs + q */
+    const double *masses;     /* [nspcs] */
+    const double *eta;        /* [nspcs^2] alpha_pq - 1 */
+    const double *gammapq;    /* [nspcs^2] */
+    const double *prefacpq;   /* [nspcs^2] */
+    const double *szpre;      /* [Mpre*3] */
+    double swpre;             /* 4 pi/Mpre */
+} fq_params;
+
+typedef struct fq_handle fq_handle;
+typedef struct fq_plan fq_plan;
+
+/* Plan summary: work counts of one DOF-evaluation. */
+typedef struct fq_plan_info {
+    int32_t n_items;          /* distinct phase-shifted inverse 3D transforms */
+    int32_t n_products;       /* pointwise products E1*E2 */
+    int32_t n_groups;         /* forward 3D transforms (product sums) */
+    int32_t n_contribs;       /* weighted accumulations into Q-hat */
+    int32_t n_loss;           /* loss-term tables */
+    int32_t n_sources;        /* input distributions */
+    int32_t n_targets;        /* output distributions */
+    int32_t n_fft_ref;        /* 3D transforms per DOF in the reference */
+    int32_t n_fft;            /* 3D transforms per DOF here */
+} fq_plan_info;
+
+/* ---- lifecycle -------------------------------------------------------- */
+
+/* Replaces DGFSScatteringModel.__init__ -> load_parameters() ->
+ * perform_precomputation() (frfs/solvers/dgfs/scattering.py:17-28,67-224;
+ * frfs/solvers/dgfsbi/scattering.py:93-256): builds the transform plan,
+ * uploads the phase tables and computes the kernel weight tables on the
+ * current HIP device (precompute_aa/_bb, vhs-gll.mako:45-101;
+ * precompute_a/_bc, vss-gll.mako:31-95). */
+int fq_create(const fq_params *prm, fq_handle **out);
+int fq_destroy(fq_handle *h);
+const char *fq_last_error(void);
+int fq_get_plan_info(const fq_handle *h, fq_plan_info *info);
+/* Bytes of device scratch held by the handle. */
+int fq_scratch_bytes(const fq_handle *h, int64_t *bytes);
+
+/* ---- evaluation (device pointers; stream is a hipStream_t, NULL = default) */
+
+/* Q for DOFs [first_dof, first_dof+ndof): replaces vhs-gll fs
+ * (frfs/solvers/dgfs/scattering.py:227-286) and single-species vss-gll fs
+ * (frfs/solvers/dgfsbi/scattering.py:323-330), and the collide loop around
+ * them (frfs/solvers/dgfs/system.py:217-218). */
+int fq_eval(fq_handle *h, const double *f, double *q,
+            int32_t nupts, int64_t ldim, int32_t neles,
+            int64_t first_dof, int64_t ndof, void *stream);
+
+/* Multi-species vss-gll: f[s], q[s] for s < nspcs, all with the same AoSoA
+ * geometry (frfs/solvers/dgfsbi/scattering.py:259-330;
+ * frfs/solvers/dgfsbi/system.py:239-251). */
+int fq_eval_multi(fq_handle *h, const double *const *f, double *const *q,
+                  int32_t nupts, int64_t ldim, int32_t neles,
+                  int64_t first_dof, int64_t ndof, void *stream);
+
+/* Bilinear vhs-gll-nodal Q(f, g) (frfs/solvers/dgfs/scattering.py:327-414). */
+int fq_eval_bilinear(fq_handle *h, const double *f, const double *g, double *q,
+                     int32_t nupts, int64_t ldim, int32_t neles,
+                     int64_t first_dof, int64_t ndof, void *stream);
+
+/* ---- host-only plan inspection (no GPU needed) ------------------------- */
+
+int fq_plan_create(const fq_params *prm, fq_plan **out);
+int fq_plan_destroy(fq_plan *p);
+int fq_plan_get_info(const fq_plan *p, fq_plan_info *info);
+/* Export the plan arrays (any pointer may be NULL):
+ *   item_src[n_items], item_theta[n_items*3],
+ *   prod_i1/prod_i2[n_products], prod_coef[n_products], prod_group[n_products],
+ *   contrib_group/contrib_target[n_contribs],
+ *   loss_src/loss_target[n_loss]. */
+int fq_plan_export(const fq_plan *p, int32_t *item_src, double *item_theta,
+                   int32_t *prod_i1, int32_t *prod_i2, double *prod_coef,
+                   int32_t *prod_group, int32_t *contrib_group,
+                   int32_t *contrib_target, int32_t *loss_src,
+                   int32_t *loss_target);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FRFS_FQ_H */
