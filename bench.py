@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Benchmark: Q(f,f) DOF-evals/s on one MI355X, mixed (FP32 transforms, FP64
+accumulation) vs plain FP64, at the workload
+
+    vss-gll, 32^3 velocity grid on [-6,6]^3, M = 12 (full sphere),
+    Nrho = 16, omega = 0.74, alpha = 1.4, Mpre = 32, 4096 DOFs of seeded
+    two-Maxwellian f (single species).
+
+A step is one evaluation of Q for all DOFs (inputs resident in HBM).  Prints
+one JSON line.  ``--gpus N`` with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {'fp64': 78.6, 'fp32': 157.3}     # MI355X vector TFLOP/s (published)
+HBM_PEAK = 8000.0                          # GB/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--ndof', type=int, default=4096)
+    ap.add_argument('--Nv', type=int, default=32)
+    ap.add_argument('--Nrho', type=int, default=16)
+    ap.add_argument('--M', type=int, default=12)
+    ap.add_argument('--Mpre', type=int, default=32)
+    ap.add_argument('--seed', type=int, default=2024)
+    ap.add_argument('--cpu-dofs-per-proc', type=int, default=2)
+    ap.add_argument('--cpu-procs', type=int, default=0, help='0 = min(cpus, 16)')
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--modes', default='fp64,mixed')
+    return ap.parse_args()
+
+
+def vss_cfg(a, precision):
+    from frfs_amd.inifile import Inifile
+    return Inifile.from_dict({
+        'non-dim': dict(T0=273.0, H0=1e-3, rho0=0.00011140244750324529, molarMass0=39.96e-3),
+        'constants': {'Nv': a.Nv, 'NvBatchSize': a.Nv ** 3, 'Nrho': a.Nrho},
+        'velocity-mesh': {'dev': 6.0, 'cmax': 0, 'Tmax': 273, 'molarMass1': 39.96e-3},
+        'spherical-design-rule': {'ssrule': 'womersley', 'M': a.M},
+        'scattering-model': {'type': 'vss-gll', 'ssrulepre': 'womersley', 'Mpre': a.Mpre,
+                             'alpha00': 1.4, 'omega00': 0.74, 'dRef00': 4.17e-10,
+                             'Tref00': 273, 'precision': precision},
+    })
+
+
+# --------------------------------------------------------------------------
+# CPU baseline: the oracle (NumPy complex128) on the host cores, run before
+# the GPU is initialised (forked workers, no exec).
+# --------------------------------------------------------------------------
+_ORC = {}
+
+
+def _cpu_worker(dofs):
+    from oracle import fastspectral as fsp
+    vm, prm, tab, fpar = _ORC['vm'], _ORC['prm'], _ORC['tab'], _ORC['fpar']
+    out = []
+    t0 = time.perf_counter()
+    for d in dofs:
+        f = np.zeros(vm['cv'].shape[1])
+        for k in range(2):
+            nk, ux, uy, uz, Tk = fpar[d, k]
+            f += fsp.maxwellian(vm['cv'], nk, [ux, uy, uz], Tk)
+        out.append(fsp.vss_fs([f], vm, tab, prm)[0])
+    return dofs, out, time.perf_counter() - t0
+
+
+def cpu_baseline(a, fpar):
+    from multiprocessing import get_context
+    from oracle import fastspectral as fsp
+    os.environ.setdefault('OMP_NUM_THREADS', '1')
+    vm = fsp.velocity_mesh(a.Nv, a.Nrho, a.M, dev=6.0, masses=[39.96e-3])
+    prm = fsp.vss_parameters(vm, {'00': 1.4}, {'00': 0.74}, {'00': 4.17e-10}, {'00': 273.0},
+                             'womersley', a.Mpre)
+    t0 = time.perf_counter()
+    tab = fsp.vss_tables(vm, prm)
+    t_tab = time.perf_counter() - t0
+    _ORC.update(vm=vm, prm=prm, tab=tab, fpar=fpar)
+    ncpu = os.cpu_count() or 1
+    procs = a.cpu_procs or min(ncpu, 16)
+    per = a.cpu_dofs_per_proc
+    total = procs * per
+    rng = np.random.default_rng(7)
+    sample = np.sort(rng.choice(a.ndof, size=min(total, a.ndof), replace=False))
+    parts = [sample[i::procs].tolist() for i in range(procs)]
+    ctx = get_context('fork')
+    # warm-up (one DOF, single process): first-touch of FFT plans
+    _cpu_worker([int(sample[0])])
+    t0 = time.perf_counter()
+    with ctx.Pool(procs) as pool:
+        res = pool.map(_cpu_worker, parts)
+    wall = time.perf_counter() - t0
+    ref = {}
+    cpu_secs = 0.0
+    for dofs, qs, secs in res:
+        cpu_secs += secs
+        for d, q in zip(dofs, qs):
+            ref[d] = q
+    return dict(value=len(sample) / wall, unit='DOF-evals/s', cores=procs, kind='port',
+                sample=f'{len(sample)} of {a.ndof} DOFs of the same workload, NumPy complex128 '
+                       f'oracle, {procs} processes x {per} DOFs, 1 thread each, host has {ncpu} CPUs; '
+                       f'tables precomputed once ({t_tab:.1f} s, untimed)',
+                single_core_dof_evals_per_s=len(sample) / cpu_secs,
+                wall_s=wall), ref
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({'error': 'this build runs on one GPU; --gpus must be 1',
+                          'n_gpus_requested': a.gpus}))
+        sys.exit(2)
+    from frfs_amd.inputs import mixture_params
+    fpar = mixture_params(a.ndof, a.seed)
+    cpu, ref = (None, {}) if a.no_cpu_baseline else cpu_baseline(a, fpar)
+
+    import torch
+    from frfs_amd.backend import HIPBackend
+    from frfs_amd.inputs import from_aosoa, mixture_device, to_aosoa
+    from frfs_amd.scattering import DGFSBiVSSGLLScatteringModel
+    from frfs_amd.velocitymesh import DGFSBiVelocityMesh
+
+    be = HIPBackend()
+    modes = a.modes.split(',')
+    res = {}
+    qs = {}
+    for mode in modes:
+        cfg = vss_cfg(a, 'mixed' if mode == 'mixed' else 'double')
+        vm = DGFSBiVelocityMesh(be, cfg, 1)
+        sm = DGFSBiVSSGLLScatteringModel(be, cfg, vm)
+        n = vm.vsize()
+        fin = be.matrix((1, n, a.ndof))
+        qout = be.matrix((1, n, a.ndof))
+        to_aosoa(mixture_device(vm.cv(), fpar), fin)
+        torch.cuda.synchronize()
+        for _ in range(a.warmup):
+            sm.fs_batch(fin, qout)
+        torch.cuda.synchronize()
+        sm._h.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            sm.fs_batch(fin, qout)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / a.steps
+        prof = sm._h.profile_read()
+        sm._h.profile(False)
+        info = sm.plan_info()
+        res[mode] = dict(ms=dt * 1e3, rate=a.ndof / dt, prof=prof, info=info,
+                         chunk=sm._h.chunk_dofs(), scratch=sm._h.scratch_bytes())
+        qs[mode] = from_aosoa(qout).cpu().numpy()
+        del sm, fin, qout
+        torch.cuda.empty_cache()
+
+    N = a.Nv
+    n = N ** 3
+    info = res[modes[0]]['info']
+    f_alg = info['n_fft_ref'] * 5 * n * math.log2(n)          # nominal flops per DOF-eval
+    head = 'mixed' if 'mixed' in res else modes[0]
+    r = res[head]
+    prec_peak = PEAK['fp32'] if head == 'mixed' else PEAK['fp64']
+    # dominant kernel: performed 1D FFT work per launch (5 N log2 N per length-N FFT)
+    tot = {k: v for k, v in r['prof'].items() if v[1] > 0}
+    dom = max(tot, key=lambda k: tot[k][0])
+    dom_ms, dom_n = tot[dom]
+    fft1d = 5 * N * math.log2(N)
+    pencils = N * N
+    steps_launch = a.steps
+    per_dof_1d = {
+        'k_fwd_z': info['n_sources'] * pencils,
+        'k_fwd_xy': info['n_sources'] * 2 * pencils,
+        'k_inv_z': info['n_items'] * pencils,
+        'k_plane': (info['n_items'] + info['n_groups']) * 2 * pencils,
+        'k_accum': (info['n_contribs'] + 2 * info['n_targets']) * pencils,
+        'k_out': info['n_targets'] * 2 * 2 * pencils,
+    }
+    dom_flops_per_launch = per_dof_1d.get(dom, 0) * fft1d * a.ndof * steps_launch / dom_n
+    dom_avg_s = dom_ms / dom_n * 1e-3
+    achieved = dom_flops_per_launch / dom_avg_s / 1e12
+    op_eff = r['rate'] * f_alg / 1e12
+    err = None
+    if 'mixed' in qs and 'fp64' in qs:
+        d = qs['mixed'] - qs['fp64']
+        err = dict(rel_l2=float(np.linalg.norm(d) / np.linalg.norm(qs['fp64'])),
+                   max_dof_rel_l2=float(np.max(np.linalg.norm(d, axis=1) / np.linalg.norm(qs['fp64'], axis=1))))
+    oracle_err = None
+    if ref and 'fp64' in qs:
+        ds = sorted(ref)
+        oracle_err = float(max(np.linalg.norm(qs['fp64'][d] - ref[d]) / np.linalg.norm(ref[d]) for d in ds))
+    out = {
+        'metric': 'Q(f,f) DOF-evals/s, 32^3 grid, mixed/surrogate vs plain FP64 at stated L2 err',
+        'value': r['rate'], 'unit': 'DOF-evals/s', 'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup,
+        'ms_per_step': r['ms'], 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32+f64acc' if head == 'mixed' else 'f64', 'data': 'synthetic',
+        'config': {'workload': f'vss-gll Q(f,f), {N}^3 grid on [-6,6]^3, M={a.M} full sphere, '
+                               f'Nrho={a.Nrho}, omega=0.74, alpha=1.4, Mpre={a.Mpre}, single species, '
+                               f'{a.ndof} DOFs of seeded two-Maxwellian f',
+                   'mode': head, 'ndof': a.ndof, 'Nv': N, 'M': a.M, 'Nrho': a.Nrho, 'Mpre': a.Mpre,
+                   'seed': a.seed, 'chunk_dofs': r['chunk'], 'scratch_GiB': r['scratch'] / 2 ** 30},
+        'fp64': {'value': res['fp64']['rate'], 'ms_per_step': res['fp64']['ms']} if 'fp64' in res else None,
+        'speedup_mixed_vs_fp64': (res['mixed']['rate'] / res['fp64']['rate']) if len(res) == 2 else None,
+        'mixed_vs_fp64_error': err,
+        'fp64_vs_oracle_max_rel_l2': oracle_err,
+        'plan': info,
+        'roofline': {'bound': 'valu', 'kernel': dom, 'achieved': achieved, 'peak': prec_peak,
+                     'unit': 'TFLOP/s', 'frac': achieved / prec_peak, 'traffic': None,
+                     'avg_launch_ms': dom_avg_s * 1e3, 'launches': dom_n,
+                     'operator_effective_tflops': op_eff, 'operator_frac': op_eff / prec_peak,
+                     'kernel_ms_per_step': {k: v[0] / a.steps for k, v in tot.items()}},
+        'cpu_baseline': cpu,
+    }
+    if 'fp64' in res and head != 'fp64':
+        rp = {k: v for k, v in res['fp64']['prof'].items() if v[1] > 0}
+        out['fp64']['kernel_ms_per_step'] = {k: v[0] / a.steps for k, v in rp.items()}
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

@@ -17,9 +17,11 @@ LIBPATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libfrfs_fq.s
 ABI_VERSION = 1
 
 SUCCESS, ERR_INVALID, ERR_UNSUPPORTED, ERR_HIP, ERR_ALLOC = 0, -1, -2, -3, -4
+NKERNELS = 9
 MODEL_VHS, MODEL_VHS_NODAL, MODEL_VSS = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 FLAG_PROJECT = 1
+FLAG_UNFUSED = 2
 
 _dp = C.POINTER(C.c_double)
 
@@ -64,6 +66,10 @@ SIGNATURES = {
     'fq_eval': (C.c_int, [_vp, _vp, _vp, _i32, _i64, _i32, _i64, _i64, _vp]),
     'fq_eval_multi': (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), _i32, _i64, _i32, _i64, _i64, _vp]),
     'fq_eval_bilinear': (C.c_int, [_vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64, _i64, _vp]),
+    'fq_profile_enable': (C.c_int, [_vp, C.c_int]),
+    'fq_profile_read': (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(_i64)]),
+    'fq_kernel_name': (C.c_char_p, [C.c_int]),
+    'fq_chunk_dofs': (C.c_int, [_vp, C.POINTER(_i64)]),
     'fq_plan_create': (C.c_int, [C.POINTER(fq_params), C.POINTER(_vp)]),
     'fq_plan_destroy': (C.c_int, [_vp]),
     'fq_plan_get_info': (C.c_int, [_vp, C.POINTER(fq_plan_info)]),
@@ -189,6 +195,22 @@ class Handle:
         b = C.c_int64()
         check(load().fq_scratch_bytes(self._h, C.byref(b)))
         return b.value
+
+    def chunk_dofs(self):
+        b = C.c_int64()
+        check(load().fq_chunk_dofs(self._h, C.byref(b)))
+        return b.value
+
+    def profile(self, on=True):
+        check(load().fq_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self):
+        """{kernel name: (total ms, launches)} since the last read."""
+        ms = (C.c_double * NKERNELS)()
+        n = (C.c_int64 * NKERNELS)()
+        check(load().fq_profile_read(self._h, ms, n))
+        lib = load()
+        return {lib.fq_kernel_name(k).decode(): (ms[k], n[k]) for k in range(NKERNELS)}
 
     def eval(self, f_ptr, q_ptr, nupts, ldim, neles, first_dof, ndof, stream=0):
         check(load().fq_eval(self._h, f_ptr, q_ptr, nupts, ldim, neles, first_dof, ndof, stream))

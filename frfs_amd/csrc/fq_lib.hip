@@ -376,6 +376,171 @@ __global__ __launch_bounds__(kPlaneThreads) void k_plane(const cpx<T> *Z, cpx<T>
 }
 
 // ---------------------------------------------------------------------------
+// k_prod: fused z-stage + plane stage.  One workgroup per (slot, group, G),
+// G in [0,4): it owns the S = N/4 output planes vz = G + 4k.  For every
+// product of the group the pruned z-DFT
+//   Z_i(I,J,G+4k) = sum_a w_S^{ak} w_N^{aG} sum_b w_4^{bG} (F . phi_z)(a + S b)
+// (w = e^{+2 pi i/.}) is computed from the L2-resident source volume
+// straight into LDS, then wave k runs the x,y transforms of plane k, the
+// product and the group sum; the group's P planes are written once.  This
+// removes the round trip of every Z_i volume through HBM.  SEQ = true
+// processes the two items of a product one after the other (half the LDS,
+// used for FP64).
+// ---------------------------------------------------------------------------
+template <int N> struct ProdGeo {
+    static constexpr int S = N / 4;
+    static constexpr int E = N / 2;
+    static constexpr int LPP = Geo<N>::LPP;
+    static constexpr int THREADS = S * LPP;
+    static constexpr int ROW = N + 1;
+    static constexpr int PLANE = N * ROW;
+};
+
+// Pruned z-DFT of one item for the planes vz = G + 4k.  With K = a + S b
+// (a < S, b < 4, N = 4S) the phase factorises, e^{i th l0(K)} = alpha_a beta_b
+// (alpha_a = e^{i th a}, beta_b = e^{i th S (b - 4[b >= 2])}), so
+//   Z(G + 4k) = FFT_S[ c_a sum_b d_b F(a + S b) ](k),
+//   d_b = w_4^{bG} beta_b,  c_a = w_N^{aG} alpha_a   (uniform per item and G),
+// and the x,y phases phi_x(I) phi_y(J) scale the S outputs of pencil (I,J).
+// zc[(item*4 + G)*(4 + S) + .] holds {d_0..d_3, c_0..c_{S-1}}.
+template <int N, typename T>
+__device__ __forceinline__ void zstage(const cpx<T> *__restrict__ src, const cpx<T> *__restrict__ zc,
+                                       const cpx<T> *__restrict__ ph, cpx<T> *dst) {
+    using PG = ProdGeo<N>;
+    constexpr int S = PG::S;
+    cpx<T> d[4], cc[S];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) d[b] = zc[b];
+#pragma unroll
+    for (int a = 0; a < S; ++a) cc[a] = zc[4 + a];
+    for (int p = threadIdx.x; p < N * N; p += PG::THREADS) {
+        const uint32_t I = uint32_t(p) / N, J = uint32_t(p) % N;
+        const uint32_t off = uint32_t(p);
+        cpx<T> t[S];
+#pragma unroll
+        for (int a = 0; a < S; ++a) t[a] = {0, 0};
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            cpx<T> x[S];
+#pragma unroll
+            for (int a = 0; a < S; ++a) x[a] = src[uint32_t((a + S * b) * N * N) + off];
+#pragma unroll
+            for (int a = 0; a < S; ++a) {
+                const cpx<T> y = fq::cmul(x[a], d[b]);
+                t[a] = t[a] + y;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < S; ++a) t[a] = fq::cmul(t[a], cc[a]);
+        fq::LFFT<S, 1>::run(t);
+        const cpx<T> pxy = fq::cmul(ph[I], ph[N + J]);
+        cpx<T> *dd = dst + I * PG::ROW + J;
+#pragma unroll
+        for (int k = 0; k < S; ++k) dd[k * PG::PLANE] = fq::cmul(t[k], pxy);
+    }
+}
+
+// E = DFT+_{x,y}(Z) of the LDS plane Lp (row stride ROW, phases already
+// applied); the plane region is reused as the transpose buffer.
+// Out: row vx = c, cyclic vy.
+template <int N, typename T>
+__device__ __forceinline__ void plane_inv_lds(cpx<T> *Lp, cpx<T> *v, int c, int half, bool act) {
+    using PG = ProdGeo<N>;
+    constexpr int E = PG::E;
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = act ? Lp[(half * E + e) * PG::ROW + c] : cpx<T>{0, 0};
+    fq::fft2_dif<N, 1>(v, half);
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) Lp[(half + 2 * e) * PG::ROW + c] = v[e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = act ? Lp[c * PG::ROW + half * E + e] : cpx<T>{0, 0};
+    fq::fft2_dif<N, 1>(v, half);
+}
+
+template <int N, typename T, bool SEQ, bool MULTI>
+__global__ __launch_bounds__(ProdGeo<N>::THREADS, (sizeof(T) == 4 ? 4 : 2)) void k_prod(const cpx<T> *FT, cpx<T> *P, const cpx<T> *ph,
+                                                              const cpx<T> *zc, const DevProd *prods,
+                                                              const int *g_p0, const int *g_np, const int *item_src,
+                                                              int ngroups, int nsrc) {
+    using PG = ProdGeo<N>;
+    using G3 = Geo<N>;
+    constexpr int E = PG::E;
+    constexpr int ZC = 4 + PG::S;
+    __shared__ cpx<T> lds[(SEQ ? 1 : 2) * PG::S * PG::PLANE];
+    const int64_t unit = blockIdx.x;
+    const int G = int(unit % 4);
+    const int g = int((unit / 4) % ngroups);
+    const int slot = int(unit / (4 * int64_t(ngroups)));
+    const int k = threadIdx.x / PG::LPP;
+    const int lt = threadIdx.x % PG::LPP;
+    const int c = lt >> 1, half = lt & 1;
+    const bool act = c < N;
+    const int vz = G + 4 * k;
+    cpx<T> *L1 = lds + k * PG::PLANE;
+    cpx<T> *L2 = SEQ ? L1 : lds + (PG::S + k) * PG::PLANE;
+    cpx<T> acc[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) acc[e] = {0, 0};
+    const int p0 = g_p0[g], np = g_np[g];
+    for (int pi = p0; pi < p0 + np; ++pi) {
+        const DevProd pr = prods[pi];
+        const cpx<T> *src1 = FT + (int64_t(slot) * nsrc + item_src[pr.i1]) * G3::VOL;
+        const cpx<T> *src2 = FT + (int64_t(slot) * nsrc + item_src[pr.i2]) * G3::VOL;
+        const cpx<T> *zc1 = zc + (int64_t(pr.i1) * 4 + G) * ZC, *zc2 = zc + (int64_t(pr.i2) * 4 + G) * ZC;
+        const cpx<T> *ph1 = ph + int64_t(pr.i1) * 3 * N, *ph2 = ph + int64_t(pr.i2) * 3 * N;
+        cpx<T> e1[E], e2[E];
+        __syncthreads();                                   // LDS free
+        zstage<N, T>(src1, zc1, ph1, lds);
+        if constexpr (!SEQ) zstage<N, T>(src2, zc2, ph2, lds + PG::S * PG::PLANE);
+        __syncthreads();
+        plane_inv_lds<N, T>(L1, e1, c, half, act);
+        if constexpr (SEQ) {
+            __syncthreads();
+            zstage<N, T>(src2, zc2, ph2, lds);
+            __syncthreads();
+        }
+        plane_inv_lds<N, T>(L2, e2, c, half, act);
+        const T cf = T(pr.coef);
+        if constexpr (MULTI) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                cpx<T> pe = fq::cmul(e1[e], e2[e]);
+                acc[e] = {acc[e].x + cf * pe.x, acc[e].y + cf * pe.y};
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) acc[e] = fq::cscale(fq::cmul(e1[e], e2[e]), cf);
+        }
+    }
+    // forward transforms of the group sum: acc is row vx = c, cyclic over vy
+    fq::fft2_dit<N, -1>(acc, half);                       // -> J' = half*E + e
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) L1[c * PG::ROW + half * E + e] = acc[e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) acc[e] = act ? L1[(half + 2 * e) * PG::ROW + c] : cpx<T>{0, 0};
+    fq::fft2_dit<N, -1>(acc, half);                       // -> I' = half*E + e
+    cpx<T> *dst = P + (int64_t(slot) * ngroups + g) * G3::VOL + int64_t(vz) * G3::N2;
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) dst[(half * E + e) * N + c] = acc[e];
+    }
+}
+
+// FP64 -> FP32 copy of the source spectra (mixed precision).
+__global__ __launch_bounds__(256) void k_cvt(const cpx<double> *F, cpx<float> *F32, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) F32[i] = cvt<float>(F[i]);
+}
+
+// ---------------------------------------------------------------------------
 // k_accum: unit = (slot, target, I').  H = sum_c W_c . DFT-_z(P_g(c)),
 // L = sum_j Lt_j . F_src(j); Y = DFT+_z of H and L.  FP64 accumulation.
 // ---------------------------------------------------------------------------
@@ -384,17 +549,18 @@ __global__ __launch_bounds__(256) void k_accum(const cpx<T> *P, const cpx<double
                                                const cpx<double> *W, const int *c_group,
                                                const int *t_c0, const int *t_cn, const int *l_src,
                                                const int *t_l0, const int *t_ln, int ncontribs,
-                                               cpx<double> *Y, int ngroups, int nsrc, int ntgt,
+                                               cpx<double> *Y, int ngroups, int nsrc, int ntgt, int D,
                                                int64_t nunits) {
     using G = Geo<N>;
     constexpr int E = G::E;
     const int lt = threadIdx.x % G::LPP;
     const int64_t unit = int64_t(blockIdx.x) * (blockDim.x / G::LPP) + threadIdx.x / G::LPP;
     if (unit >= nunits) return;
-    const int I = int(unit % N);
-    const int64_t dt = unit / N;
-    const int t = int(dt % ntgt);
-    const int slot = int(dt / ntgt);
+    // slot fastest: neighbouring units read the same weight rows W[c][.][I]
+    const int slot = int(unit % D);
+    const int64_t ti = unit / D;
+    const int t = int(ti % ntgt);
+    const int I = int(ti / ntgt);
     const int J = lt >> 1, half = lt & 1;
     const bool act = J < N;
     const int64_t off = int64_t(I) * N + J;
@@ -594,12 +760,22 @@ struct fq_handle {
     int *l_src = nullptr, *t_l0 = nullptr, *t_ln = nullptr;
     cpx<double> *W = nullptr;    // [contribs + loss][N^3]
     double *ginv = nullptr;      // [25]
+    void *zc = nullptr;          // [items][4][4 + N/4] k_prod z-stage coefficients in T
+    bool fused = true;           // k_prod path (else k_inv_z + k_plane)
+    cpx<float> *F32 = nullptr;   // FP32 copy of F (mixed, fused)
     // scratch
     cpx<double> *F = nullptr;
     void *Z = nullptr;           // also T0 and Y
     void *Pg = nullptr;
     size_t scratch = 0;
+    // optional per-kernel HIP-event timing
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_rec;
 };
+
+static const char *kKernelNames[FQ_NKERNELS] = {"k_fwd_z", "k_fwd_xy", "k_inv_z", "k_plane",
+                                                "k_accum", "k_out", "k_project", "k_prod", "k_cvt"};
 
 namespace {
 
@@ -617,6 +793,29 @@ template <typename T> int upload_phases(fq_handle *h) {
     cpx<T> *d = nullptr;
     int rc = upload(&d, ph);
     h->ph = d;
+    // z-stage coefficients of k_prod: for item i and vz-group G (N = 4S),
+    // d_b = e^{2 pi i bG/4} e^{i th S b'} (b' = b, or b - 4 for b >= 2) and
+    // c_a = e^{2 pi i aG/N} e^{i th a}
+    const int S = N / 4, ZC = 4 + S;
+    std::vector<cpx<T>> zc(P.items.size() * 4 * ZC);
+    for (size_t i = 0; i < P.items.size(); ++i) {
+        const double th = P.items[i].th[2];
+        for (int G = 0; G < 4; ++G) {
+            cpx<T> *z = zc.data() + (i * 4 + G) * ZC;
+            for (int b = 0; b < 4; ++b) {
+                const double bp = b < 2 ? b : b - 4;
+                const double ang = 2 * M_PI * ((b * G) % 4) / 4.0 + th * S * bp;
+                z[b] = {T(std::cos(ang)), T(std::sin(ang))};
+            }
+            for (int a = 0; a < S; ++a) {
+                const double ang = 2 * M_PI * double((a * G) % N) / N + th * a;
+                z[4 + a] = {T(std::cos(ang)), T(std::sin(ang))};
+            }
+        }
+    }
+    cpx<T> *dz = nullptr;
+    if (rc == FQ_SUCCESS) rc = upload(&dz, zc);
+    h->zc = dz;
     return rc;
 }
 
@@ -716,30 +915,82 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     cpx<T> *Pg = reinterpret_cast<cpx<T> *>(h->Pg);
     cpx<double> *Y = reinterpret_cast<cpx<double> *>(h->Z);
     const cpx<T> *ph = reinterpret_cast<const cpx<T> *>(h->ph);
+    // HIP-event brackets around each launch when profiling is on
+    auto evget = [&]() {
+        hipEvent_t e;
+        if (!h->ev_pool.empty()) { e = h->ev_pool.back(); h->ev_pool.pop_back(); }
+        else (void)hipEventCreate(&e);
+        return e;
+    };
+    std::pair<hipEvent_t, hipEvent_t> cur{};
+    auto pre = [&]() {
+        if (h->prof) { cur = {evget(), evget()}; (void)hipEventRecord(cur.first, st); }
+    };
+    auto post = [&](int k) {
+        if (h->prof) { (void)hipEventRecord(cur.second, st); h->ev_rec.push_back({k, cur}); }
+    };
     auto blocks = [&](int64_t units) { return unsigned((units + upb - 1) / upb); };
     auto pblocks = [&](int64_t units) { return unsigned((units + upp - 1) / upp); };
     const dim3 bt(kPencilThreads), pt(kPlaneThreads);
     int64_t u;
     u = int64_t(D) * nsrc * N;
+    pre();
     hipLaunchKernelGGL(k_fwd_z<N>, dim3(blocks(u)), bt, 0, st, ptr, dof, T0, nsrc, u);
+    post(0);
+    pre();
     hipLaunchKernelGGL(k_fwd_xy<N>, dim3(pblocks(u)), pt, 0, st, T0, h->F, u);
+    post(1);
+    if (h->fused) {
+        const cpx<T> *FT = reinterpret_cast<const cpx<T> *>(h->F);
+        if constexpr (std::is_same<T, float>::value) {
+            const int64_t ne = int64_t(D) * nsrc * G::VOL;
+            pre();
+            hipLaunchKernelGGL(k_cvt, dim3(unsigned((ne + 255) / 256)), dim3(256), 0, st, h->F, h->F32, ne);
+            post(8);
+            FT = reinterpret_cast<const cpx<T> *>(h->F32);
+        }
+        pre();
+        bool multi = false;
+        for (int g = 0; g < ngroups; ++g) multi |= P.group_np[g] > 1;
+        const cpx<T> *zc = reinterpret_cast<const cpx<T> *>(h->zc);
+        const dim3 grid(unsigned(int64_t(D) * ngroups * 4)), blk(ProdGeo<N>::THREADS);
+        if (multi)
+            hipLaunchKernelGGL((k_prod<N, T, true, true>), grid, blk, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0,
+                               h->g_np, h->item_src, ngroups, nsrc);
+        else
+            hipLaunchKernelGGL((k_prod<N, T, true, false>), grid, blk, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0,
+                               h->g_np, h->item_src, ngroups, nsrc);
+        post(7);
+    } else {
     u = int64_t(D) * nitems * N;
+    pre();
     hipLaunchKernelGGL((k_inv_z<N, T>), dim3(blocks(u)), bt, 0, st, h->F, Z, ph, h->item_src, nitems, nsrc, u);
+    post(2);
     u = int64_t(D) * ngroups * N;
     bool multi = false;
     for (int g = 0; g < ngroups; ++g) multi |= P.group_np[g] > 1;
+    pre();
     if (multi)
         hipLaunchKernelGGL((k_plane<N, T, true>), dim3(pblocks(u)), pt, 0, st, Z, Pg, ph, h->prods, h->g_p0,
                            h->g_np, nitems, ngroups, u);
     else
         hipLaunchKernelGGL((k_plane<N, T, false>), dim3(pblocks(u)), pt, 0, st, Z, Pg, ph, h->prods, h->g_p0,
                            h->g_np, nitems, ngroups, u);
+    post(3);
+    }
     u = int64_t(D) * ntgt * N;
+    pre();
     hipLaunchKernelGGL((k_accum<N, T>), dim3(blocks(u)), bt, 0, st, Pg, h->F, h->W, h->c_group, h->t_c0,
-                       h->t_cn, h->l_src, h->t_l0, h->t_ln, int(P.contribs.size()), Y, ngroups, nsrc, ntgt, u);
+                       h->t_cn, h->l_src, h->t_l0, h->t_ln, int(P.contribs.size()), Y, ngroups, nsrc, ntgt, D, u);
+    post(4);
+    pre();
     hipLaunchKernelGGL(k_out<N>, dim3(pblocks(u)), pt, 0, st, Y, ptr, dof, ntgt, u);
-    if (h->flags & FQ_FLAG_PROJECT)
+    post(5);
+    if (h->flags & FQ_FLAG_PROJECT) {
+        pre();
         hipLaunchKernelGGL(k_project, dim3(unsigned(D * ntgt)), dim3(256), 0, st, ptr, dof, ntgt, N, P.L, h->ginv);
+        post(6);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(FQ_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return FQ_SUCCESS;
@@ -852,6 +1103,7 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     }
     h->precision = prm->precision;
     h->flags = prm->flags;
+    h->fused = !(prm->flags & FQ_FLAG_UNFUSED);
     if (hipGetDevice(&h->device) != hipSuccess) {
         delete h;
         return fail(FQ_ERR_HIP, "no HIP device");
@@ -891,16 +1143,19 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     const size_t per_dof = per_dof_F + per_dof_Z + per_dof_P;
     int64_t chunk = prm->max_chunk;
     if (chunk <= 0) {
-        const size_t budget = size_t(8) << 30;   // 8 GiB default scratch budget
+        const size_t budget = size_t(32) << 30;   // 32 GiB default scratch budget (of 288 GB HBM)
         chunk = std::max<int64_t>(1, int64_t(budget / per_dof));
-        chunk = std::min<int64_t>(chunk, 1024);
+        chunk = std::min<int64_t>(chunk, 4096);
     }
     h->chunk = chunk;
     if (rc == FQ_SUCCESS) {
         hipError_t e1 = hipMalloc(&h->F, per_dof_F * chunk);
         hipError_t e2 = hipMalloc(&h->Z, per_dof_Z * chunk);
         hipError_t e3 = hipMalloc(&h->Pg, per_dof_P * chunk);
-        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+        hipError_t e4 = hipSuccess;
+        if (h->precision == FQ_PREC_MIXED)
+            e4 = hipMalloc(&h->F32, size_t(P.nsrc) * n * sizeof(cpx<float>) * chunk);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess)
             rc = fail(FQ_ERR_ALLOC, "hipMalloc scratch failed (reduce max_chunk)");
         h->scratch = per_dof * chunk;
     }
@@ -914,11 +1169,46 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     return FQ_SUCCESS;
 }
 
+int fq_profile_enable(fq_handle *h, int on) {
+    if (!h) return fail(FQ_ERR_INVALID, "null handle");
+    for (auto &r : h->ev_rec) { h->ev_pool.push_back(r.second.first); h->ev_pool.push_back(r.second.second); }
+    h->ev_rec.clear();
+    h->prof = on != 0;
+    return FQ_SUCCESS;
+}
+
+int fq_profile_read(fq_handle *h, double *ms, int64_t *launches) {
+    if (!h || !ms || !launches) return fail(FQ_ERR_INVALID, "null argument");
+    for (int k = 0; k < FQ_NKERNELS; ++k) { ms[k] = 0; launches[k] = 0; }
+    for (auto &r : h->ev_rec) {
+        HIPCHK(hipEventSynchronize(r.second.second));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, r.second.first, r.second.second));
+        ms[r.first] += t;
+        launches[r.first] += 1;
+        h->ev_pool.push_back(r.second.first);
+        h->ev_pool.push_back(r.second.second);
+    }
+    h->ev_rec.clear();
+    return FQ_SUCCESS;
+}
+
+const char *fq_kernel_name(int k) { return (k >= 0 && k < FQ_NKERNELS) ? kKernelNames[k] : ""; }
+
+int fq_chunk_dofs(const fq_handle *h, int64_t *chunk) {
+    if (!h || !chunk) return fail(FQ_ERR_INVALID, "null argument");
+    *chunk = h->chunk;
+    return FQ_SUCCESS;
+}
+
 int fq_destroy(fq_handle *h) {
     if (!h) return FQ_SUCCESS;
+    for (auto &r : h->ev_rec) { (void)hipEventDestroy(r.second.first); (void)hipEventDestroy(r.second.second); }
+    for (auto e : h->ev_pool) (void)hipEventDestroy(e);
     for (void *p : {h->ph, (void *)h->item_src, (void *)h->prods, (void *)h->g_p0, (void *)h->g_np,
                     (void *)h->c_group, (void *)h->t_c0, (void *)h->t_cn, (void *)h->l_src, (void *)h->t_l0,
-                    (void *)h->t_ln, (void *)h->W, (void *)h->ginv, (void *)h->F, h->Z, h->Pg})
+                    (void *)h->t_ln, (void *)h->W, (void *)h->ginv, (void *)h->F, h->Z, h->Pg, h->zc,
+                    (void *)h->F32})
         if (p) (void)hipFree(p);
     delete h;
     return FQ_SUCCESS;

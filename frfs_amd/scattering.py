@@ -14,7 +14,8 @@ DGFSSystem.collide, frfs/solvers/dgfs/system.py:217-218).  All work runs in
 Extra configuration keys (absent from the reference, whose operator is
 always FP64): ``[scattering-model] precision = double | mixed`` and
 ``conservation-projection = 0 | 1``; ``max-chunk`` caps the DOFs per
-internal batch.
+internal batch; ``fused = 0`` selects the three-kernel path that stages the
+inverse transforms through HBM (for comparison).
 """
 from abc import ABCMeta, abstractmethod
 from math import gamma as _gammafn
@@ -48,7 +49,10 @@ def _precision(cfg):
 
 
 def _flags(cfg):
-    return _lib.FLAG_PROJECT if cfg.getint('scattering-model', 'conservation-projection', 0) else 0
+    fl = _lib.FLAG_PROJECT if cfg.getint('scattering-model', 'conservation-projection', 0) else 0
+    if not cfg.getint('scattering-model', 'fused', 1):
+        fl |= _lib.FLAG_UNFUSED
+    return fl
 
 
 def _stream(backend):

@@ -62,8 +62,10 @@ enum fq_precision {
 };
 
 enum fq_flags {
-    FQ_FLAG_PROJECT = 1       /* project Q onto the complement of the collision
+    FQ_FLAG_PROJECT = 1,      /* project Q onto the complement of the collision
                                  invariants (mass/momentum/energy), in FP64 */
+    FQ_FLAG_UNFUSED = 2       /* stage every inverse transform through HBM
+                                 (three-kernel path; for comparison) */
 };
 
 /*
@@ -153,6 +155,21 @@ int fq_eval_multi(fq_handle *h, const double *const *f, double *const *q,
 int fq_eval_bilinear(fq_handle *h, const double *f, const double *g, double *q,
                      int32_t nupts, int64_t ldim, int32_t neles,
                      int64_t first_dof, int64_t ndof, void *stream);
+
+/* ---- instrumentation --------------------------------------------------- */
+
+#define FQ_NKERNELS 9
+/* Bracket every kernel launch with HIP events on the launch stream (on = 1)
+ * or stop (on = 0); either call discards earlier records.  No reference
+ * counterpart: the reference times only its precompute with cuda.Event
+ * (frfs/solvers/dgfsbi/scattering.py:166-205). */
+int fq_profile_enable(fq_handle *h, int on);
+/* Wait for the recorded events; ms[k] / launches[k] = summed duration and
+ * launch count of kernel k (k < FQ_NKERNELS) since the last read. */
+int fq_profile_read(fq_handle *h, double *ms, int64_t *launches);
+const char *fq_kernel_name(int k);
+/* DOFs per internal batch (kernel launch sequence). */
+int fq_chunk_dofs(const fq_handle *h, int64_t *chunk);
 
 /* ---- host-only plan inspection (no GPU needed) ------------------------- */
 

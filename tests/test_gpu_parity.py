@@ -44,11 +44,12 @@ def _run_vhs(backend, cfg, f, nupts, neles, model_cls=None, per_dof=False):
     return _from_io(fout.get()), sm
 
 
+@pytest.mark.parametrize('fused', [1, 0])
 @pytest.mark.parametrize('prec,tol', [('double', FP64_TOL), ('mixed', MIXED_TOL)])
-def test_vhs_config1_shape(backend, prec, tol):
+def test_vhs_config1_shape(backend, prec, tol, fused):
     """VHS omega=0.5, 16^3 on [-5,5]^3, M=6 half sphere, Nrho=8."""
     from oracle import fastspectral as fsp
-    cfg = vhs_cfg(16, 8, 6, 5.0, precision=prec)
+    cfg = vhs_cfg(16, 8, 6, 5.0, precision=prec, fused=fused)
     vm, prm, tab = oracle_vhs(cfg)
     nupts, neles = 2, 3
     f = fsp.two_maxwellian_batch(vm['cv'], nupts * neles, seed=11)
@@ -69,17 +70,18 @@ def test_vhs_per_dof_fs_matches_batch(backend):
     assert np.array_equal(Qb, Qd)
 
 
+@pytest.mark.parametrize('prec,tol', [('double', FP64_TOL), ('mixed', MIXED_TOL)])
 @pytest.mark.parametrize('N,Nrho,dev', [(8, 4, 4.0), (24, 6, 5.0), (32, 4, 6.0)])
-def test_vhs_grid_sizes(backend, N, Nrho, dev):
+def test_vhs_grid_sizes(backend, N, Nrho, dev, prec, tol):
     from oracle import fastspectral as fsp
-    cfg = vhs_cfg(N, Nrho, 6, dev)
+    cfg = vhs_cfg(N, Nrho, 6, dev, precision=prec)
     vm, prm, tab = oracle_vhs(cfg)
     f = fsp.two_maxwellian_batch(vm['cv'], 2, seed=N)
     print(f'[vhs N={N}]', flush=True)
     Q, _ = _run_vhs(backend, cfg, f, 1, 2)
     errs = [rel_l2(Q[d], fsp.vhs_fs(f[d], vm, tab, prm['prefac'])) for d in range(len(f))]
-    print(N, max(errs))
-    assert max(errs) <= FP64_TOL
+    print(N, prec, max(errs))
+    assert max(errs) <= tol
 
 
 def test_vhs_nodal_bilinear(backend):
@@ -119,11 +121,12 @@ def _run_vss(backend, cfg, fs, nupts, neles):
     return [_from_io(q.get()) for q in qout], sm
 
 
+@pytest.mark.parametrize('fused', [1, 0])
 @pytest.mark.parametrize('prec,tol', [('double', FP64_TOL), ('mixed', MIXED_TOL)])
-def test_vss_single_species(backend, prec, tol):
+def test_vss_single_species(backend, prec, tol, fused):
     """VSS omega=0.74 alpha=1.4, full sphere M=12, Mpre=32."""
     from oracle import fastspectral as fsp
-    cfg = vss_cfg(16, 6, 12, 5.0, precision=prec)
+    cfg = vss_cfg(16, 6, 12, 5.0, precision=prec, fused=fused)
     vm, prm, tab = oracle_vss(cfg, 1)
     f = fsp.two_maxwellian_batch(vm['cv'], 3, seed=21)
     print(f'[vss-1 {prec}]', flush=True)
