@@ -1,0 +1,240 @@
+"""Pins of the CPU oracle against what mathematics fixes (the reference ships
+no tests, golden vectors or fixtures for this path).
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+term, a sign, a transposed index, a wrong quadrature factor) fails it.
+"""
+import numpy as np
+import pytest
+import scipy.special as sp
+
+from oracle import fastspectral as fsp
+from tests.cases import ARGON, oracle_vhs, oracle_vss, vhs_cfg, vss_cfg
+
+
+# ---------------------------------------------------------------- quadrature
+@pytest.mark.parametrize('n', [4, 6, 8, 12, 16])
+def test_gauss_legendre_matches_numpy(n):
+    z, w = fsp.zwgj(n, 0.0, 0.0)
+    zr, wr = np.polynomial.legendre.leggauss(n)
+    assert np.max(np.abs(z - zr)) < 1e-14
+    assert np.max(np.abs(w - wr)) < 1e-14
+
+
+@pytest.mark.parametrize('a,b', [(0.5, -0.3), (1.0, 2.0), (-0.5, -0.5)])
+def test_gauss_jacobi_matches_scipy(a, b):
+    z, w = fsp.zwgj(9, a, b)
+    zr, wr = sp.roots_jacobi(9, a, b)
+    assert np.max(np.abs(z - zr)) < 1e-13
+    assert np.max(np.abs(w - wr) / wr) < 1e-12
+
+
+@pytest.mark.parametrize('a,b', [(0.0, 0.0), (0.5, 1.5)])
+def test_gauss_lobatto_exactness(a, b):
+    n = 7
+    z, w = fsp.zwglj(n, a, b)
+    assert abs(z[0] + 1) < 1e-14 and abs(z[-1] - 1) < 1e-14
+    # exact for x^k, k <= 2n-3, against the Jacobi weight (1-x)^a (1+x)^b
+    for k in range(2 * n - 2):
+        exact = sum(sp.comb(k, j) * 2.0 ** j * (-1) ** (k - j) * 2 ** (a + b + 1) * sp.beta(a + 1, b + j + 1)
+                    for j in range(k + 1))       # x = 2t - 1, t ~ Beta
+        assert abs(np.dot(w, z ** k) - exact) < 1e-11 * max(1, abs(exact))
+
+
+# ---------------------------------------------------------------- designs
+@pytest.mark.parametrize('npts', [6, 12, 32, 48, 192])
+def test_design_invariants(npts):
+    s = fsp.load_design('womersley', npts)
+    assert s.shape == (npts, 3)
+    assert np.max(np.abs(np.linalg.norm(s, axis=1) - 1)) < 1e-12
+    h = npts // 2
+    assert np.array_equal(s[h:], -s[:h])
+    assert np.max(np.abs(s.T @ s / npts - np.eye(3) / 3)) < 1e-12
+
+
+def test_design_missing_raises():
+    with pytest.raises(ValueError, match='No suitable spherical quadrature rule'):
+        fsp.load_design('womersley', 24)
+
+
+# ---------------------------------------------------------------- FFT conventions
+@pytest.mark.parametrize('N', [4, 6, 8])
+def test_dft_conventions_brute_force(N):
+    rng = np.random.default_rng(N)
+    x = rng.normal(size=(N, N, N)) + 1j * rng.normal(size=(N, N, N))
+    k = np.arange(N)
+    Wm = np.exp(-2j * np.pi * np.outer(k, k) / N)
+    Wp = np.conj(Wm)
+    fwd = np.einsum('ai,bj,ck,ijk->abc', Wm, Wm, Wm, x)
+    inv = np.einsum('ai,bj,ck,ijk->abc', Wp, Wp, Wp, x)
+    assert np.allclose(fsp.dft_fwd(x, (0, 1, 2)), fwd, atol=1e-12)
+    assert np.allclose(fsp.dft_inv(x, (0, 1, 2)), inv, atol=1e-12)
+
+
+def test_lvec_nyquist_convention():
+    l = fsp.lvec(4)
+    assert set(np.unique(l[0])) == {-2, -1, 0, 1}
+    assert l[2, 2] == -2 and l[0, 2 * 16] == -2
+
+
+# ---------------------------------------------------------------- tables
+def _sphere_avg(vec, pts):
+    """(1/4pi) * spherical quadrature of e^{i vec.sigma} (degree-19 design)."""
+    return np.mean(np.exp(1j * (pts @ vec)), axis=0)
+
+
+def test_bb1_is_spherical_average_of_plane_wave():
+    cfg = vhs_cfg(16, 8, 6, 5.0)
+    vm, prm, tab = oracle_vhs(cfg)
+    pts = fsp.load_design('womersley', 192)
+    l = fsp.lvec(16)
+    for p in (0, 3, 7):
+        c = np.pi / vm['L'] * vm['qz'][p] / 2.
+        sel = np.where(np.linalg.norm(l, axis=0) * c < 2.0)[0][:200]
+        want = np.real(_sphere_avg(c * l[:, sel], pts)) * 4 * np.pi * vm['qz'][p] ** (prm['gamma'] + 2)
+        assert np.max(np.abs(tab['bb1'][p, sel] - want)) < 1e-5 * np.max(np.abs(want))
+
+
+def test_vss_b_table_alpha1_is_plane_wave_average():
+    cfg = vss_cfg(8, 3, 12, 5.0, alpha={'00': 1.0}, Mpre=192)
+    vm, prm, tab = oracle_vss(cfg, 1)
+    l = fsp.lvec(8)
+    b = tab['b']['00']
+    for x in range(3):
+        c = np.pi / vm['L'] * vm['qz'][x] * 0.5       # mass factor m/(2m)
+        sel = np.where(np.linalg.norm(l, axis=0) * c < 2.0)[0]
+        want = 4 * np.pi * vm['qz'][x] ** (prm['gamma']['00'] + 2) * np.sinc(c * np.linalg.norm(l[:, sel], axis=0) / np.pi)
+        for y in (0, 5):
+            assert np.max(np.abs(b[x, y, sel] - want)) < 1e-5 * np.max(np.abs(want))
+
+
+# ---------------------------------------------------------------- operator identities
+@pytest.fixture(scope='module')
+def vhs16():
+    cfg = vhs_cfg(16, 6, 6, 5.0)
+    vm, prm, tab = oracle_vhs(cfg)
+    f = fsp.two_maxwellian_batch(vm['cv'], 2, seed=99)
+    return vm, prm, tab, f
+
+
+def test_quadratic_homogeneity(vhs16):
+    vm, prm, tab, f = vhs16
+    q1 = fsp.vhs_fs(f[0], vm, tab, prm['prefac'])
+    q3 = fsp.vhs_fs(3.0 * f[0], vm, tab, prm['prefac'])
+    assert np.linalg.norm(q3 - 9 * q1) / np.linalg.norm(9 * q1) < 1e-13
+
+
+def test_shift_equivariance(vhs16):
+    vm, prm, tab, f = vhs16
+    N = vm['Nv']
+    g = f[0].reshape(N, N, N)
+    q = fsp.vhs_fs(g.ravel(), vm, tab, prm['prefac']).reshape(N, N, N)
+    gs = np.roll(g, (2, -3, 1), axis=(0, 1, 2))
+    qs = fsp.vhs_fs(gs.ravel(), vm, tab, prm['prefac']).reshape(N, N, N)
+    assert np.linalg.norm(qs - np.roll(q, (2, -3, 1), axis=(0, 1, 2))) / np.linalg.norm(q) < 1e-13
+
+
+def test_half_sphere_cos_sin_equals_full_sphere_product(vhs16):
+    """A^2 + B^2 with cos/sin on M half-sphere points and weight 2 sw equals the
+    product form e^{+ia}F x e^{-ia}F over the full 2M design with weight 4pi/2M."""
+    vm, prm, tab, f = vhs16
+    N, Nrho, M = vm['Nv'], vm['Nrho'], vm['M']
+    n = N ** 3
+    full = fsp.load_design('womersley', 2 * M)
+    l = fsp.lvec(N)
+    F = np.fft.fftn(f[1].reshape(N, N, N)).ravel() / n
+    H = np.zeros(n, complex)
+    for p in range(Nrho):
+        c = np.pi / vm['L'] * vm['qz'][p] / 2
+        G = np.zeros((N, N, N), complex)
+        for s in full:
+            a = c * (s @ l)
+            Ep = np.fft.ifftn((np.exp(1j * a) * F).reshape(N, N, N)) * n
+            Em = np.fft.ifftn((np.exp(-1j * a) * F).reshape(N, N, N)) * n
+            G += Ep * Em
+        H += (vm['qw'][p] * (4 * np.pi / (2 * M)) / n) * tab['bb1'][p] * np.fft.fftn(G).ravel()
+    gain = np.real(np.fft.ifftn(H.reshape(N, N, N)) * n).ravel()
+    loss = np.real(np.fft.ifftn((tab['bb2'] * F).reshape(N, N, N)) * n).ravel()
+    q_full = prm['prefac'] * (gain - f[1] * loss)
+    q_ref = fsp.vhs_fs(f[1], vm, tab, prm['prefac'])
+    assert np.linalg.norm(q_full - q_ref) / np.linalg.norm(q_ref) < 1e-13
+
+
+def test_maxwellian_is_nearly_stationary(vhs16):
+    vm, prm, tab, f = vhs16
+    M0 = fsp.maxwellian(vm['cv'], 1.0, [0.2, -0.1, 0.0], 1.0)
+    qm = fsp.vhs_fs(M0, vm, tab, prm['prefac'])
+    qf = fsp.vhs_fs(f[0], vm, tab, prm['prefac'])
+    assert np.linalg.norm(qm) / np.linalg.norm(qf) < 3e-3
+
+
+def test_collision_invariants_small():
+    cfg = vhs_cfg(24, 8, 6, 6.0)
+    vm, prm, tab = oracle_vhs(cfg)
+    f = fsp.maxwellian(vm['cv'], 1.0, [0.4, 0, 0], 0.8) + fsp.maxwellian(vm['cv'], 0.6, [-0.3, 0.2, 0], 0.6)
+    q = fsp.vhs_fs(f, vm, tab, prm['prefac'])
+    m = fsp.moments(q, vm['cv'], vm['cw'])
+    scale = np.sum(np.abs(q)) * vm['cw'] * np.array([1, 1, 1, 1, 2])
+    # the spectral method conserves to ~5e-3 here; a dropped gain/loss term
+    # or a wrong weight gives O(1) defects
+    assert np.all(np.abs(m) < 1e-2 * scale)
+
+
+@pytest.mark.parametrize('N,Nrho,dev,tol', [(16, 8, 5.0, 8e-3), (24, 12, 6.0, 6e-4)])
+def test_bkw_time_derivative(N, Nrho, dev, tol):
+    """Maxwell molecules (omega = 1): the BKW solution of
+    frfs/solvers/dgfs/initcond.py:120-140 satisfies Q(f)/prefac = 4 pi df/dt."""
+    cfg = vhs_cfg(N, Nrho, 6, dev, omega=1.0)
+    vm, prm, tab = oracle_vhs(cfg)
+    t0, eps = 5.5, 1e-5
+    f = fsp.bkw(vm['cv'], t0)
+    dfdt = (fsp.bkw(vm['cv'], t0 + eps) - fsp.bkw(vm['cv'], t0 - eps)) / (2 * eps)
+    q = fsp.vhs_fs(f, vm, tab, 1.0)
+    assert np.linalg.norm(q - 4 * np.pi * dfdt) / np.linalg.norm(q) < tol
+
+
+def test_vss_alpha1_single_species_matches_vhs():
+    """VSS with alpha = 1, one species, unit mass reduces to VHS up to the
+    Mpre quadrature of b and c."""
+    cfgh = vhs_cfg(8, 4, 6, 5.0, omega=0.74)
+    vmh, prmh, tabh = oracle_vhs(cfgh)
+    cfgs = vss_cfg(8, 4, 12, 5.0, alpha={'00': 1.0}, omega={'00': 0.74}, Mpre=192)
+    vms, prms, tabs = oracle_vss(cfgs, 1)
+    f = fsp.two_maxwellian_batch(vmh['cv'], 1, seed=4)[0]
+    qh = fsp.vhs_fs(f, vmh, tabh, prmh['prefac'])
+    qs = fsp.vss_fs([f], vms, tabs, prms)[0]
+    # measured 1e-4 (quadrature of the degree-19 design at the highest modes)
+    assert np.linalg.norm(qs - qh) / np.linalg.norm(qh) < 1e-3
+
+
+def test_vss_two_species_total_mass_momentum():
+    """Per-species mass and total momentum/energy of a two-species Q are
+    invariants of the exact operator; the discrete defect is small."""
+    m0 = ARGON['molarMass0']
+    cfg = vss_cfg(24, 6, 12, 6.0, masses=(m0, 2 * m0), Mpre=48)
+    vm, prm, tab = oracle_vss(cfg, 2)
+    ms = vm['masses']
+    f0 = fsp.maxwellian(vm['cv'], 1.0, [0.3, 0, 0], 0.9, ms[0])
+    f1 = fsp.maxwellian(vm['cv'], 0.5, [-0.2, 0.1, 0], 0.7, ms[1])
+    q0, q1 = fsp.vss_fs([f0, f1], vm, tab, prm)
+    cw = vm['cw']
+    scale = (np.sum(np.abs(q0)) + np.sum(np.abs(q1))) * cw
+    # measured <5e-3 at 24^3 (spectral conservation error); missing or
+    # mis-weighted pair terms give O(1) defects
+    assert abs(np.sum(q0) * cw) < 1e-2 * scale and abs(np.sum(q1) * cw) < 1e-2 * scale
+    mom0 = ms[0] * (q0 @ vm['cv'][0]) * cw
+    mom = mom0 + ms[1] * (q1 @ vm['cv'][0]) * cw
+    # species 0 loses momentum at O(1); the exchange cancels in the total
+    assert abs(mom0) > 0.2 * scale and abs(mom) < 2e-2 * scale
+
+
+# ---------------------------------------------------------------- AoSoA
+def test_aosoa_pack_positions():
+    rng = np.random.default_rng(0)
+    a = rng.normal(size=(3, 5, 37))
+    buf = fsp.aosoa_pack(a)
+    nrow, ld = fsp.aosoa_shape(3, 5, 37)
+    assert buf.shape == (nrow, ld) and ld % 16 == 0
+    for u, v, e in [(0, 0, 0), (2, 4, 36), (1, 3, 31), (1, 2, 32)]:
+        assert buf.ravel()[u * ld + fsp.soa_ix(e, v, 5)] == a[u, v, e]
+    assert np.array_equal(fsp.aosoa_unpack(buf, a.shape), a)
