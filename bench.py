@@ -40,6 +40,8 @@ def parse():
     ap.add_argument('--cpu-procs', type=int, default=0, help='0 = min(cpus, 16)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--modes', default='fp64,mixed')
+    ap.add_argument('--calib-copy-mib', type=int, default=0,
+                    help='run one device-to-device copy of this size first (counter calibration)')
     return ap.parse_args()
 
 
@@ -133,6 +135,12 @@ def main():
     from frfs_amd.velocitymesh import DGFSBiVelocityMesh
 
     be = HIPBackend()
+    if a.calib_copy_mib:
+        src = torch.ones(a.calib_copy_mib << 17, dtype=torch.float64, device='cuda')
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     modes = a.modes.split(',')
     res = {}
     qs = {}

@@ -12,7 +12,8 @@ import os
 
 import numpy as np
 
-LIBPATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libfrfs_fq.so')
+LIBPATH = os.environ.get('FRFS_FQ_LIB') or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), 'libfrfs_fq.so')
 
 ABI_VERSION = 1
 

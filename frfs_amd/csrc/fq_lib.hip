@@ -568,12 +568,24 @@ __global__ __launch_bounds__(256) void k_accum(const cpx<T> *P, const cpx<double
 #pragma unroll
     for (int e = 0; e < E; ++e) H[e] = {0, 0};
     const int c0 = t_c0[t], cn = t_cn[t];
+    const cpx<T> *pbase = P + int64_t(slot) * ngroups * G::VOL + off;
+    // software pipeline: the pencil of contribution ci+1 is in flight while
+    // contribution ci is transformed and accumulated
+    cpx<T> nxt[E];
+    if (cn > 0) {
+        const cpx<T> *src = pbase + int64_t(c_group[c0]) * G::VOL;
+#pragma unroll
+        for (int e = 0; e < E; ++e) nxt[e] = act ? src[int64_t(half + 2 * e) * G::N2] : cpx<T>{0, 0};
+    }
     for (int ci = c0; ci < c0 + cn; ++ci) {
-        const int g = c_group[ci];
-        const cpx<T> *src = P + (int64_t(slot) * ngroups + g) * G::VOL + off;
         cpx<T> v[E];
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = act ? src[int64_t(half + 2 * e) * G::N2] : cpx<T>{0, 0};
+        for (int e = 0; e < E; ++e) v[e] = nxt[e];
+        if (ci + 1 < c0 + cn) {
+            const cpx<T> *src = pbase + int64_t(c_group[ci + 1]) * G::VOL;
+#pragma unroll
+            for (int e = 0; e < E; ++e) nxt[e] = act ? src[int64_t(half + 2 * e) * G::N2] : cpx<T>{0, 0};
+        }
         fq::fft2_dit<N, -1>(v, half);             // -> K' = half*E + e
         const cpx<double> *w = W + int64_t(ci) * G::VOL + off;
 #pragma unroll
@@ -1137,10 +1149,13 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     // scratch: F (FP64, nsrc) + max(Z (items, T), T0 (nsrc, FP64), Y (2 ntgt, FP64)) + P (groups, T)
     const size_t tsz = h->precision == FQ_PREC_MIXED ? sizeof(cpx<float>) : sizeof(cpx<double>);
     const size_t per_dof_F = size_t(P.nsrc) * n * sizeof(cpx<double>);
-    const size_t per_dof_Z = std::max({P.items.size() * n * tsz, size_t(P.nsrc) * n * sizeof(cpx<double>),
+    // Z holds the staged inverse transforms (three-kernel path only), T0 and Y
+    const size_t per_dof_Z = std::max({h->fused ? size_t(0) : P.items.size() * n * tsz,
+                                       size_t(P.nsrc) * n * sizeof(cpx<double>),
                                        size_t(2 * P.ntgt) * n * sizeof(cpx<double>)});
     const size_t per_dof_P = size_t(P.ngroups) * n * tsz;
-    const size_t per_dof = per_dof_F + per_dof_Z + per_dof_P;
+    const size_t per_dof_F32 = h->precision == FQ_PREC_MIXED ? size_t(P.nsrc) * n * sizeof(cpx<float>) : 0;
+    const size_t per_dof = per_dof_F + per_dof_Z + per_dof_P + per_dof_F32;
     int64_t chunk = prm->max_chunk;
     if (chunk <= 0) {
         const size_t budget = size_t(32) << 30;   // 32 GiB default scratch budget (of 288 GB HBM)
