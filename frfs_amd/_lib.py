@@ -19,6 +19,8 @@ ABI_VERSION = 1
 
 SUCCESS, ERR_INVALID, ERR_UNSUPPORTED, ERR_HIP, ERR_ALLOC = 0, -1, -2, -3, -4
 NKERNELS = 9
+AXNPBY_MAX = 4
+NMOMENTS = 14
 MODEL_VHS, MODEL_VHS_NODAL, MODEL_VSS = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 FLAG_PROJECT = 1
@@ -71,6 +73,10 @@ SIGNATURES = {
     'fq_profile_read': (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(_i64)]),
     'fq_kernel_name': (C.c_char_p, [C.c_int]),
     'fq_chunk_dofs': (C.c_int, [_vp, C.POINTER(_i64)]),
+    'fq_axnpby': (C.c_int, [_i64, _vp, _i32, C.POINTER(_vp), C.POINTER(C.c_double), _vp]),
+    'fq_copy_dgfs': (C.c_int, [_i32, _i32, _i32, _vp, _i64, _i32, _i32, _vp, _i64, _i32, _i32, _vp]),
+    'fq_moments': (C.c_int, [_i32, C.c_double, C.c_double, C.c_double, _vp, _i32, _i64, _i32, _i64, _i64,
+                             _vp, _vp]),
     'fq_plan_create': (C.c_int, [C.POINTER(fq_params), C.POINTER(_vp)]),
     'fq_plan_destroy': (C.c_int, [_vp]),
     'fq_plan_get_info': (C.c_int, [_vp, C.POINTER(fq_plan_info)]),
@@ -233,3 +239,21 @@ class Handle:
 
     def __del__(self):
         self.close()
+
+
+def axnpby(n, x0, xs, a, stream=0):
+    """x0 = a[0] x0 + sum a[k+1] xs[k] over n doubles (device pointers)."""
+    nv = len(xs)
+    xa = (C.c_void_p * max(nv, 1))(*xs)
+    aa = (C.c_double * (nv + 1))(*a)
+    check(load().fq_axnpby(n, x0, nv, xa, aa, stream))
+
+
+def copy_dgfs(nrow, neles, count, src, ldim_src, nvar_src, sidx_src, dst, ldim_dst, nvar_dst, sidx_dst,
+              stream=0):
+    check(load().fq_copy_dgfs(nrow, neles, count, src, ldim_src, nvar_src, sidx_src, dst, ldim_dst,
+                              nvar_dst, sidx_dst, stream))
+
+
+def moments(Nv, L, cw, mass_ratio, f, nupts, ldim, neles, first_dof, ndof, out, stream=0):
+    check(load().fq_moments(Nv, L, cw, mass_ratio, f, nupts, ldim, neles, first_dof, ndof, out, stream))

@@ -1045,6 +1045,8 @@ int eval_impl(fq_handle *h, const AosoaPtrs &ptr, int32_t nupts, int64_t ldim, i
 
 }  // namespace
 
+int fq_internal_fail(int code, const std::string &msg) { return fail(code, msg); }
+
 extern "C" {
 
 const char *fq_last_error(void) { return g_err.c_str(); }

@@ -171,6 +171,32 @@ const char *fq_kernel_name(int k);
 /* DOFs per internal batch (kernel launch sequence). */
 int fq_chunk_dofs(const fq_handle *h, int64_t *chunk);
 
+/* ---- kernels on either side of Q (stateless, AoSoA banks) -------------- */
+
+#define FQ_AXNPBY_MAX 4
+/* x0[i] = a[0]*x0[i] + sum_{k<nv} a[k+1]*xs[k][i] for i < n over whole banks
+ * (a[0] == 0 never reads x0; a[0] == 1 accumulates): replaces the
+ * axnpby_dgfs_full kernel (frfs/backends/cuda/kernels/axnpby_dgfs_full.mako,
+ * frfs/backends/cuda/blasext.py:46-75).  nv <= FQ_AXNPBY_MAX. */
+int fq_axnpby(int64_t n, double *x0, int32_t nv, const double *const *xs, const double *a, void *stream);
+
+/* dst[u, sidx_dst + v, e] = src[u, sidx_src + v, e] for v < count, every
+ * point u < nrow and element e < neles, between AoSoA matrices of nvar_src /
+ * nvar_dst variables: replaces copy_dgfs (frfs/backends/cuda/kernels/copy_dgfs.mako,
+ * frfs/backends/cuda/blasext.py:77-138, copy_to_reg / copy_from_reg). */
+int fq_copy_dgfs(int32_t nrow, int32_t neles, int32_t count, const double *src, int64_t ldim_src,
+                 int32_t nvar_src, int32_t sidx_src, double *dst, int64_t ldim_dst, int32_t nvar_dst,
+                 int32_t sidx_dst, void *stream);
+
+#define FQ_NMOMENTS 14
+/* Non-dimensional moments per DOF, out[d*FQ_NMOMENTS + k] in the order of
+ * _compute_moments_3D (frfs/plugins/dgfsmomwriterstd.py:142-214):
+ * rho, ux, uy, uz, T, qx, qy, qz, Pxx, Pxy, Pxz, Pyy, Pyz, Pzz, for the
+ * velocity grid of Nv points per axis on [-L, L]^3 with cell volume cw and
+ * species mass ratio mass_ratio. */
+int fq_moments(int32_t Nv, double L, double cw, double mass_ratio, const double *f, int32_t nupts,
+               int64_t ldim, int32_t neles, int64_t first_dof, int64_t ndof, double *out, void *stream);
+
 /* ---- host-only plan inspection (no GPU needed) ------------------------- */
 
 int fq_plan_create(const fq_params *prm, fq_plan **out);

@@ -475,3 +475,18 @@ def moments(f, cv, cw, mass=1.0):
                      np.sum(f * cv[0]) * cw, np.sum(f * cv[1]) * cw,
                      np.sum(f * cv[2]) * cw,
                      np.sum(f * np.sum(cv * cv, axis=0)) * cw]) * mass
+
+
+def bulk_moments_3d(f, cv, cw, mr=1.0):
+    """Non-dimensional moments of _compute_moments_3D
+    (frfs/plugins/dgfsmomwriterstd.py:142-214) for one DOF:
+    rho, ux, uy, uz, T, qx, qy, qz, Pxx, Pxy, Pxz, Pyy, Pyz, Pzz."""
+    mcw = mr * cw
+    rho = np.sum(f) * mcw
+    u = [np.dot(f, cv[d]) * mcw / rho for d in range(3)]
+    c = [cv[d] - u[d] for d in range(3)]
+    c2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2]
+    T = np.sum(f * c2) * (2.0 / 3.0 * mcw * mr) / rho
+    q = [mr * np.sum(f * c2 * c[d]) * mcw for d in range(3)]
+    P = [2 * mr * np.sum(f * c[i] * c[j]) * mcw for i, j in ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))]
+    return np.array([rho, *u, T, *q, *P])

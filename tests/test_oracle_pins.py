@@ -238,3 +238,16 @@ def test_aosoa_pack_positions():
     for u, v, e in [(0, 0, 0), (2, 4, 36), (1, 3, 31), (1, 2, 32)]:
         assert buf.ravel()[u * ld + fsp.soa_ix(e, v, 5)] == a[u, v, e]
     assert np.array_equal(fsp.aosoa_unpack(buf, a.shape), a)
+
+
+def test_moments_recover_maxwellian_parameters():
+    """A Maxwellian of density 1.3, velocity u and temperature T (frfs
+    non-dimensional form exp(-|c-u|^2/T)) has moments rho, u, T, zero heat
+    flux and pressure tensor rho*T*I up to grid/truncation error."""
+    vm = fsp.velocity_mesh(32, 4, 6, dev=6.0, **ARGON)
+    u, T = np.array([0.3, -0.2, 0.1]), 0.9
+    f = fsp.maxwellian(vm['cv'], 1.3, u, T)
+    m = fsp.bulk_moments_3d(f, vm['cv'], vm['cw'])
+    assert abs(m[0] - 1.3) < 1e-8 and np.allclose(m[1:4], u, atol=1e-8) and abs(m[4] - T) < 1e-8
+    assert np.allclose(m[5:8], 0, atol=1e-8)
+    assert np.allclose(m[8:14], [1.3 * T, 0, 0, 1.3 * T, 0, 1.3 * T], atol=1e-8)
