@@ -77,6 +77,8 @@ SIGNATURES = {
     'fq_copy_dgfs': (C.c_int, [_i32, _i32, _i32, _vp, _i64, _i32, _i32, _vp, _i64, _i32, _i32, _vp]),
     'fq_moments': (C.c_int, [_i32, C.c_double, C.c_double, C.c_double, _vp, _i32, _i64, _i32, _i64, _i64,
                              _vp, _vp]),
+    'fq_fr_rhs_quad': (C.c_int, [_i32, _i32, _i32, C.c_double, C.c_double, _vp, _vp, _vp, _vp, _vp, _i64,
+                                 _i32, _vp]),
     'fq_plan_create': (C.c_int, [C.POINTER(fq_params), C.POINTER(_vp)]),
     'fq_plan_destroy': (C.c_int, [_vp]),
     'fq_plan_get_info': (C.c_int, [_vp, C.POINTER(fq_plan_info)]),

@@ -106,9 +106,91 @@ __global__ __launch_bounds__(256) void k_moments(int N, double L, double cw, dou
     }
 }
 
+// FR advection right-hand side -div(c f) on a periodic nelx x nely mesh of
+// affine hx x hy quads, order P, one thread per (velocity v, element e):
+//   rhs = -rcpdjac [ (hy/2) cx Dx u + (hx/2) cy Dy u + M3 F* ],
+//   F*_f = |n_f| c_n u_upwind   (upwind common normal flux, rusanov.mako)
+// ops = {Dx[NU][NU], Dy[NU][NU], M0[NF][NU], M3[NU][NF]} (row major).
+template <int P>
+__global__ __launch_bounds__(256) void k_fr_rhs_quad(int nelx, int nely, double hx, double hy,
+                                                     const double *__restrict__ ops, const double *__restrict__ cvx,
+                                                     const double *__restrict__ cvy, const double *__restrict__ u,
+                                                     double *__restrict__ out, int64_t ldim, int nvar) {
+    constexpr int N1 = P + 1, NU = N1 * N1, NF = 4 * N1;
+    const int neles = nelx * nely;
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= int64_t(nvar) * neles) return;
+    const int e = int(t % neles);
+    const int v = int(t / neles);
+    const double *Dx = ops, *Dy = ops + NU * NU, *M0 = ops + 2 * NU * NU, *M3 = M0 + NF * NU;
+    auto at = [&](int upt, int ee) { return upt * ldim + (int64_t(ee / 32) * nvar + v) * 32 + ee % 32; };
+    const int ex = e % nelx, ey = e / nelx;
+    const int nb[4] = {ex + nelx * ((ey + nely - 1) % nely), (ex + 1) % nelx + nelx * ey,
+                       ex + nelx * ((ey + 1) % nely), (ex + nelx - 1) % nelx + nelx * ey};
+    const double nx[4] = {0, 1, 0, -1}, ny[4] = {-1, 0, 1, 0};
+    const double cx = cvx[v], cy = cvy[v];
+    double ue[NU];
+#pragma unroll
+    for (int k = 0; k < NU; ++k) ue[k] = u[at(k, e)];
+    double fstar[NF];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        double un[NU];
+#pragma unroll
+        for (int k = 0; k < NU; ++k) un[k] = u[at(k, nb[f])];
+        const int fo = (f + 2) % 4;
+        const double cn = cx * nx[f] + cy * ny[f];
+        const double magn = (f % 2 == 0) ? 0.5 * hx : 0.5 * hy;
+#pragma unroll
+        for (int k = 0; k < N1; ++k) {
+            double uo = 0, ui = 0;
+#pragma unroll
+            for (int m = 0; m < NU; ++m) {
+                uo += M0[(f * N1 + k) * NU + m] * ue[m];
+                ui += M0[(fo * N1 + k) * NU + m] * un[m];
+            }
+            fstar[f * N1 + k] = magn * cn * (cn > 0 ? uo : ui);
+        }
+    }
+    const double ax = 0.5 * hy * cx, ay = 0.5 * hx * cy, rcp = -4.0 / (hx * hy);
+#pragma unroll
+    for (int i = 0; i < NU; ++i) {
+        double sx = 0, sy = 0, sc = 0;
+#pragma unroll
+        for (int k = 0; k < NU; ++k) {
+            sx += Dx[i * NU + k] * ue[k];
+            sy += Dy[i * NU + k] * ue[k];
+        }
+#pragma unroll
+        for (int j = 0; j < NF; ++j) sc += M3[i * NF + j] * fstar[j];
+        out[at(i, e)] = rcp * (ax * sx + ay * sy + sc);
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+int fq_fr_rhs_quad(int32_t order, int32_t nelx, int32_t nely, double hx, double hy, const double *ops,
+                   const double *cvx, const double *cvy, const double *u, double *out, int64_t ldim, int32_t nvar,
+                   void *stream) {
+    if (nelx <= 0 || nely <= 0 || !(hx > 0) || !(hy > 0) || !ops || !cvx || !cvy || !u || !out || nvar <= 0)
+        return fq_internal_fail(FQ_ERR_INVALID, "fq_fr_rhs_quad: invalid arguments");
+    const int64_t nparr = (int64_t(nelx) * nely + 31) / 32 * 32;
+    if (ldim < int64_t(nvar) * nparr) return fq_internal_fail(FQ_ERR_INVALID, "fq_fr_rhs_quad: ldim too small");
+    if (u == out) return fq_internal_fail(FQ_ERR_INVALID, "fq_fr_rhs_quad: in-place evaluation is not supported");
+    const int64_t tot = int64_t(nvar) * nelx * nely;
+    const dim3 grid(unsigned((tot + 255) / 256)), blk(256);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    switch (order) {
+    case 1: hipLaunchKernelGGL(k_fr_rhs_quad<1>, grid, blk, 0, st, nelx, nely, hx, hy, ops, cvx, cvy, u, out, ldim, nvar); break;
+    case 2: hipLaunchKernelGGL(k_fr_rhs_quad<2>, grid, blk, 0, st, nelx, nely, hx, hy, ops, cvx, cvy, u, out, ldim, nvar); break;
+    case 3: hipLaunchKernelGGL(k_fr_rhs_quad<3>, grid, blk, 0, st, nelx, nely, hx, hy, ops, cvx, cvy, u, out, ldim, nvar); break;
+    default: return fq_internal_fail(FQ_ERR_UNSUPPORTED, "fq_fr_rhs_quad: order must be 1, 2 or 3");
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? FQ_SUCCESS : fq_internal_fail(FQ_ERR_HIP, std::string("fq_fr_rhs_quad: ") + hipGetErrorString(e));
+}
 
 int fq_axnpby(int64_t n, double *x0, int32_t nv, const double *const *xs, const double *a, void *stream) {
     if (n < 0 || !x0 || !a || nv < 0 || nv > FQ_AXNPBY_MAX || (nv > 0 && !xs))

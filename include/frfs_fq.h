@@ -197,6 +197,19 @@ int fq_copy_dgfs(int32_t nrow, int32_t neles, int32_t count, const double *src, 
 int fq_moments(int32_t Nv, double L, double cw, double mass_ratio, const double *f, int32_t nupts,
                int64_t ldim, int32_t neles, int64_t first_dof, int64_t ndof, double *out, void *stream);
 
+/* Flux-reconstruction advection right-hand side out = -div(c f) for every
+ * velocity v < nvar of AoSoA banks of ioshape ((order+1)^2, nvar, nelx*nely)
+ * on a periodic structured mesh of affine hx x hy quads (element
+ * e = ex + nelx*ey), with upwind common fluxes: replaces the reference's
+ * tflux / intcflux / rusanov / negdivconf kernels and the M0-M3 operator
+ * products (frfs/solvers/dgfs/kernels/{tflux,intcflux,negdivconf}.mako,
+ * rsolvers/rusanov.mako, frfs/solvers/dgfs/elements.py:26-164) for this mesh
+ * class.  ops = {Dx, Dy, M0, M3} from frfs_amd/fr.py (device, row major);
+ * cvx/cvy = velocity components (device, nvar).  order in {1, 2, 3}. */
+int fq_fr_rhs_quad(int32_t order, int32_t nelx, int32_t nely, double hx, double hy, const double *ops,
+                   const double *cvx, const double *cvy, const double *u, double *out, int64_t ldim,
+                   int32_t nvar, void *stream);
+
 /* ---- host-only plan inspection (no GPU needed) ------------------------- */
 
 int fq_plan_create(const fq_params *prm, fq_plan **out);
