@@ -23,11 +23,18 @@ sys.path.insert(0, ROOT)
 
 PEAK = {'fp64': 78.6, 'fp32': 157.3}     # MI355X vector TFLOP/s (published)
 HBM_PEAK = 8000.0                          # GB/s
+# HBM bytes per DOF of the fused product kernel at the 32^3 workload, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (1272 DOFs per launch),
+# scaled by what the same counters read for a 1 GiB device copy (FETCH x2.0,
+# WRITE x1.0); the summary with its command is profiles/k_prod_vss32_traffic.txt.
+K_PROD_TRAFFIC_PER_DOF = {'mixed': 58.42e6, 'fp64': 83.76e6}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--workload', default='vss32', choices=['vss32', 'rhs24'],
+                    help='vss32: the Q(f,f) headline; rhs24: full SSP-RK2 step (FR advection + Q)')
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--ndof', type=int, default=4096)
@@ -118,12 +125,76 @@ def cpu_baseline(a, fpar):
                 wall_s=wall), ref
 
 
+def bench_rhs(a):
+    """Full explicit step: 32x32 periodic quads, p = 2 (9216 DOFs), 24^3 velocity
+    grid on [-5,5]^3, vhs-gll M = 6 (the 12-point design as 6 antipodal pairs),
+    Nrho = 12; SSP-RK2 with FR upwind advection and Q, each step captured once
+    in a HIP graph and replayed.  Initial f: Maxwellian with a seeded
+    sinusoidal shear u_x = 0.05 sin(2 pi y + phase)."""
+    import torch
+    from frfs_amd.backend import HIPBackend
+    from frfs_amd.fr import PeriodicQuadAdvection
+    from frfs_amd.inifile import Inifile
+    from frfs_amd.system import DGFSCollisionSystem, STEPPERS
+    be = HIPBackend()
+    nel, N = 32, 24
+    res = {}
+    for mode in a.modes.split(','):
+        cfg = Inifile.from_dict({
+            'non-dim': dict(T0=273.0, H0=1e-3, rho0=0.00011140244750324529, molarMass0=39.96e-3),
+            'constants': {'Nv': N, 'NvBatchSize': N ** 3, 'Nrho': 12},
+            'velocity-mesh': {'dev': 5.0, 'cmax': 0, 'Tmax': 273},
+            'spherical-design-rule': {'ssrule': 'womersley', 'M': 6},
+            'scattering-model': {'type': 'vhs-gll', 'omega': 0.5, 'dRef': 4.17e-10, 'Tref': 273,
+                                 'precision': 'mixed' if mode == 'mixed' else 'double',
+                                 'conservation-projection': 1 if mode == 'mixed' else 0}})
+        sysm = DGFSCollisionSystem(be, cfg, 9, nel * nel)
+        adv = PeriodicQuadAdvection(be, sysm.vm, nel, nel, 1.0 / nel, 1.0 / nel, order=2)
+        sysm.rhs = lambda t, uin, fout: adv(sysm.banks[uin], sysm.banks[fout])
+        stp = STEPPERS['dgfs-tvd-rk2'](sysm, graph=True)
+        rng = np.random.default_rng(a.seed)
+        phase = rng.uniform(0, 2 * np.pi)
+        ploc = adv.ploc()
+        cv = torch.as_tensor(sysm.vm.cv(), device='cuda')
+        ux = torch.as_tensor(0.05 * np.sin(2 * np.pi * ploc[:, 1, :] + phase), device='cuda')   # (9, neles)
+        f0 = (np.pi ** -1.5) * torch.exp(-((cv[0][None, :, None] - ux[:, None, :]) ** 2
+                                           + cv[1][None, :, None] ** 2 + cv[2][None, :, None] ** 2))
+        sysm.banks[0].set(f0.cpu().numpy())
+        dt = 1e-3
+        for k in range(a.warmup):
+            stp.step(k * dt, dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(a.steps):
+            stp.step(k * dt, dt)
+        torch.cuda.synchronize()
+        res[mode] = (time.perf_counter() - t0) / a.steps
+        del sysm, adv, stp
+        torch.cuda.empty_cache()
+    ndof = nel * nel * 9
+    head = 'mixed' if 'mixed' in res else list(res)[0]
+    out = {'metric': 'full SSP-RK2 step DOF-steps/s (FR advection + Q(f,f)), 24^3 grid, 9216 DOFs',
+           'value': ndof / res[head], 'unit': 'DOF-steps/s', 'n_gpus': 1, 'steps': a.steps,
+           'warmup': a.warmup, 'ms_per_step': res[head] * 1e3, 'higher_is_better': True,
+           'scaling': 'weak', 'vs_baseline': None,
+           'dtype': 'f32+f64acc+proj' if head == 'mixed' else 'f64', 'data': 'synthetic',
+           'config': {'workload': 'dgfs-tvd-rk2 step, 32x32 periodic quads p=2 (9216 DOFs), 24^3 grid on '
+                                  '[-5,5]^3, vhs-gll M=6 (12-point design) Nrho=12, HIP graph per step',
+                      'mode': head},
+           'collision_dof_evals_per_s': 2 * ndof / res[head],
+           'fp64': ({'value': ndof / res['fp64'], 'ms_per_step': res['fp64'] * 1e3} if 'fp64' in res else None)}
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU; --gpus must be 1',
                           'n_gpus_requested': a.gpus}))
         sys.exit(2)
+    if a.workload == 'rhs24':
+        bench_rhs(a)
+        return
     from frfs_amd.inputs import mixture_params
     fpar = mixture_params(a.ndof, a.seed)
     cpu, ref = (None, {}) if a.no_cpu_baseline else cpu_baseline(a, fpar)
@@ -190,6 +261,8 @@ def main():
         'k_fwd_xy': info['n_sources'] * 2 * pencils,
         'k_inv_z': info['n_items'] * pencils,
         'k_plane': (info['n_items'] + info['n_groups']) * 2 * pencils,
+        # fused: z, x and y transforms of every item, x and y of every group
+        'k_prod': (3 * info['n_items'] + 2 * info['n_groups']) * pencils,
         'k_accum': (info['n_contribs'] + 2 * info['n_targets']) * pencils,
         'k_out': info['n_targets'] * 2 * 2 * pencils,
     }
@@ -197,6 +270,9 @@ def main():
     dom_avg_s = dom_ms / dom_n * 1e-3
     achieved = dom_flops_per_launch / dom_avg_s / 1e12
     op_eff = r['rate'] * f_alg / 1e12
+    traffic = None
+    if dom == 'k_prod' and N == 32 and a.Nrho == 16 and a.M == 12:
+        traffic = K_PROD_TRAFFIC_PER_DOF[head] * a.ndof * steps_launch / dom_n
     err = None
     if 'mixed' in qs and 'fp64' in qs:
         d = qs['mixed'] - qs['fp64']
@@ -222,8 +298,11 @@ def main():
         'fp64_vs_oracle_max_rel_l2': oracle_err,
         'plan': info,
         'roofline': {'bound': 'valu', 'kernel': dom, 'achieved': achieved, 'peak': prec_peak,
-                     'unit': 'TFLOP/s', 'frac': achieved / prec_peak, 'traffic': None,
+                     'unit': 'TFLOP/s', 'frac': achieved / prec_peak, 'traffic': traffic,
+                     'traffic_GBps': traffic / dom_avg_s / 1e9 if traffic else None,
+                     'hbm_frac': traffic / dom_avg_s / 1e9 / HBM_PEAK if traffic else None,
                      'avg_launch_ms': dom_avg_s * 1e3, 'launches': dom_n,
+                     'flops_per_launch': dom_flops_per_launch,
                      'operator_effective_tflops': op_eff, 'operator_frac': op_eff / prec_peak,
                      'kernel_ms_per_step': {k: v[0] / a.steps for k, v in tot.items()}},
         'cpu_baseline': cpu,
