@@ -33,6 +33,15 @@ template <typename T> __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b)
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 template <typename T> __device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) { return {a.x * s, a.y * s}; }
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// t += a * b as four fused multiply-adds
+template <typename T> __device__ __forceinline__ void cmac(cpx<T> &t, cpx<T> a, cpx<T> b) {
+    t.x = fmadd(a.x, b.x, t.x);
+    t.x = fmadd(-a.y, b.y, t.x);
+    t.y = fmadd(a.x, b.y, t.y);
+    t.y = fmadd(a.y, b.x, t.y);
+}
 
 // ---------------------------------------------------------------------------
 // compile-time loop
@@ -179,7 +188,8 @@ template <int E, int S> struct LFFT {
 // Lane-pair exchange: value of lane ^ 1 (DPP quad_perm [1,0,3,2]).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int swap_adj_i32(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+    // every lane of the quad is a valid source, so no "old" value is needed
+    return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);
 }
 __device__ __forceinline__ float swap_adj(float v) {
     return __builtin_bit_cast(float, swap_adj_i32(__builtin_bit_cast(int, v)));
@@ -192,6 +202,18 @@ __device__ __forceinline__ double swap_adj(double v) {
     return __builtin_bit_cast(double, r);
 }
 template <typename T> __device__ __forceinline__ cpx<T> swap_adj(cpx<T> v) { return {swap_adj(v.x), swap_adj(v.y)}; }
+// partner's value + (hi ? -v : v).  FP32: a sign select feeding an add whose
+// other operand is the DPP swap, which folds into one v_add_f32_dpp.  FP64
+// has no DPP form of its add, so the sign enters one FMA.
+template <typename T> __device__ __forceinline__ cpx<T> pair_combine(cpx<T> v, bool hi) {
+    const cpx<T> p = swap_adj(v);
+    if constexpr (std::is_same_v<T, float>) {
+        return {p.x + (hi ? -v.x : v.x), p.y + (hi ? -v.y : v.y)};
+    } else {
+        const T sg = hi ? T(-1) : T(1);
+        return {fmadd(sg, v.x, p.x), fmadd(sg, v.y, p.y)};
+    }
+}
 
 // Runtime-selected twiddle for the high lane only: lane half 0 multiplies by 1.
 template <int N, int S, int K, typename T> __device__ __forceinline__ cpx<T> twmul_hi(cpx<T> v, bool hi) {
@@ -203,14 +225,10 @@ template <int N, int S, int K, typename T> __device__ __forceinline__ cpx<T> twm
 template <int N, int S, typename T> __device__ __forceinline__ void fft2_dif(cpx<T> *v, int half) {
     constexpr int E = N / 2;
     const bool hi = half != 0;
-    const T sg = hi ? T(-1) : T(1);
     sfor<0, E>([&](auto bb) {
         constexpr int b = decltype(bb)::value;
-        cpx<T> p = swap_adj(v[b]);
         // half 0: x0 + x1 ; half 1: x0 - x1  (x0 = lo lane's value, x1 = hi lane's)
-        cpx<T> y = {p.x + sg * v[b].x, p.y + sg * v[b].y};
-        // lane 0 holds x0 so y = x0 + x1; lane 1 holds x1 so y = x0 - x1
-        v[b] = twmul_hi<N, S, b>(y, hi);
+        v[b] = twmul_hi<N, S, b>(pair_combine(v[b], hi), hi);
     });
     LFFT<E, S>::run(v);
 }
@@ -219,14 +237,12 @@ template <int N, int S, typename T> __device__ __forceinline__ void fft2_dif(cpx
 template <int N, int S, typename T> __device__ __forceinline__ void fft2_dit(cpx<T> *v, int half) {
     constexpr int E = N / 2;
     const bool hi = half != 0;
-    const T sg = hi ? T(-1) : T(1);
     LFFT<E, S>::run(v);
     sfor<0, E>([&](auto jj) {
         constexpr int j = decltype(jj)::value;
         cpx<T> w = twmul_hi<N, S, j>(v[j], hi);      // hi lane: Y1[j] * w^j
-        cpx<T> p = swap_adj(w);
         // lane 0: Y0 + wY1 ; lane 1: Y0 - wY1
-        v[j] = {p.x + sg * w.x, p.y + sg * w.y};
+        v[j] = pair_combine(w, hi);
     });
 }
 

@@ -64,6 +64,31 @@ constexpr int kPlaneThreads = 128;            // plane kernels (LDS transpose pe
 
 template <typename T> __device__ __forceinline__ cpx<T> ld(const cpx<T> *p) { return *p; }
 
+// Element idx of a workgroup-uniform base.  The byte offset is formed in 32
+// bits, so the load takes the SGPR-base + 32-bit VGPR-offset form instead of a
+// 64-bit per-lane address.
+template <typename T> __device__ __forceinline__ cpx<T> ldu(const cpx<T> *base, uint32_t idx) {
+    return *reinterpret_cast<const cpx<T> *>(reinterpret_cast<const char *>(base) +
+                                             uint32_t(idx * uint32_t(sizeof(cpx<T>))));
+}
+
+// Buffer view of a workgroup-uniform array of `bytes` bytes.  Loads take a
+// per-lane 32-bit byte offset plus a uniform byte offset held in an SGPR, so
+// a run of loads at constant strides shares one address register.  An offset
+// past the end reads zero (buffer range check) instead of faulting.
+struct BufView {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ BufView(const void *base, uint32_t bytes)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, int(bytes), 0x00020000)) {}
+    template <typename T> __device__ __forceinline__ cpx<T> ld(uint32_t voff, uint32_t soff) const;
+};
+template <> __device__ __forceinline__ cpx<float> BufView::ld<float>(uint32_t voff, uint32_t soff) const {
+    return __builtin_bit_cast(cpx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, int(voff), int(soff), 0));
+}
+template <> __device__ __forceinline__ cpx<double> BufView::ld<double>(uint32_t voff, uint32_t soff) const {
+    return __builtin_bit_cast(cpx<double>, __builtin_amdgcn_raw_buffer_load_b128(r, int(voff), int(soff), 0));
+}
+
 template <typename To, typename Ti> __device__ __forceinline__ cpx<To> cvt(cpx<Ti> v) {
     return {To(v.x), To(v.y)};
 }
@@ -383,9 +408,8 @@ __global__ __launch_bounds__(kPlaneThreads) void k_plane(const cpx<T> *Z, cpx<T>
 // (w = e^{+2 pi i/.}) is computed from the L2-resident source volume
 // straight into LDS, then wave k runs the x,y transforms of plane k, the
 // product and the group sum; the group's P planes are written once.  This
-// removes the round trip of every Z_i volume through HBM.  SEQ = true
-// processes the two items of a product one after the other (half the LDS,
-// used for FP64).
+// removes the round trip of every Z_i volume through HBM.  The two items of
+// a product pass through the same LDS planes one after the other.
 // ---------------------------------------------------------------------------
 template <int N> struct ProdGeo {
     static constexpr int S = N / 4;
@@ -408,30 +432,40 @@ __device__ __forceinline__ void zstage(const cpx<T> *__restrict__ src, const cpx
                                        const cpx<T> *__restrict__ ph, cpx<T> *dst) {
     using PG = ProdGeo<N>;
     constexpr int S = PG::S;
+    constexpr int ITER = (N * N + PG::THREADS - 1) / PG::THREADS;
+    // d_0 = c_0 = 1 exactly (zero angle), so those products are skipped
     cpx<T> d[4], cc[S];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) d[b] = zc[b];
+    for (int b = 1; b < 4; ++b) d[b] = zc[b];
 #pragma unroll
-    for (int a = 0; a < S; ++a) cc[a] = zc[4 + a];
-    for (int p = threadIdx.x; p < N * N; p += PG::THREADS) {
-        const uint32_t I = uint32_t(p) / N, J = uint32_t(p) % N;
-        const uint32_t off = uint32_t(p);
+    for (int a = 1; a < S; ++a) cc[a] = zc[4 + a];
+    constexpr uint32_t PB = uint32_t(N * N * sizeof(cpx<T>));     // plane bytes
+    const BufView sv(src, uint32_t(N) * PB);
+#pragma unroll 1
+    for (int it = 0; it < ITER; ++it) {
+        const uint32_t p = threadIdx.x + uint32_t(it * PG::THREADS);
+        if (ITER * PG::THREADS != N * N && p >= uint32_t(N * N)) break;
+        const uint32_t I = p / N, J = p % N;
+        const uint32_t vo = p * uint32_t(sizeof(cpx<T>));
+        // the 4S loads go out in two batches of 2S, each issued before its first use
         cpx<T> t[S];
 #pragma unroll
-        for (int a = 0; a < S; ++a) t[a] = {0, 0};
+        for (int h2 = 0; h2 < 2; ++h2) {
+            cpx<T> x[2][S];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            cpx<T> x[S];
+            for (int bb = 0; bb < 2; ++bb) {
 #pragma unroll
-            for (int a = 0; a < S; ++a) x[a] = src[uint32_t((a + S * b) * N * N) + off];
+                for (int a = 0; a < S; ++a) x[bb][a] = sv.ld<T>(vo, uint32_t(a + S * (2 * h2 + bb)) * PB);
+            }
 #pragma unroll
             for (int a = 0; a < S; ++a) {
-                const cpx<T> y = fq::cmul(x[a], d[b]);
-                t[a] = t[a] + y;
+                if (h2 == 0) t[a] = x[0][a];
+                else fq::cmac(t[a], x[0][a], d[2]);
+                fq::cmac(t[a], x[1][a], d[2 * h2 + 1]);
             }
         }
 #pragma unroll
-        for (int a = 0; a < S; ++a) t[a] = fq::cmul(t[a], cc[a]);
+        for (int a = 1; a < S; ++a) t[a] = fq::cmul(t[a], cc[a]);
         fq::LFFT<S, 1>::run(t);
         const cpx<T> pxy = fq::cmul(ph[I], ph[N + J]);
         cpx<T> *dd = dst + I * PG::ROW + J;
@@ -461,16 +495,16 @@ __device__ __forceinline__ void plane_inv_lds(cpx<T> *Lp, cpx<T> *v, int c, int 
     fq::fft2_dif<N, 1>(v, half);
 }
 
-template <int N, typename T, bool SEQ, bool MULTI>
-__global__ __launch_bounds__(ProdGeo<N>::THREADS, (sizeof(T) == 4 ? 4 : 2)) void k_prod(const cpx<T> *FT, cpx<T> *P, const cpx<T> *ph,
-                                                              const cpx<T> *zc, const DevProd *prods,
-                                                              const int *g_p0, const int *g_np, const int *item_src,
-                                                              int ngroups, int nsrc) {
+template <int N, typename T, bool MULTI>
+__global__ __launch_bounds__(ProdGeo<N>::THREADS, (sizeof(T) == 4 ? 4 : 2)) void k_prod(const cpx<T> *__restrict__ FT, cpx<T> *__restrict__ P, const cpx<T> *__restrict__ ph,
+                                                              const cpx<T> *__restrict__ zc, const DevProd *__restrict__ prods,
+                                                              const int *__restrict__ g_p0, const int *__restrict__ g_np,
+                                                              const int *__restrict__ item_src, int ngroups, int nsrc) {
     using PG = ProdGeo<N>;
     using G3 = Geo<N>;
     constexpr int E = PG::E;
     constexpr int ZC = 4 + PG::S;
-    __shared__ cpx<T> lds[(SEQ ? 1 : 2) * PG::S * PG::PLANE];
+    __shared__ cpx<T> lds[PG::S * PG::PLANE];
     const int64_t unit = blockIdx.x;
     const int G = int(unit % 4);
     const int g = int((unit / 4) % ngroups);
@@ -481,7 +515,6 @@ __global__ __launch_bounds__(ProdGeo<N>::THREADS, (sizeof(T) == 4 ? 4 : 2)) void
     const bool act = c < N;
     const int vz = G + 4 * k;
     cpx<T> *L1 = lds + k * PG::PLANE;
-    cpx<T> *L2 = SEQ ? L1 : lds + (PG::S + k) * PG::PLANE;
     cpx<T> acc[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) acc[e] = {0, 0};
@@ -495,23 +528,21 @@ __global__ __launch_bounds__(ProdGeo<N>::THREADS, (sizeof(T) == 4 ? 4 : 2)) void
         cpx<T> e1[E], e2[E];
         __syncthreads();                                   // LDS free
         zstage<N, T>(src1, zc1, ph1, lds);
-        if constexpr (!SEQ) zstage<N, T>(src2, zc2, ph2, lds + PG::S * PG::PLANE);
         __syncthreads();
         plane_inv_lds<N, T>(L1, e1, c, half, act);
-        if constexpr (SEQ) {
-            __syncthreads();
-            zstage<N, T>(src2, zc2, ph2, lds);
-            __syncthreads();
-        }
-        plane_inv_lds<N, T>(L2, e2, c, half, act);
+        __syncthreads();
+        zstage<N, T>(src2, zc2, ph2, lds);
+        __syncthreads();
+        plane_inv_lds<N, T>(L1, e2, c, half, act);
         const T cf = T(pr.coef);
         if constexpr (MULTI) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                cpx<T> pe = fq::cmul(e1[e], e2[e]);
-                acc[e] = {acc[e].x + cf * pe.x, acc[e].y + cf * pe.y};
+                const cpx<T> pe = fq::cmul(e1[e], e2[e]);
+                acc[e] = {fq::fmadd(cf, pe.x, acc[e].x), fq::fmadd(cf, pe.y, acc[e].y)};
             }
         } else {
+            // one product per group: acc is not live across the item stages
 #pragma unroll
             for (int e = 0; e < E; ++e) acc[e] = fq::cscale(fq::cmul(e1[e], e2[e]), cf);
         }
@@ -540,12 +571,16 @@ __global__ __launch_bounds__(256) void k_cvt(const cpx<double> *F, cpx<float> *F
     if (i < n) F32[i] = cvt<float>(F[i]);
 }
 
+// Minimum waves per SIMD requested for k_accum: three for FP32 transforms
+// (the 24-point radix-3 variant needs more registers and keeps two), two for FP64.
+constexpr int accum_waves(int N, size_t tsize) { return tsize == 4 && N != 24 ? 3 : 2; }
+
 // ---------------------------------------------------------------------------
 // k_accum: unit = (slot, target, I').  H = sum_c W_c . DFT-_z(P_g(c)),
 // L = sum_j Lt_j . F_src(j); Y = DFT+_z of H and L.  FP64 accumulation.
 // ---------------------------------------------------------------------------
 template <int N, typename T>
-__global__ __launch_bounds__(256) void k_accum(const cpx<T> *P, const cpx<double> *F,
+__global__ __launch_bounds__(256, accum_waves(N, sizeof(T))) void k_accum(const cpx<T> *P, const cpx<double> *F,
                                                const cpx<double> *W, const int *c_group,
                                                const int *t_c0, const int *t_cn, const int *l_src,
                                                const int *t_l0, const int *t_ln, int ncontribs,
@@ -554,7 +589,11 @@ __global__ __launch_bounds__(256) void k_accum(const cpx<T> *P, const cpx<double
     using G = Geo<N>;
     constexpr int E = G::E;
     const int lt = threadIdx.x % G::LPP;
-    const int64_t unit = int64_t(blockIdx.x) * (blockDim.x / G::LPP) + threadIdx.x / G::LPP;
+    int64_t unit = int64_t(blockIdx.x) * (blockDim.x / G::LPP) + threadIdx.x / G::LPP;
+    // With 64 lanes per unit a unit is one wave: its indices are wave-uniform
+    // and live in SGPRs, and the pencil loads below use buffer views.
+    constexpr bool UNI = G::LPP == 64;
+    if constexpr (UNI) unit = __builtin_amdgcn_readfirstlane(int(unit));
     if (unit >= nunits) return;
     // slot fastest: neighbouring units read the same weight rows W[c][.][I]
     const int slot = int(unit % D);
@@ -569,30 +608,54 @@ __global__ __launch_bounds__(256) void k_accum(const cpx<T> *P, const cpx<double
     for (int e = 0; e < E; ++e) H[e] = {0, 0};
     const int c0 = t_c0[t], cn = t_cn[t];
     const cpx<T> *pbase = P + int64_t(slot) * ngroups * G::VOL + off;
-    // software pipeline: the pencil of contribution ci+1 is in flight while
-    // contribution ci is transformed and accumulated
-    cpx<T> nxt[E];
-    if (cn > 0) {
-        const cpx<T> *src = pbase + int64_t(c_group[c0]) * G::VOL;
+    // lane byte offsets inside one volume (P pencil element half + 2e, W element half*E + e)
+    const uint32_t voP = uint32_t(off + int64_t(half) * G::N2) * uint32_t(sizeof(cpx<T>));
+    const uint32_t voW = uint32_t(off + int64_t(half) * E * G::N2) * uint32_t(sizeof(cpx<double>));
+    auto load_pencil = [&](int ci, cpx<T> *dst) {
+        if constexpr (UNI) {
+            const BufView pv(P + (int64_t(slot) * ngroups + c_group[ci]) * G::VOL, uint32_t(G::VOL * sizeof(cpx<T>)));
 #pragma unroll
-        for (int e = 0; e < E; ++e) nxt[e] = act ? src[int64_t(half + 2 * e) * G::N2] : cpx<T>{0, 0};
+            for (int e = 0; e < E; ++e)
+                dst[e] = act ? pv.ld<T>(voP, uint32_t(2 * e * G::N2 * sizeof(cpx<T>))) : cpx<T>{0, 0};
+        } else {
+            const cpx<T> *src = pbase + int64_t(c_group[ci]) * G::VOL;
+#pragma unroll
+            for (int e = 0; e < E; ++e) dst[e] = act ? src[int64_t(half + 2 * e) * G::N2] : cpx<T>{0, 0};
+        }
+    };
+    // FP32 pencils: software pipeline, the pencil of contribution ci+1 is in
+    // flight while contribution ci is transformed and accumulated.  FP64
+    // pencils are loaded in place (the register budget has no room for both).
+    constexpr bool PF = sizeof(T) == 4;
+    cpx<T> nxt[PF ? E : 1];
+    if constexpr (PF) {
+        if (cn > 0) load_pencil(c0, nxt);
     }
     for (int ci = c0; ci < c0 + cn; ++ci) {
         cpx<T> v[E];
+        if constexpr (PF) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = nxt[e];
-        if (ci + 1 < c0 + cn) {
-            const cpx<T> *src = pbase + int64_t(c_group[ci + 1]) * G::VOL;
-#pragma unroll
-            for (int e = 0; e < E; ++e) nxt[e] = act ? src[int64_t(half + 2 * e) * G::N2] : cpx<T>{0, 0};
+            for (int e = 0; e < E; ++e) v[e] = nxt[e];
+            if (ci + 1 < c0 + cn) load_pencil(ci + 1, nxt);
+        } else {
+            load_pencil(ci, v);
         }
         fq::fft2_dit<N, -1>(v, half);             // -> K' = half*E + e
-        const cpx<double> *w = W + int64_t(ci) * G::VOL + off;
+        if constexpr (UNI) {
+            const BufView wv(W + int64_t(ci) * G::VOL, uint32_t(G::VOL * sizeof(cpx<double>)));
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const cpx<double> we = act ? w[int64_t(half * E + e) * G::N2] : cpx<double>{0, 0};
-            const cpx<double> pe = cvt<double>(v[e]);
-            H[e] = H[e] + fq::cmul(we, pe);
+            for (int e = 0; e < E; ++e) {
+                const cpx<double> we =
+                    act ? wv.ld<double>(voW, uint32_t(e * G::N2 * sizeof(cpx<double>))) : cpx<double>{0, 0};
+                fq::cmac(H[e], we, cvt<double>(v[e]));
+            }
+        } else {
+            const cpx<double> *w = W + int64_t(ci) * G::VOL + off;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const cpx<double> we = act ? w[int64_t(half * E + e) * G::N2] : cpx<double>{0, 0};
+                fq::cmac(H[e], we, cvt<double>(v[e]));
+            }
         }
     }
     fq::fft2_dif<N, 1>(H, half);                  // -> vz = half + 2e
@@ -962,15 +1025,15 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
             FT = reinterpret_cast<const cpx<T> *>(h->F32);
         }
         pre();
-        bool multi = false;
-        for (int g = 0; g < ngroups; ++g) multi |= P.group_np[g] > 1;
         const cpx<T> *zc = reinterpret_cast<const cpx<T> *>(h->zc);
         const dim3 grid(unsigned(int64_t(D) * ngroups * 4)), blk(ProdGeo<N>::THREADS);
+        bool multi = false;
+        for (int g = 0; g < ngroups; ++g) multi |= P.group_np[g] > 1;
         if (multi)
-            hipLaunchKernelGGL((k_prod<N, T, true, true>), grid, blk, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0,
+            hipLaunchKernelGGL((k_prod<N, T, true>), grid, blk, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0,
                                h->g_np, h->item_src, ngroups, nsrc);
         else
-            hipLaunchKernelGGL((k_prod<N, T, true, false>), grid, blk, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0,
+            hipLaunchKernelGGL((k_prod<N, T, false>), grid, blk, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0,
                                h->g_np, h->item_src, ngroups, nsrc);
         post(7);
     } else {
