@@ -33,8 +33,9 @@ K_PROD_TRAFFIC_PER_DOF = {'mixed': 58.42e6, 'fp64': 83.76e6}
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--workload', default='vss32', choices=['vss32', 'rhs24'],
-                    help='vss32: the Q(f,f) headline; rhs24: full SSP-RK2 step (FR advection + Q)')
+    ap.add_argument('--workload', default='vss32', choices=['vss32', 'vss32x2', 'rhs24'],
+                    help='vss32: the Q(f,f) headline; vss32x2: two species, four pairs; '
+                         'rhs24: full SSP-RK2 step (FR advection + Q)')
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--ndof', type=int, default=4096)
@@ -46,13 +47,15 @@ def parse():
     ap.add_argument('--cpu-dofs-per-proc', type=int, default=2)
     ap.add_argument('--cpu-procs', type=int, default=0, help='0 = min(cpus, 16)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--modes', default='fp64,mixed')
+    ap.add_argument('--no-reference-proj', action='store_true',
+                    help='skip the untimed FP64-with-projection reference evaluation')
+    ap.add_argument('--modes', default='fp64,mixed,mixedproj')
     ap.add_argument('--calib-copy-mib', type=int, default=0,
                     help='run one device-to-device copy of this size first (counter calibration)')
     return ap.parse_args()
 
 
-def vss_cfg(a, precision):
+def vss_cfg(a, precision, project=0):
     from frfs_amd.inifile import Inifile
     return Inifile.from_dict({
         'non-dim': dict(T0=273.0, H0=1e-3, rho0=0.00011140244750324529, molarMass0=39.96e-3),
@@ -61,7 +64,8 @@ def vss_cfg(a, precision):
         'spherical-design-rule': {'ssrule': 'womersley', 'M': a.M},
         'scattering-model': {'type': 'vss-gll', 'ssrulepre': 'womersley', 'Mpre': a.Mpre,
                              'alpha00': 1.4, 'omega00': 0.74, 'dRef00': 4.17e-10,
-                             'Tref00': 273, 'precision': precision},
+                             'Tref00': 273, 'precision': precision,
+                             'conservation-projection': project},
     })
 
 
@@ -125,6 +129,91 @@ def cpu_baseline(a, fpar):
                 wall_s=wall), ref
 
 
+def bench_multi(a):
+    """Two species (molar masses 1:2), all four pairs (p, q) in one batched
+    launch: vss-gll, 32^3 grid on [-6,6]^3, M = 12, Nrho = 16, Mpre = 32;
+    omega 0.74 (00) and 0.81 (11), 0.775 across; dRef 4.17e-10 (00),
+    1.3 x that (11), the mean across; alpha = 1.4.  2048 DOFs per species:
+    the seeded two-Maxwellian mixture of the single-species workload, with
+    species 1 taking a share x ~ U(0.2, 0.8) of the number density.  One
+    DOF-eval is Q_p for one species at one DOF, all its pairs included."""
+    import torch
+    from frfs_amd.backend import HIPBackend
+    from frfs_amd.inifile import Inifile
+    from frfs_amd.inputs import from_aosoa, mixture_device, mixture_params, to_aosoa
+    from frfs_amd.scattering import DGFSBiVSSGLLScatteringModel
+    from frfs_amd.velocitymesh import DGFSBiVelocityMesh
+    be = HIPBackend()
+    ndof, N = a.ndof // 2, a.Nv
+    fpar = mixture_params(ndof, a.seed)
+    share = np.random.default_rng(a.seed + 1).uniform(0.2, 0.8, ndof)
+    omega = {'00': 0.74, '11': 0.81, '01': 0.775, '10': 0.775}
+    dref = {'00': 4.17e-10, '11': 1.3 * 4.17e-10, '01': 1.15 * 4.17e-10, '10': 1.15 * 4.17e-10}
+    res, qs = {}, {}
+    modes = [m for m in a.modes.split(',') if m in ('fp64', 'mixed')]
+    runs = [(m, m == 'mixed', True) for m in modes]
+    if 'mixed' in modes:
+        runs.append(('fp64proj', True, False))     # accuracy reference only, untimed
+    for mode, proj, timed in runs:
+        sm = {'type': 'vss-gll', 'ssrulepre': 'womersley', 'Mpre': a.Mpre,
+              'precision': 'mixed' if mode == 'mixed' else 'double',
+              'conservation-projection': 1 if proj else 0}
+        for pq in omega:
+            sm.update({'alpha' + pq: 1.4, 'omega' + pq: omega[pq], 'dRef' + pq: dref[pq], 'Tref' + pq: 273})
+        cfg = Inifile.from_dict({
+            'non-dim': dict(T0=273.0, H0=1e-3, rho0=0.00011140244750324529, molarMass0=39.96e-3),
+            'constants': {'Nv': N, 'NvBatchSize': N ** 3, 'Nrho': a.Nrho},
+            'velocity-mesh': {'dev': 6.0, 'cmax': 0, 'Tmax': 273, 'molarMass1': 39.96e-3,
+                              'molarMass2': 2 * 39.96e-3},
+            'spherical-design-rule': {'ssrule': 'womersley', 'M': a.M},
+            'scattering-model': sm})
+        vm = DGFSBiVelocityMesh(be, cfg, 2)
+        model = DGFSBiVSSGLLScatteringModel(be, cfg, vm)
+        n = vm.vsize()
+        fin = [be.matrix((1, n, ndof)) for _ in range(2)]
+        qout = [be.matrix((1, n, ndof)) for _ in range(2)]
+        sh = torch.as_tensor(share, device='cuda')[:, None]
+        for s, m in enumerate(vm.masses()):
+            f = mixture_device(vm.cv(), fpar, mass=m) * (sh if s == 1 else 1 - sh)
+            to_aosoa(f, fin[s])
+        torch.cuda.synchronize()
+        for _ in range(a.warmup if timed else 1):
+            model.fs_batch(*fin, *qout)
+        torch.cuda.synchronize()
+        if timed:
+            t0 = time.perf_counter()
+            for _ in range(a.steps):
+                model.fs_batch(*fin, *qout)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / a.steps
+            res[mode] = dict(ms=dt * 1e3, rate=2 * ndof / dt, info=model.plan_info())
+        qs[mode] = [from_aosoa(q).cpu().numpy() for q in qout]
+        del model, fin, qout
+        torch.cuda.empty_cache()
+    head = 'mixed' if 'mixed' in res else list(res)[0]
+
+    def rel(x, y):
+        return [float(np.linalg.norm(x[s] - y[s]) / np.linalg.norm(y[s])) for s in range(2)]
+    err = None
+    if 'mixed' in qs and 'fp64' in qs:
+        err = {'mixed_proj_vs_fp64_proj': rel(qs['mixed'], qs['fp64proj']),
+               'mixed_proj_vs_fp64': rel(qs['mixed'], qs['fp64']),
+               'fp64_proj_vs_fp64 (projection correction)': rel(qs['fp64proj'], qs['fp64'])}
+    out = {'metric': 'two-species Q_p DOF-evals/s (4 pairs in one launch), 32^3 grid',
+           'value': res[head]['rate'], 'unit': 'DOF-evals/s', 'n_gpus': 1, 'steps': a.steps,
+           'warmup': a.warmup, 'ms_per_step': res[head]['ms'], 'higher_is_better': True,
+           'scaling': 'weak', 'vs_baseline': None,
+           'dtype': 'f32+f64acc' if head == 'mixed' else 'f64', 'data': 'synthetic',
+           'config': {'workload': f'vss-gll two species, masses 1:2, {N}^3 grid on [-6,6]^3, M={a.M}, '
+                                  f'Nrho={a.Nrho}, Mpre={a.Mpre}, {ndof} DOFs per species, species-1 '
+                                  f'density share U(0.2,0.8)', 'mode': head, 'ndof_per_species': ndof},
+           'plan': res[head]['info'],
+           'fp64': ({'value': res['fp64']['rate'], 'ms_per_step': res['fp64']['ms']} if 'fp64' in res else None),
+           'speedup_mixed_vs_fp64': (res['mixed']['rate'] / res['fp64']['rate']) if len(res) == 2 else None,
+           'rel_l2_per_species': err}
+    print(json.dumps(out))
+
+
 def bench_rhs(a):
     """Full explicit step: 32x32 periodic quads, p = 2 (9216 DOFs), 24^3 velocity
     grid on [-5,5]^3, vhs-gll M = 6 (the 12-point design as 6 antipodal pairs),
@@ -139,7 +228,7 @@ def bench_rhs(a):
     be = HIPBackend()
     nel, N = 32, 24
     res = {}
-    for mode in a.modes.split(','):
+    for mode in [m for m in a.modes.split(',') if m in ('fp64', 'mixed')]:
         cfg = Inifile.from_dict({
             'non-dim': dict(T0=273.0, H0=1e-3, rho0=0.00011140244750324529, molarMass0=39.96e-3),
             'constants': {'Nv': N, 'NvBatchSize': N ** 3, 'Nrho': 12},
@@ -177,7 +266,7 @@ def bench_rhs(a):
            'value': ndof / res[head], 'unit': 'DOF-steps/s', 'n_gpus': 1, 'steps': a.steps,
            'warmup': a.warmup, 'ms_per_step': res[head] * 1e3, 'higher_is_better': True,
            'scaling': 'weak', 'vs_baseline': None,
-           'dtype': 'f32+f64acc+proj' if head == 'mixed' else 'f64', 'data': 'synthetic',
+           'dtype': 'f32+f64acc' if head == 'mixed' else 'f64', 'data': 'synthetic',
            'config': {'workload': 'dgfs-tvd-rk2 step, 32x32 periodic quads p=2 (9216 DOFs), 24^3 grid on '
                                   '[-5,5]^3, vhs-gll M=6 (12-point design) Nrho=12, HIP graph per step',
                       'mode': head},
@@ -194,6 +283,9 @@ def main():
         sys.exit(2)
     if a.workload == 'rhs24':
         bench_rhs(a)
+        return
+    if a.workload == 'vss32x2':
+        bench_multi(a)
         return
     from frfs_amd.inputs import mixture_params
     fpar = mixture_params(a.ndof, a.seed)
@@ -215,15 +307,29 @@ def main():
     modes = a.modes.split(',')
     res = {}
     qs = {}
-    for mode in modes:
-        cfg = vss_cfg(a, 'mixed' if mode == 'mixed' else 'double')
+    # fp64: plain FP64; mixed: FP32 transforms and products, FP64 weight
+    # accumulation; mixedproj: mixed plus the FP64 conservation projection.
+    # FP64 with the projection is evaluated once (untimed) so that rounding
+    # and the projection's own correction are reported separately.
+    runs = [(m, m == 'mixedproj', True) for m in modes]
+    if 'mixedproj' in modes and not a.no_reference_proj:
+        runs.append(('fp64proj', True, False))
+    for mode, proj, timed in runs:
+        cfg = vss_cfg(a, 'mixed' if mode.startswith('mixed') else 'double', int(proj))
         vm = DGFSBiVelocityMesh(be, cfg, 1)
         sm = DGFSBiVSSGLLScatteringModel(be, cfg, vm)
         n = vm.vsize()
+        vm_cv = vm.cv()
         fin = be.matrix((1, n, a.ndof))
         qout = be.matrix((1, n, a.ndof))
         to_aosoa(mixture_device(vm.cv(), fpar), fin)
         torch.cuda.synchronize()
+        if not timed:
+            sm.fs_batch(fin, qout)
+            qs[mode] = from_aosoa(qout).cpu().numpy()
+            del sm, fin, qout
+            torch.cuda.empty_cache()
+            continue
         for _ in range(a.warmup):
             sm.fs_batch(fin, qout)
         torch.cuda.synchronize()
@@ -248,7 +354,7 @@ def main():
     f_alg = info['n_fft_ref'] * 5 * n * math.log2(n)          # nominal flops per DOF-eval
     head = 'mixed' if 'mixed' in res else modes[0]
     r = res[head]
-    prec_peak = PEAK['fp32'] if head == 'mixed' else PEAK['fp64']
+    prec_peak = PEAK['fp32'] if head.startswith('mixed') else PEAK['fp64']
     # dominant kernel: performed 1D FFT work per launch (5 N log2 N per length-N FFT)
     tot = {k: v for k, v in r['prof'].items() if v[1] > 0}
     dom = max(tot, key=lambda k: tot[k][0])
@@ -272,12 +378,32 @@ def main():
     op_eff = r['rate'] * f_alg / 1e12
     traffic = None
     if dom == 'k_prod' and N == 32 and a.Nrho == 16 and a.M == 12:
-        traffic = K_PROD_TRAFFIC_PER_DOF[head] * a.ndof * steps_launch / dom_n
+        traffic = K_PROD_TRAFFIC_PER_DOF['mixed' if head.startswith('mixed') else 'fp64'] * a.ndof * steps_launch / dom_n
     err = None
+
+    def relerr(x, y):
+        d = x - y
+        return dict(rel_l2=float(np.linalg.norm(d) / np.linalg.norm(y)),
+                    max_dof_rel_l2=float(np.max(np.linalg.norm(d, axis=1) / np.linalg.norm(y, axis=1))))
+    proj = None
     if 'mixed' in qs and 'fp64' in qs:
-        d = qs['mixed'] - qs['fp64']
-        err = dict(rel_l2=float(np.linalg.norm(d) / np.linalg.norm(qs['fp64'])),
-                   max_dof_rel_l2=float(np.max(np.linalg.norm(d, axis=1) / np.linalg.norm(qs['fp64'], axis=1))))
+        err = relerr(qs['mixed'], qs['fp64'])
+    # collision invariants per DOF: max |sum_v Q phi| / sum_v |Q phi| over the batch
+    cv = np.asarray(vm_cv)
+    phi = np.stack([np.ones(cv.shape[1]), cv[0], cv[1], cv[2], (cv ** 2).sum(0)])
+
+    def defect(q):
+        return float(np.max(np.abs(q @ phi.T) / (np.abs(q) @ np.abs(phi.T))))
+    if 'mixedproj' in res:
+        proj = {'value': res['mixedproj']['rate'], 'ms_per_step': res['mixedproj']['ms'],
+                'moment_defect': defect(qs['mixedproj'])}
+        if 'fp64' in qs:
+            proj['moment_defect_fp64'] = defect(qs['fp64'])
+            proj['vs_fp64'] = relerr(qs['mixedproj'], qs['fp64'])
+        if 'fp64proj' in qs:
+            proj['vs_fp64_with_projection'] = relerr(qs['mixedproj'], qs['fp64proj'])
+            if 'fp64' in qs:
+                proj['projection_correction'] = relerr(qs['fp64proj'], qs['fp64'])
     oracle_err = None
     if ref and 'fp64' in qs:
         ds = sorted(ref)
@@ -293,8 +419,10 @@ def main():
                    'mode': head, 'ndof': a.ndof, 'Nv': N, 'M': a.M, 'Nrho': a.Nrho, 'Mpre': a.Mpre,
                    'seed': a.seed, 'chunk_dofs': r['chunk'], 'scratch_GiB': r['scratch'] / 2 ** 30},
         'fp64': {'value': res['fp64']['rate'], 'ms_per_step': res['fp64']['ms']} if 'fp64' in res else None,
-        'speedup_mixed_vs_fp64': (res['mixed']['rate'] / res['fp64']['rate']) if len(res) == 2 else None,
+        'speedup_mixed_vs_fp64': (res['mixed']['rate'] / res['fp64']['rate'])
+        if 'mixed' in res and 'fp64' in res else None,
         'mixed_vs_fp64_error': err,
+        'mixed_with_projection': proj,
         'fp64_vs_oracle_max_rel_l2': oracle_err,
         'plan': info,
         'roofline': {'bound': 'valu', 'kernel': dom, 'achieved': achieved, 'peak': prec_peak,

@@ -746,12 +746,19 @@ __global__ __launch_bounds__(kPlaneThreads) void k_out(const cpx<double> *Y, Aos
 }
 
 // ---------------------------------------------------------------------------
-// Conservation projection (FP64): Q <- Q - sum_k lambda_k phi_k with
-// phi = {1, v_x, v_y, v_z, |v|^2}, lambda = Gram^{-1} (sum_v Q phi cw):
-// the least-squares correction that zeroes the discrete mass, momentum and
-// energy moments (invariants of frfs/plugins/dgfsmomwriterstd.py:142-214).
-// One block per (slot, target).
+// Conservation projection (FP64).  With S species of mass ratios m_s the
+// collision invariants are the mass of every species and the total momentum
+// and energy, sum_s m_s {v, |v|^2} Q_s (momentum and energy move between
+// species; invariants of frfs/plugins/dgfsmomwriterstd.py:142-214).  The
+// constraint rows a = 0..S+3 are
+//   phi_s(s', v) = [s = s'],  phi_{S+d}(s', v) = m_s' v_d,  phi_{S+3}(s', v) = m_s' |v|^2,
+// and Q <- Q - sum_a lambda_a phi_a with lambda = G^{-1} (A Q),
+// G = A A^T: the least-squares correction that zeroes every invariant.  For
+// one species this is the projection onto the complement of {1, v, |v|^2}.
+// One block per slot, all species of the slot together.
 // ---------------------------------------------------------------------------
+constexpr int kProjMaxRows = 4 + 4;            // up to 4 species
+
 __device__ __forceinline__ void phis(int64_t v, int N, double L, double *ph) {
     const int I = int(v / (int64_t(N) * N)), J = int((v / N) % N), K = int(v % N);
     const double h = 2.0 * L / N;
@@ -759,45 +766,57 @@ __device__ __forceinline__ void phis(int64_t v, int N, double L, double *ph) {
     ph[0] = 1; ph[1] = cx; ph[2] = cy; ph[3] = cz; ph[4] = cx * cx + cy * cy + cz * cz;
 }
 
-__global__ __launch_bounds__(256) void k_project(AosoaPtrs ptr, Dof dof, int ntgt, int N, double L,
-                                                 const double *ginv) {
-    const int slot = blockIdx.x / ntgt, t = blockIdx.x % ntgt;
-    double *q = ptr.q[t];
+__global__ __launch_bounds__(256) void k_project(AosoaPtrs ptr, Dof dof, int ns, int N, double L,
+                                                 const double *ginv, const double *mass) {
+    const int slot = blockIdx.x;
     const int64_t b = dof.base(slot);
     const int64_t n = int64_t(N) * N * N;
-    double m[5] = {0, 0, 0, 0, 0};
-    for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
-        double ph[5];
-        phis(v, N, L, ph);
-        const double qv = q[b + v * dof.soasz];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) m[k] += qv * ph[k];
-    }
+    const int K = ns + 4;
     __shared__ double red[5][256];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = m[k];
+    __shared__ double aq[kProjMaxRows];
+    if (threadIdx.x < kProjMaxRows) aq[threadIdx.x] = 0;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
+    for (int s = 0; s < ns; ++s) {
+        const double *q = ptr.q[s];
+        double m[5] = {0, 0, 0, 0, 0};
+        for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
+            double ph[5];
+            phis(v, N, L, ph);
+            const double qv = q[b + v * dof.soasz];
 #pragma unroll
-            for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+            for (int k = 0; k < 5; ++k) m[k] += qv * ph[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = m[k];
+        __syncthreads();
+        for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+            if (threadIdx.x < st)
+#pragma unroll
+                for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            aq[s] = red[0][0];
+            for (int k = 1; k < 5; ++k) aq[ns + k - 1] += mass[s] * red[k][0];
+        }
         __syncthreads();
     }
-    double lam[5];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) {
-        double s = 0;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) s += ginv[i * 5 + k] * red[k][0];
-        lam[i] = s;
+    double lam[kProjMaxRows];
+    for (int i = 0; i < K; ++i) {
+        double acc = 0;
+        for (int k = 0; k < K; ++k) acc += ginv[i * K + k] * aq[k];
+        lam[i] = acc;
     }
-    for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
-        double ph[5];
-        phis(v, N, L, ph);
-        double corr = 0;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) corr += lam[k] * ph[k];
-        q[b + v * dof.soasz] -= corr;
+    for (int s = 0; s < ns; ++s) {
+        double *q = ptr.q[s];
+        const double ms = mass[s];
+        for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
+            double ph[5];
+            phis(v, N, L, ph);
+            const double corr = lam[s] + ms * (lam[ns] * ph[1] + lam[ns + 1] * ph[2] + lam[ns + 2] * ph[3] +
+                                               lam[ns + 3] * ph[4]);
+            q[b + v * dof.soasz] -= corr;
+        }
     }
 }
 
@@ -834,7 +853,8 @@ struct fq_handle {
     int *c_group = nullptr, *t_c0 = nullptr, *t_cn = nullptr;
     int *l_src = nullptr, *t_l0 = nullptr, *t_ln = nullptr;
     cpx<double> *W = nullptr;    // [contribs + loss][N^3]
-    double *ginv = nullptr;      // [25]
+    double *ginv = nullptr;      // [(ns+4)^2] inverse Gram matrix of the conservation rows
+    double *pmass = nullptr;     // [ns] species mass ratios used by the projection
     void *zc = nullptr;          // [items][4][4 + N/4] k_prod z-stage coefficients in T
     bool fused = true;           // k_prod path (else k_inv_z + k_plane)
     cpx<float> *F32 = nullptr;   // FP32 copy of F (mixed, fused)
@@ -941,11 +961,16 @@ int build_tables(fq_handle *h) {
 }
 
 int build_projection(fq_handle *h) {
-    // Gram matrix of {1, v, |v|^2} over the velocity grid (host FP64), inverted.
+    // Base Gram matrix of {1, v, |v|^2} over the velocity grid (host FP64),
+    // assembled into the Gram matrix of the species constraint rows and inverted.
     const fq::Plan &P = h->P;
     const int N = P.N;
     const double L = P.L;
-    double Gm[25] = {0};
+    const int ns = P.ntgt;
+    if (ns > 4) return fail(FQ_ERR_UNSUPPORTED, "conservation projection supports up to 4 species");
+    std::vector<double> m(ns, 1.0);
+    for (int s = 0; s < ns && s < int(P.masses.size()); ++s) m[s] = P.masses[s];
+    double Gb[5][5] = {{0}};
     const double h_ = 2.0 * L / N;
     for (int I = 0; I < N; ++I)
         for (int J = 0; J < N; ++J)
@@ -953,29 +978,44 @@ int build_projection(fq_handle *h) {
                 const double cx = -L + L / N + h_ * I, cy = -L + L / N + h_ * J, cz = -L + L / N + h_ * K;
                 const double ph[5] = {1, cx, cy, cz, cx * cx + cy * cy + cz * cz};
                 for (int a = 0; a < 5; ++a)
-                    for (int b = 0; b < 5; ++b) Gm[a * 5 + b] += ph[a] * ph[b];
+                    for (int b = 0; b < 5; ++b) Gb[a][b] += ph[a] * ph[b];
             }
-    // Gauss-Jordan inverse
-    double A[5][10];
-    for (int a = 0; a < 5; ++a)
-        for (int b = 0; b < 10; ++b) A[a][b] = b < 5 ? Gm[a * 5 + b] : (b - 5 == a ? 1.0 : 0.0);
-    for (int c = 0; c < 5; ++c) {
+    const int K = ns + 4;
+    double sm2 = 0;
+    for (int s = 0; s < ns; ++s) sm2 += m[s] * m[s];
+    std::vector<double> G(size_t(K) * K, 0.0);
+    for (int r = 0; r < ns; ++r) {
+        G[r * K + r] = Gb[0][0];
+        for (int k = 1; k < 5; ++k) G[r * K + ns + k - 1] = G[(ns + k - 1) * K + r] = m[r] * Gb[0][k];
+    }
+    for (int a = 1; a < 5; ++a)
+        for (int b = 1; b < 5; ++b) G[(ns + a - 1) * K + ns + b - 1] = sm2 * Gb[a][b];
+    // Gauss-Jordan inverse with partial pivoting
+    std::vector<double> A(size_t(K) * 2 * K, 0.0);
+    for (int a = 0; a < K; ++a) {
+        for (int b = 0; b < K; ++b) A[a * 2 * K + b] = G[a * K + b];
+        A[a * 2 * K + K + a] = 1.0;
+    }
+    for (int c = 0; c < K; ++c) {
         int piv = c;
-        for (int r = c + 1; r < 5; ++r)
-            if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
-        for (int b = 0; b < 10; ++b) std::swap(A[c][b], A[piv][b]);
-        const double d = A[c][c];
-        for (int b = 0; b < 10; ++b) A[c][b] /= d;
-        for (int r = 0; r < 5; ++r)
+        for (int r = c + 1; r < K; ++r)
+            if (std::fabs(A[r * 2 * K + c]) > std::fabs(A[piv * 2 * K + c])) piv = r;
+        for (int b = 0; b < 2 * K; ++b) std::swap(A[c * 2 * K + b], A[piv * 2 * K + b]);
+        const double d = A[c * 2 * K + c];
+        if (d == 0.0) return fail(FQ_ERR_INVALID, "singular conservation Gram matrix");
+        for (int b = 0; b < 2 * K; ++b) A[c * 2 * K + b] /= d;
+        for (int r = 0; r < K; ++r)
             if (r != c) {
-                const double f = A[r][c];
-                for (int b = 0; b < 10; ++b) A[r][b] -= f * A[c][b];
+                const double f = A[r * 2 * K + c];
+                for (int b = 0; b < 2 * K; ++b) A[r * 2 * K + b] -= f * A[c * 2 * K + b];
             }
     }
-    std::vector<double> gi(25);
-    for (int a = 0; a < 5; ++a)
-        for (int b = 0; b < 5; ++b) gi[a * 5 + b] = A[a][b + 5];
-    return upload(&h->ginv, gi);
+    std::vector<double> gi(size_t(K) * K);
+    for (int a = 0; a < K; ++a)
+        for (int b = 0; b < K; ++b) gi[a * K + b] = A[a * 2 * K + K + b];
+    int rc = upload(&h->ginv, gi);
+    if (rc == FQ_SUCCESS) rc = upload(&h->pmass, m);
+    return rc;
 }
 
 template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, const Dof &dof, int D,
@@ -1063,7 +1103,8 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     post(5);
     if (h->flags & FQ_FLAG_PROJECT) {
         pre();
-        hipLaunchKernelGGL(k_project, dim3(unsigned(D * ntgt)), dim3(256), 0, st, ptr, dof, ntgt, N, P.L, h->ginv);
+        hipLaunchKernelGGL(k_project, dim3(unsigned(D)), dim3(256), 0, st, ptr, dof, ntgt, N, P.L, h->ginv,
+                           h->pmass);
         post(6);
     }
     hipError_t e = hipGetLastError();
@@ -1090,8 +1131,11 @@ int eval_impl(fq_handle *h, const AosoaPtrs &ptr, int32_t nupts, int64_t ldim, i
     HIPCHK(hipGetDevice(&dev));
     if (dev != h->device) return fail(FQ_ERR_INVALID, "handle was created on another device");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    for (int64_t c0 = 0; c0 < ndof; c0 += h->chunk) {
-        const int D = int(std::min<int64_t>(h->chunk, ndof - c0));
+    // equal batches of at most h->chunk DOFs (no short tail batch)
+    const int64_t nbatch = (ndof + h->chunk - 1) / h->chunk;
+    const int64_t per = (ndof + nbatch - 1) / nbatch;
+    for (int64_t c0 = 0; c0 < ndof; c0 += per) {
+        const int D = int(std::min<int64_t>(per, ndof - c0));
         Dof dof{first_dof + c0, nupts, 32, ldim, n};
         int rc;
         switch (h->P.N) {
@@ -1287,8 +1331,8 @@ int fq_destroy(fq_handle *h) {
     for (auto e : h->ev_pool) (void)hipEventDestroy(e);
     for (void *p : {h->ph, (void *)h->item_src, (void *)h->prods, (void *)h->g_p0, (void *)h->g_np,
                     (void *)h->c_group, (void *)h->t_c0, (void *)h->t_cn, (void *)h->l_src, (void *)h->t_l0,
-                    (void *)h->t_ln, (void *)h->W, (void *)h->ginv, (void *)h->F, h->Z, h->Pg, h->zc,
-                    (void *)h->F32})
+                    (void *)h->t_ln, (void *)h->W, (void *)h->ginv, (void *)h->pmass, (void *)h->F, h->Z,
+                    h->Pg, h->zc, (void *)h->F32})
         if (p) (void)hipFree(p);
     delete h;
     return FQ_SUCCESS;

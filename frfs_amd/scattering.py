@@ -15,7 +15,9 @@ Extra configuration keys (absent from the reference, whose operator is
 always FP64): ``[scattering-model] precision = double | mixed`` and
 ``conservation-projection = 0 | 1``; ``max-chunk`` caps the DOFs per
 internal batch; ``fused = 0`` selects the three-kernel path that stages the
-inverse transforms through HBM (for comparison).
+inverse transforms through HBM (for comparison); ``autotune = 1`` makes the
+first ``fs_batch`` call time the candidate schedules on its own inputs and
+keep the fastest (``frfs_amd.autotune``; ``autotune()`` does it on demand).
 """
 from abc import ABCMeta, abstractmethod
 from math import gamma as _gammafn
@@ -23,6 +25,7 @@ from math import gamma as _gammafn
 import numpy as np
 
 from . import _lib
+from . import autotune as _autotune
 from .sphericaldesign import get_sphquadrule
 
 
@@ -95,6 +98,19 @@ class DGFSScatteringModel(metaclass=ABCMeta):
     def plan_info(self):
         return self._h.info()
 
+    # -- schedule autotuning (shared by every model) --------------------------
+    def _tune(self, run, ndof):
+        info = self._h.info()
+        key = (type(self).__name__, self._params.c.precision, self._params.c.flags & _lib.FLAG_PROJECT,
+               self._params.c.Nv, info['n_items'], info['n_groups'], info['n_targets'], int(ndof))
+        self._h.close()
+        self._h, self.tune_report = _autotune.tune(self._params, run, key)
+        self._tuned = True
+
+    def _maybe_tune(self, run, ndof):
+        if not getattr(self, '_tuned', False) and self.cfg.getint('scattering-model', 'autotune', 0):
+            self._tune(run, ndof)
+
 
 class DGFSVHSGLLScatteringModel(DGFSScatteringModel):
     """vhs-gll (frfs/solvers/dgfs/scattering.py:44-286)."""
@@ -124,12 +140,27 @@ class DGFSVHSGLLScatteringModel(DGFSScatteringModel):
         self._h.eval(d_arr_in._as_parameter_, d_arr_out._as_parameter_, nrow, ldim, neles,
                      elem * nrow + upt, 1, _stream(self.backend))
 
-    def fs_batch(self, d_arr_in, d_arr_out, first_dof=0, ndof=None):
+    def _runner(self, d_arr_in, d_arr_out, first_dof, ndof):
         nrow, ldim, _ = d_arr_in.traits
         nvar, neles = d_arr_in.ioshape[1:]
         ndof = nrow * neles - first_dof if ndof is None else ndof
-        self._h.eval(d_arr_in._as_parameter_, d_arr_out._as_parameter_, nrow, ldim, neles,
-                     first_dof, ndof, _stream(self.backend))
+
+        def run(h):
+            h.eval(d_arr_in._as_parameter_, d_arr_out._as_parameter_, nrow, ldim, neles,
+                   first_dof, ndof, _stream(self.backend))
+        return run, ndof
+
+    def fs_batch(self, d_arr_in, d_arr_out, first_dof=0, ndof=None):
+        run, ndof = self._runner(d_arr_in, d_arr_out, first_dof, ndof)
+        self._maybe_tune(run, ndof)
+        run(self._h)
+
+    def autotune(self, d_arr_in, d_arr_out, first_dof=0, ndof=None):
+        """Time the candidate schedules on these arrays and keep the fastest
+        (``d_arr_out`` is overwritten).  Returns the tuning report."""
+        run, ndof = self._runner(d_arr_in, d_arr_out, first_dof, ndof)
+        self._tune(run, ndof)
+        return self.tune_report
 
 
 class DGFSVHSGLLNodalScatteringModel(DGFSVHSGLLScatteringModel):
@@ -144,13 +175,26 @@ class DGFSVHSGLLNodalScatteringModel(DGFSVHSGLLScatteringModel):
                               d_arr_out._as_parameter_, nrow, ldim, neles,
                               elem * nrow + upt, 1, _stream(self.backend))
 
-    def fs_batch(self, d_arr_in1, d_arr_in2, d_arr_out, first_dof=0, ndof=None):
+    def _runner(self, d_arr_in1, d_arr_in2, d_arr_out, first_dof, ndof):
         nrow, ldim, _ = d_arr_in1.traits
         nvar, neles = d_arr_in1.ioshape[1:]
         ndof = nrow * neles - first_dof if ndof is None else ndof
-        self._h.eval_bilinear(d_arr_in1._as_parameter_, d_arr_in2._as_parameter_,
-                              d_arr_out._as_parameter_, nrow, ldim, neles,
-                              first_dof, ndof, _stream(self.backend))
+
+        def run(h):
+            h.eval_bilinear(d_arr_in1._as_parameter_, d_arr_in2._as_parameter_,
+                            d_arr_out._as_parameter_, nrow, ldim, neles,
+                            first_dof, ndof, _stream(self.backend))
+        return run, ndof
+
+    def fs_batch(self, d_arr_in1, d_arr_in2, d_arr_out, first_dof=0, ndof=None):
+        run, ndof = self._runner(d_arr_in1, d_arr_in2, d_arr_out, first_dof, ndof)
+        self._maybe_tune(run, ndof)
+        run(self._h)
+
+    def autotune(self, d_arr_in1, d_arr_in2, d_arr_out, first_dof=0, ndof=None):
+        run, ndof = self._runner(d_arr_in1, d_arr_in2, d_arr_out, first_dof, ndof)
+        self._tune(run, ndof)
+        return self.tune_report
 
 
 class DGFSBiScatteringModel(metaclass=ABCMeta):
@@ -176,6 +220,9 @@ class DGFSBiScatteringModel(metaclass=ABCMeta):
 
     def plan_info(self):
         return self._h.info()
+
+    _tune = DGFSScatteringModel._tune
+    _maybe_tune = DGFSScatteringModel._maybe_tune
 
 
 class DGFSBiVSSGLLScatteringModel(DGFSBiScatteringModel):
@@ -221,7 +268,7 @@ class DGFSBiVSSGLLScatteringModel(DGFSBiScatteringModel):
             max_chunk=self.cfg.getint('scattering-model', 'max-chunk', 0))
         self._h = _lib.Handle(self._params)
 
-    def _call(self, arrs, first_dof, ndof):
+    def _runner(self, arrs, first_dof, ndof):
         ns = self._nspcs
         if len(arrs) != 2 * ns:
             raise ValueError('Need to be equal')
@@ -233,14 +280,27 @@ class DGFSBiVSSGLLScatteringModel(DGFSBiScatteringModel):
         ndof = nrow * neles - first_dof if ndof is None else ndof
         fp = [a._as_parameter_ for a in arrs[:ns]]
         qp = [a._as_parameter_ for a in arrs[ns:]]
-        if ns == 1:
-            self._h.eval(fp[0], qp[0], nrow, ldim, neles, first_dof, ndof, _stream(self.backend))
-        else:
-            self._h.eval_multi(fp, qp, nrow, ldim, neles, first_dof, ndof, _stream(self.backend))
+
+        def run(h):
+            if ns == 1:
+                h.eval(fp[0], qp[0], nrow, ldim, neles, first_dof, ndof, _stream(self.backend))
+            else:
+                h.eval_multi(fp, qp, nrow, ldim, neles, first_dof, ndof, _stream(self.backend))
+        return run, ndof
 
     def fs(self, elem, upt, *args):
         nrow = args[0].traits[0]
-        self._call(args, elem * nrow + upt, 1)
+        run, _ = self._runner(args, elem * nrow + upt, 1)
+        run(self._h)
 
     def fs_batch(self, *args, first_dof=0, ndof=None):
-        self._call(args, first_dof, ndof)
+        run, ndof = self._runner(args, first_dof, ndof)
+        self._maybe_tune(run, ndof)
+        run(self._h)
+
+    def autotune(self, *args, first_dof=0, ndof=None):
+        """Time the candidate schedules on these arrays (f_0..f_{S-1},
+        Q_0..Q_{S-1}; the Q arrays are overwritten) and keep the fastest."""
+        run, ndof = self._runner(args, first_dof, ndof)
+        self._tune(run, ndof)
+        return self.tune_report

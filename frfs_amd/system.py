@@ -38,6 +38,12 @@ class DGFSCollisionSystem:
     def collide(self, t, uinbank, foutbank):
         self.sm.fs_batch(self.banks[uinbank], self.banks[foutbank])
 
+    def prepare(self):
+        """With ``[scattering-model] autotune = 1``, tune the collision
+        schedule on the solution bank (the last bank is overwritten)."""
+        if self.cfg.getint('scattering-model', 'autotune', 0) and not getattr(self.sm, '_tuned', False):
+            self.sm.autotune(self.banks[0], self.banks[-1])
+
     def axnpby(self, dst, terms):
         """banks[dst] = a0*banks[dst] + sum(a*banks[i] for a, i in terms[1:])."""
         a0 = terms[0]
@@ -68,6 +74,10 @@ class DGFSBiCollisionSystem:
     def collide(self, t, uinbank, foutbank):
         arr = [b[uinbank] for b in self.banks] + [b[foutbank] for b in self.banks]
         self.sm.fs_batch(*arr)
+
+    def prepare(self):
+        if self.cfg.getint('scattering-model', 'autotune', 0) and not getattr(self.sm, '_tuned', False):
+            self.sm.autotune(*([b[0] for b in self.banks] + [b[-1] for b in self.banks]))
 
     def axnpby(self, dst, terms):
         for sb in self.banks:
@@ -110,16 +120,14 @@ class BaseStepper:
         self._graphs = {}
         self.idxcurr = 0
 
-    def _rhs(self, t, uin, fout):
-        if self.system.rhs is not None:
-            self.system.rhs(t, uin, fout)
-
     def step(self, t, dt):
         if not self.graph:
             return self._step(t, dt)
         torch = self.system.backend.torch
         g = self._graphs.get(dt)
         if g is None:
+            # one-time schedule tuning runs eagerly, never inside a capture
+            self.system.prepare()
             # capture one full step (fixed banks, in-place update of r0); the
             # library allocates nothing and issues no memsets during a step
             torch.cuda.synchronize()

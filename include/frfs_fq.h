@@ -62,8 +62,9 @@ enum fq_precision {
 };
 
 enum fq_flags {
-    FQ_FLAG_PROJECT = 1,      /* project Q onto the complement of the collision
-                                 invariants (mass/momentum/energy), in FP64 */
+    FQ_FLAG_PROJECT = 1,      /* least-squares FP64 correction of Q that zeroes the
+                                 collision invariants: the mass of every species
+                                 and the total (mass-weighted) momentum and energy */
     FQ_FLAG_UNFUSED = 2       /* stage every inverse transform through HBM
                                  (three-kernel path; for comparison) */
 };

@@ -1,0 +1,95 @@
+"""Schedule autotuner: selection, caching and failure handling on the host
+with stand-in handles, and on the GPU that the tuned handle computes the same
+Q as the default schedule."""
+import numpy as np
+import pytest
+
+
+class _FakeHandle:
+    made = []
+
+    def __init__(self, params):
+        fused = not (params.c.flags & 2)
+        if params.c.max_chunk == 512 and fused:
+            from frfs_amd._lib import ERR_ALLOC, FQError
+            raise FQError(ERR_ALLOC, 'no memory')
+        self.cost = 0.004 if fused else 0.001
+        self.closed = False
+        _FakeHandle.made.append(self)
+
+    def chunk_dofs(self):
+        return 7
+
+    def close(self):
+        self.closed = True
+
+
+class _P:
+    def __init__(self):
+        self.c = type('fq_params_stub', (), {})()
+        self.c.flags, self.c.max_chunk = 1, 0       # FLAG_PROJECT set, automatic chunk
+
+
+def test_tune_picks_fastest_and_caches(monkeypatch):
+    import time
+    from frfs_amd import _lib, autotune
+    monkeypatch.setattr(_lib, 'Handle', _FakeHandle)
+    monkeypatch.setattr(autotune, '_sync', lambda: None)
+    autotune.clear_cache()
+    _FakeHandle.made = []
+    p = _P()
+    h, rep = autotune.tune(p, lambda h: time.sleep(h.cost), key=('k', 1), reps=1)
+    assert rep['choice'] == {'fused': 0, 'max_chunk': 0}
+    assert not h.closed
+    assert [x.closed for x in _FakeHandle.made] == [True, False]   # the slower fused one released
+    assert any('error' in t for t in rep['timings'])                 # failed candidate reported
+    assert p.c.flags == 1 | 2 and p.c.max_chunk == 0                 # params keep the choice
+    h2, rep2 = autotune.tune(_P(), lambda h: 1 / 0, key=('k', 1))    # cached: nothing is run
+    assert rep2['cached'] and rep2['choice'] == rep['choice']
+    autotune.clear_cache()
+
+
+def test_tune_raises_when_nothing_builds(monkeypatch):
+    from frfs_amd import _lib, autotune
+
+    def boom(params):
+        raise _lib.FQError(_lib.ERR_ALLOC, 'no memory')
+    monkeypatch.setattr(_lib, 'Handle', boom)
+    autotune.clear_cache()
+    p = _P()
+    with pytest.raises(_lib.FQError):
+        autotune.tune(p, lambda h: None, key=('none',))
+    assert p.c.flags == 1 and p.c.max_chunk == 0                     # restored
+
+
+@pytest.mark.gpu
+def test_tuned_schedule_matches_default(backend):
+    import torch
+    from frfs_amd import autotune
+    from frfs_amd.inputs import mixture_device, mixture_params, to_aosoa
+    from frfs_amd.scattering import DGFSBiVSSGLLScatteringModel
+    from frfs_amd.velocitymesh import DGFSBiVelocityMesh
+    from tests.cases import vss_cfg
+    autotune.clear_cache()
+    cfg = vss_cfg(16, 6, 12, 5.0)
+    vm = DGFSBiVelocityMesh(backend, cfg, 1)
+    n = vm.vsize()
+    ndof = 40
+    fin = backend.matrix((1, n, ndof))
+    q0 = backend.matrix((1, n, ndof))
+    q1 = backend.matrix((1, n, ndof))
+    to_aosoa(mixture_device(vm.cv(), mixture_params(ndof, 11)), fin)
+    ref = DGFSBiVSSGLLScatteringModel(backend, cfg, vm)
+    ref.fs_batch(fin, q0)
+    tuned = DGFSBiVSSGLLScatteringModel(backend, cfg, vm)
+    rep = tuned.autotune(fin, q1)
+    print('autotune report', rep)
+    assert not rep['cached'] and len(rep['timings']) == 3
+    q1.set(np.zeros(q1.ioshape))
+    tuned.fs_batch(fin, q1)
+    torch.cuda.synchronize()
+    a, b = q0.get(), q1.get()
+    assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(a)
+    again = DGFSBiVSSGLLScatteringModel(backend, cfg, vm)
+    assert again.autotune(fin, q1)['cached']
+    autotune.clear_cache()

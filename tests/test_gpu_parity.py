@@ -154,3 +154,66 @@ def test_vss_two_species(backend):
         assert rel_l2(Q0[d], r0) <= FP64_TOL
         assert rel_l2(Q1[d], r1) <= FP64_TOL
     print(sm.plan_info())
+
+
+def _two_species_case(prec='double', project=0):
+    m0 = 39.96e-3
+    return vss_cfg(16, 4, 12, 5.0, masses=(m0, 2 * m0), precision=prec, project=project,
+                   omega={'00': 0.74, '01': 0.775, '10': 0.775, '11': 0.81},
+                   dref={'00': 4.17e-10, '01': 4.17e-10 * 1.15, '10': 4.17e-10 * 1.15,
+                         '11': 4.17e-10 * 1.3})
+
+
+def test_vss_two_species_mixed(backend):
+    from oracle import fastspectral as fsp
+    cfg = _two_species_case('mixed')
+    vm, prm, tab = oracle_vss(cfg, 2)
+    f0 = fsp.two_maxwellian_batch(vm['cv'], 2, seed=41, mass=vm['masses'][0])
+    f1 = fsp.two_maxwellian_batch(vm['cv'], 2, seed=42, mass=vm['masses'][1], dens_range=(0.2, 0.8))
+    print('[vss-2 mixed]', flush=True)
+    (Q0, Q1), _ = _run_vss(backend, cfg, [f0, f1], 1, 2)
+    for d in range(2):
+        r0, r1 = fsp.vss_fs([f0[d], f1[d]], vm, tab, prm)
+        assert rel_l2(Q0[d], r0) <= MIXED_TOL
+        assert rel_l2(Q1[d], r1) <= MIXED_TOL
+
+
+@pytest.mark.parametrize('nspecies', [1, 2])
+def test_projection_zeroes_invariants(backend, nspecies):
+    """The projection leaves every species' mass and the total mass-weighted
+    momentum and energy at zero, and its correction lies in the span of the
+    constraint functions {e_s, m_s v, m_s |v|^2}: the two properties that fix
+    the least-squares correction uniquely.  Momentum and energy of a single
+    species of a mixture are not constrained (they move between species)."""
+    from oracle import fastspectral as fsp
+    if nspecies == 1:
+        cfg0, cfg1 = vss_cfg(16, 4, 12, 5.0), vss_cfg(16, 4, 12, 5.0, project=1)
+    else:
+        cfg0, cfg1 = _two_species_case(), _two_species_case(project=1)
+    vm, prm, tab = oracle_vss(cfg0, nspecies)
+    masses = vm['masses'][:nspecies]
+    fs = [fsp.two_maxwellian_batch(vm['cv'], 3, seed=50 + s, mass=masses[s]) for s in range(nspecies)]
+    print(f'[projection {nspecies}]', flush=True)
+    Qa, _ = _run_vss(backend, cfg0, fs, 1, 3)
+    Qb, _ = _run_vss(backend, cfg1, fs, 1, 3)
+    cv = vm['cv']
+    phi = np.stack([np.ones(cv.shape[1]), cv[0], cv[1], cv[2], (cv ** 2).sum(0)])     # (5, n)
+    n = cv.shape[1]
+    for d in range(3):
+        # constraint rows over the stacked species vector
+        rows = []
+        for s in range(nspecies):
+            r = np.zeros(nspecies * n)
+            r[s * n:(s + 1) * n] = 1
+            rows.append(r)
+        for k in range(1, 5):
+            rows.append(np.concatenate([masses[s] * phi[k] for s in range(nspecies)]))
+        A = np.array(rows)
+        qa = np.concatenate([Qa[s][d] for s in range(nspecies)])
+        qb = np.concatenate([Qb[s][d] for s in range(nspecies)])
+        scale = np.abs(A) @ np.abs(qa)
+        assert np.all(np.abs(A @ qb) <= 1e-11 * scale), (A @ qb, scale)
+        delta = qa - qb
+        coef, *_ = np.linalg.lstsq(A.T, delta, rcond=None)
+        assert np.linalg.norm(A.T @ coef - delta) <= 1e-9 * np.linalg.norm(delta) + 1e-300
+        print(nspecies, d, 'defect removed', np.linalg.norm(delta) / np.linalg.norm(qa))
