@@ -414,7 +414,10 @@ __global__ __launch_bounds__(kPlaneThreads) void k_plane(const cpx<T> *Z, cpx<T>
 template <int N> struct ProdGeo {
     static constexpr int S = N / 4;
     static constexpr int E = N / 2;
-    static constexpr int LPP = Geo<N>::LPP;
+    // planes are packed back to back, 2N lanes each (N = 24: 48 lanes, no
+    // idle lanes inside a plane; plane starts stay 4-lane aligned for the
+    // quad-permute pair exchange)
+    static constexpr int LPP = 2 * N;
     static constexpr int THREADS = S * LPP;
     static constexpr int ROW = N + 1;
     static constexpr int PLANE = N * ROW;
@@ -659,6 +662,126 @@ __global__ __launch_bounds__(256, accum_waves(N, sizeof(T))) void k_accum(const 
         }
     }
     fq::fft2_dif<N, 1>(H, half);                  // -> vz = half + 2e
+    cpx<double> *yg = Y + ((int64_t(slot) * ntgt + t) * 2 + 0) * G::VOL + off;
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) yg[int64_t(half + 2 * e) * G::N2] = H[e];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) H[e] = {0, 0};
+    const int l0 = t_l0[t], ln = t_ln[t];
+    for (int li = l0; li < l0 + ln; ++li) {
+        const int s = l_src[li];
+        const cpx<double> *fs = F + (int64_t(slot) * nsrc + s) * G::VOL + off;
+        const cpx<double> *w = W + int64_t(ncontribs + li) * G::VOL + off;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int64_t k = int64_t(half * E + e) * G::N2;
+            if (act) H[e] = H[e] + fq::cmul(w[k], fs[k]);
+        }
+    }
+    fq::fft2_dif<N, 1>(H, half);
+    cpx<double> *yl = Y + ((int64_t(slot) * ntgt + t) * 2 + 1) * G::VOL + off;
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) yl[int64_t(half + 2 * e) * G::N2] = H[e];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_accum_lds: the same accumulation for grids whose pencil row fills one
+// wave (N = 24, 32).  One workgroup = four consecutive DOF slots of one
+// (target, row I'), one wave each.  The weight row block W[c][.][I'][.]
+// (N x N FP64 values) is the same for the four slots, so the workgroup loads
+// it once per contribution into a double-buffered LDS area, one contribution
+// ahead of its use, and the waves read it from LDS: the weight latency is
+// hidden behind a whole contribution and weight reads from L2 drop fourfold.
+// Waves whose slot lies past the batch take part in the copies and barriers
+// only.
+// ---------------------------------------------------------------------------
+template <int N, typename T>
+__global__ __launch_bounds__(256, accum_waves(N, sizeof(T))) void k_accum_lds(
+    const cpx<T> *__restrict__ P, const cpx<double> *__restrict__ F, const cpx<double> *__restrict__ W,
+    const int *__restrict__ c_group, const int *__restrict__ t_c0, const int *__restrict__ t_cn,
+    const int *__restrict__ l_src, const int *__restrict__ t_l0, const int *__restrict__ t_ln, int ncontribs,
+    cpx<double> *__restrict__ Y, int ngroups, int nsrc, int ntgt, int D) {
+    using G = Geo<N>;
+    static_assert(G::LPP == 64, "one pencil row per wave");
+    constexpr int E = G::E;
+    constexpr int WR = N * N;                     // weight row block (K', J)
+    constexpr int CP = (WR + 255) / 256;          // 16-byte copies per thread
+    __shared__ cpx<double> wl[2][WR];
+    const int nsg = (D + 3) / 4;
+    const int b = blockIdx.x;
+    const int sg = b % nsg;
+    const int t = (b / nsg) % ntgt;
+    const int I = b / (nsg * ntgt);
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / 64);
+    const int slot = sg * 4 + wave;
+    const bool live = slot < D;                   // wave-uniform
+    const int lt = threadIdx.x % 64;
+    const int J = lt >> 1, half = lt & 1;
+    const bool act = J < N;
+    const int64_t off = int64_t(I) * N + J;
+    const int c0 = t_c0[t], cn = t_cn[t];
+    // weight copies: element idx = K' * N + J' of the row block
+    cpx<double> wp[CP];
+    auto wload = [&](int ci) {
+        const cpx<double> *w = W + int64_t(ci) * G::VOL + int64_t(I) * N;
+#pragma unroll
+        for (int k = 0; k < CP; ++k) {
+            const int idx = int(threadIdx.x) + 256 * k;
+            if (WR % 256 == 0 || idx < WR) wp[k] = w[int64_t(idx / N) * G::N2 + idx % N];
+        }
+    };
+    auto wstore = [&](int buf) {
+#pragma unroll
+        for (int k = 0; k < CP; ++k) {
+            const int idx = int(threadIdx.x) + 256 * k;
+            if (WR % 256 == 0 || idx < WR) wl[buf][idx] = wp[k];
+        }
+    };
+    const cpx<T> *pbase = P + int64_t(live ? slot : 0) * ngroups * G::VOL + off;
+    auto load_pencil = [&](int ci, cpx<T> *dst) {
+        const cpx<T> *src = pbase + int64_t(c_group[ci]) * G::VOL;
+#pragma unroll
+        for (int e = 0; e < E; ++e) dst[e] = (live && act) ? src[int64_t(half + 2 * e) * G::N2] : cpx<T>{0, 0};
+    };
+    cpx<double> H[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) H[e] = {0, 0};
+    // FP32 pencils are prefetched one contribution ahead; FP64 pencils are
+    // loaded in place (no register room for both)
+    constexpr bool PF = sizeof(T) == 4;
+    cpx<T> nxt[PF ? E : 1];
+    if (cn > 0) {
+        wload(c0);
+        if constexpr (PF) load_pencil(c0, nxt);
+    }
+    for (int ci = c0, it = 0; ci < c0 + cn; ++ci, ++it) {
+        const int buf = it & 1;
+        wstore(buf);                                  // reads of buf ended before the last barrier
+        if (ci + 1 < c0 + cn) wload(ci + 1);
+        __syncthreads();
+        cpx<T> v[E];
+        if constexpr (PF) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = nxt[e];
+            if (ci + 1 < c0 + cn) load_pencil(ci + 1, nxt);
+        } else {
+            load_pencil(ci, v);
+        }
+        if (live) {
+            fq::fft2_dit<N, -1>(v, half);             // -> K' = half*E + e
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const cpx<double> we = act ? wl[buf][(half * E + e) * N + J] : cpx<double>{0, 0};
+                fq::cmac(H[e], we, cvt<double>(v[e]));
+            }
+        }
+    }
+    if (!live) return;                                // no barrier follows
+    fq::fft2_dif<N, 1>(H, half);                      // -> vz = half + 2e
     cpx<double> *yg = Y + ((int64_t(slot) * ntgt + t) * 2 + 0) * G::VOL + off;
     if (act) {
 #pragma unroll
@@ -1095,8 +1218,13 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     }
     u = int64_t(D) * ntgt * N;
     pre();
-    hipLaunchKernelGGL((k_accum<N, T>), dim3(blocks(u)), bt, 0, st, Pg, h->F, h->W, h->c_group, h->t_c0,
-                       h->t_cn, h->l_src, h->t_l0, h->t_ln, int(P.contribs.size()), Y, ngroups, nsrc, ntgt, D, u);
+    if constexpr (Geo<N>::LPP == 64 && sizeof(T) == 4)
+        hipLaunchKernelGGL((k_accum_lds<N, T>), dim3(unsigned(int64_t((D + 3) / 4) * ntgt * N)), dim3(256), 0, st,
+                           Pg, h->F, h->W, h->c_group, h->t_c0, h->t_cn, h->l_src, h->t_l0, h->t_ln,
+                           int(P.contribs.size()), Y, ngroups, nsrc, ntgt, D);
+    else
+        hipLaunchKernelGGL((k_accum<N, T>), dim3(blocks(u)), bt, 0, st, Pg, h->F, h->W, h->c_group, h->t_c0,
+                           h->t_cn, h->l_src, h->t_l0, h->t_ln, int(P.contribs.size()), Y, ngroups, nsrc, ntgt, D, u);
     post(4);
     pre();
     hipLaunchKernelGGL(k_out<N>, dim3(pblocks(u)), pt, 0, st, Y, ptr, dof, ntgt, u);
