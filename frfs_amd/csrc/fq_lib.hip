@@ -414,10 +414,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_plane(const cpx<T> *Z, cpx<T>
 template <int N> struct ProdGeo {
     static constexpr int S = N / 4;
     static constexpr int E = N / 2;
-    // planes are packed back to back, 2N lanes each (N = 24: 48 lanes, no
-    // idle lanes inside a plane; plane starts stay 4-lane aligned for the
-    // quad-permute pair exchange)
-    static constexpr int LPP = 2 * N;
+    static constexpr int LPP = Geo<N>::LPP;
     static constexpr int THREADS = S * LPP;
     static constexpr int ROW = N + 1;
     static constexpr int PLANE = N * ROW;
@@ -430,12 +427,12 @@ template <int N> struct ProdGeo {
 //   d_b = w_4^{bG} beta_b,  c_a = w_N^{aG} alpha_a   (uniform per item and G),
 // and the x,y phases phi_x(I) phi_y(J) scale the S outputs of pencil (I,J).
 // zc[(item*4 + G)*(4 + S) + .] holds {d_0..d_3, c_0..c_{S-1}}.
-template <int N, typename T>
+template <int N, typename T, int THREADS = ProdGeo<N>::THREADS>
 __device__ __forceinline__ void zstage(const cpx<T> *__restrict__ src, const cpx<T> *__restrict__ zc,
                                        const cpx<T> *__restrict__ ph, cpx<T> *dst) {
     using PG = ProdGeo<N>;
     constexpr int S = PG::S;
-    constexpr int ITER = (N * N + PG::THREADS - 1) / PG::THREADS;
+    constexpr int ITER = (N * N + THREADS - 1) / THREADS;
     // d_0 = c_0 = 1 exactly (zero angle), so those products are skipped
     cpx<T> d[4], cc[S];
 #pragma unroll
@@ -446,8 +443,8 @@ __device__ __forceinline__ void zstage(const cpx<T> *__restrict__ src, const cpx
     const BufView sv(src, uint32_t(N) * PB);
 #pragma unroll 1
     for (int it = 0; it < ITER; ++it) {
-        const uint32_t p = threadIdx.x + uint32_t(it * PG::THREADS);
-        if (ITER * PG::THREADS != N * N && p >= uint32_t(N * N)) break;
+        const uint32_t p = threadIdx.x + uint32_t(it * THREADS);
+        if (ITER * THREADS != N * N && p >= uint32_t(N * N)) break;
         const uint32_t I = p / N, J = p % N;
         const uint32_t vo = p * uint32_t(sizeof(cpx<T>));
         // the 4S loads go out in two batches of 2S, each issued before its first use
@@ -477,6 +474,21 @@ __device__ __forceinline__ void zstage(const cpx<T> *__restrict__ src, const cpx
     }
 }
 
+// Synchronisation around the in-place transpose of one LDS plane.  When a
+// plane is exactly one wave (2N = 64 lanes) only that wave touches it: LDS
+// executes one wave's accesses in issue order, so a wave-level barrier that
+// keeps the compiler from moving LDS accesses across it is enough.  Planes
+// that straddle waves need the workgroup barrier.
+template <int N> __device__ __forceinline__ void plane_sync() {
+    if constexpr (ProdGeo<N>::LPP == 64) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // E = DFT+_{x,y}(Z) of the LDS plane Lp (row stride ROW, phases already
 // applied); the plane region is reused as the transpose buffer.
 // Out: row vx = c, cyclic vy.
@@ -487,12 +499,12 @@ __device__ __forceinline__ void plane_inv_lds(cpx<T> *Lp, cpx<T> *v, int c, int 
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = act ? Lp[(half * E + e) * PG::ROW + c] : cpx<T>{0, 0};
     fq::fft2_dif<N, 1>(v, half);
-    __syncthreads();
+    plane_sync<N>();
     if (act) {
 #pragma unroll
         for (int e = 0; e < E; ++e) Lp[(half + 2 * e) * PG::ROW + c] = v[e];
     }
-    __syncthreads();
+    plane_sync<N>();
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = act ? Lp[c * PG::ROW + half * E + e] : cpx<T>{0, 0};
     fq::fft2_dif<N, 1>(v, half);
@@ -552,12 +564,12 @@ __global__ __launch_bounds__(ProdGeo<N>::THREADS, (sizeof(T) == 4 ? 4 : 2)) void
     }
     // forward transforms of the group sum: acc is row vx = c, cyclic over vy
     fq::fft2_dit<N, -1>(acc, half);                       // -> J' = half*E + e
-    __syncthreads();
+    plane_sync<N>();
     if (act) {
 #pragma unroll
         for (int e = 0; e < E; ++e) L1[c * PG::ROW + half * E + e] = acc[e];
     }
-    __syncthreads();
+    plane_sync<N>();
 #pragma unroll
     for (int e = 0; e < E; ++e) acc[e] = act ? L1[(half + 2 * e) * PG::ROW + c] : cpx<T>{0, 0};
     fq::fft2_dit<N, -1>(acc, half);                       // -> I' = half*E + e
@@ -566,6 +578,100 @@ __global__ __launch_bounds__(ProdGeo<N>::THREADS, (sizeof(T) == 4 ? 4 : 2)) void
 #pragma unroll
         for (int e = 0; e < E; ++e) dst[(half * E + e) * N + c] = acc[e];
     }
+}
+
+// ---------------------------------------------------------------------------
+// k_prod1: the product stage with one lane per pencil (FP32, N = 32).  The
+// workgroup (slot, group, G) is the same as k_prod's, with the same pruned
+// z stage, but a plane's 32 pencils sit on 32 lanes (two planes per wave),
+// each holding the whole pencil: the x and y transforms are in-register
+// FFTs with no lane exchange.  This trades half the occupancy (64-value
+// pencils need about twice the registers) for about a third fewer
+// instructions per transform.
+// ---------------------------------------------------------------------------
+template <int N> struct Prod1Geo {
+    static constexpr int S = N / 4;
+    static constexpr int THREADS = S * N;          // one lane per pencil
+};
+
+template <int N, typename T>
+__device__ __forceinline__ void plane_inv1(cpx<T> *Lp, cpx<T> *v, int c) {
+    using PG = ProdGeo<N>;
+    // column c over I -> vx (natural order)
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = Lp[i * PG::ROW + c];
+    fq::LFFT<N, 1>::run(v);
+#pragma unroll
+    for (int i = 0; i < N; ++i) Lp[i * PG::ROW + c] = v[i];
+    plane_sync<32>();                              // the plane lives in one wave
+    // row vx = c over J -> vy
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] = Lp[c * PG::ROW + j];
+    fq::LFFT<N, 1>::run(v);
+}
+
+template <int N, typename T, bool MULTI>
+__global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
+    const cpx<T> *__restrict__ FT, cpx<T> *__restrict__ P, const cpx<T> *__restrict__ ph,
+    const cpx<T> *__restrict__ zc, const DevProd *__restrict__ prods, const int *__restrict__ g_p0,
+    const int *__restrict__ g_np, const int *__restrict__ item_src, int ngroups, int nsrc) {
+    using PG = ProdGeo<N>;
+    using P1 = Prod1Geo<N>;
+    using G3 = Geo<N>;
+    constexpr int ZC = 4 + PG::S;
+    static_assert(2 * N == 64, "two planes per wave");
+    __shared__ cpx<T> lds[PG::S * PG::PLANE];
+    const int64_t unit = blockIdx.x;
+    const int G = int(unit % 4);
+    const int g = int((unit / 4) % ngroups);
+    const int slot = int(unit / (4 * int64_t(ngroups)));
+    const int k = threadIdx.x / N;                 // plane of this lane
+    const int c = threadIdx.x % N;                 // pencil (column, then row) of this lane
+    const int vz = G + 4 * k;
+    cpx<T> *L1 = lds + k * PG::PLANE;
+    cpx<T> acc[N];
+#pragma unroll
+    for (int e = 0; e < N; ++e) acc[e] = {0, 0};
+    const int p0 = g_p0[g], np = g_np[g];
+    for (int pi = p0; pi < p0 + np; ++pi) {
+        const DevProd pr = prods[pi];
+        const cpx<T> *src1 = FT + (int64_t(slot) * nsrc + item_src[pr.i1]) * G3::VOL;
+        const cpx<T> *src2 = FT + (int64_t(slot) * nsrc + item_src[pr.i2]) * G3::VOL;
+        const cpx<T> *zc1 = zc + (int64_t(pr.i1) * 4 + G) * ZC, *zc2 = zc + (int64_t(pr.i2) * 4 + G) * ZC;
+        const cpx<T> *ph1 = ph + int64_t(pr.i1) * 3 * N, *ph2 = ph + int64_t(pr.i2) * 3 * N;
+        cpx<T> e1[N], e2[N];
+        __syncthreads();                                   // LDS free
+        zstage<N, T, P1::THREADS>(src1, zc1, ph1, lds);
+        __syncthreads();
+        plane_inv1<N, T>(L1, e1, c);
+        __syncthreads();
+        zstage<N, T, P1::THREADS>(src2, zc2, ph2, lds);
+        __syncthreads();
+        plane_inv1<N, T>(L1, e2, c);
+        const T cf = T(pr.coef);
+        if constexpr (MULTI) {
+#pragma unroll
+            for (int e = 0; e < N; ++e) {
+                const cpx<T> pe = fq::cmul(e1[e], e2[e]);
+                acc[e] = {fq::fmadd(cf, pe.x, acc[e].x), fq::fmadd(cf, pe.y, acc[e].y)};
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < N; ++e) acc[e] = fq::cscale(fq::cmul(e1[e], e2[e]), cf);
+        }
+    }
+    // forward transforms of the group sum: lane c holds row vx = c over vy
+    fq::LFFT<N, -1>::run(acc);                            // -> J'
+    plane_sync<32>();
+#pragma unroll
+    for (int j = 0; j < N; ++j) L1[c * PG::ROW + j] = acc[j];
+    plane_sync<32>();
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[i] = L1[i * PG::ROW + c];   // column J' = c over vx
+    fq::LFFT<N, -1>::run(acc);                            // -> I'
+    cpx<T> *dst = P + (int64_t(slot) * ngroups + g) * G3::VOL + int64_t(vz) * G3::N2;
+#pragma unroll
+    for (int i = 0; i < N; ++i) dst[i * N + c] = acc[i];
 }
 
 // FP64 -> FP32 copy of the source spectra (mixed precision).
@@ -1190,6 +1296,21 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
         pre();
         const cpx<T> *zc = reinterpret_cast<const cpx<T> *>(h->zc);
         const dim3 grid(unsigned(int64_t(D) * ngroups * 4)), blk(ProdGeo<N>::THREADS);
+#ifndef FQ_PROD_ONE_LANE
+#define FQ_PROD_ONE_LANE 1
+#endif
+        if constexpr (FQ_PROD_ONE_LANE && N == 32 && sizeof(T) == 4) {
+            bool multi1 = false;
+            for (int g = 0; g < ngroups; ++g) multi1 |= P.group_np[g] > 1;
+            const dim3 blk1(Prod1Geo<N>::THREADS);
+            if (multi1)
+                hipLaunchKernelGGL((k_prod1<N, T, true>), grid, blk1, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0,
+                                   h->g_np, h->item_src, ngroups, nsrc);
+            else
+                hipLaunchKernelGGL((k_prod1<N, T, false>), grid, blk1, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0,
+                                   h->g_np, h->item_src, ngroups, nsrc);
+            post(7);
+        } else {
         bool multi = false;
         for (int g = 0; g < ngroups; ++g) multi |= P.group_np[g] > 1;
         if (multi)
@@ -1199,6 +1320,7 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
             hipLaunchKernelGGL((k_prod<N, T, false>), grid, blk, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0,
                                h->g_np, h->item_src, ngroups, nsrc);
         post(7);
+        }
     } else {
     u = int64_t(D) * nitems * N;
     pre();
@@ -1218,7 +1340,7 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     }
     u = int64_t(D) * ntgt * N;
     pre();
-    if constexpr (Geo<N>::LPP == 64 && sizeof(T) == 4)
+    if constexpr (N == 32 && sizeof(T) == 4)
         hipLaunchKernelGGL((k_accum_lds<N, T>), dim3(unsigned(int64_t((D + 3) / 4) * ntgt * N)), dim3(256), 0, st,
                            Pg, h->F, h->W, h->c_group, h->t_c0, h->t_cn, h->l_src, h->t_l0, h->t_ln,
                            int(P.contribs.size()), Y, ngroups, nsrc, ntgt, D);

@@ -136,6 +136,21 @@ def test_vss_single_species(backend, prec, tol, fused):
     assert max(errs) <= tol
 
 
+@pytest.mark.parametrize('prec,tol', [('double', FP64_TOL), ('mixed', MIXED_TOL)])
+def test_vss_single_species_32(backend, prec, tol):
+    """32^3 grid, one product per group: the one-lane-per-pencil FP32 product
+    kernel and the FP64 two-lane kernel against the oracle."""
+    from oracle import fastspectral as fsp
+    cfg = vss_cfg(32, 4, 12, 6.0, precision=prec)
+    vm, prm, tab = oracle_vss(cfg, 1)
+    f = fsp.two_maxwellian_batch(vm['cv'], 2, seed=32)
+    print(f'[vss-1 32^3 {prec}]', flush=True)
+    (Q,), sm = _run_vss(backend, cfg, [f], 1, 2)
+    errs = [rel_l2(Q[d], fsp.vss_fs([f[d]], vm, tab, prm)[0]) for d in range(2)]
+    print(prec, max(errs), sm.plan_info())
+    assert max(errs) <= tol
+
+
 def test_vss_two_species(backend):
     """Two species, mass ratio 1:2, four pairs, uneven omega/diameters."""
     from oracle import fastspectral as fsp
