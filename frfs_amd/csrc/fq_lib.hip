@@ -581,32 +581,35 @@ __global__ __launch_bounds__(ProdGeo<N>::THREADS, (sizeof(T) == 4 ? 4 : 2)) void
 }
 
 // ---------------------------------------------------------------------------
-// k_prod1: the product stage with one lane per pencil (FP32, N = 32).  The
-// workgroup (slot, group, G) is the same as k_prod's, with the same pruned
-// z stage, but a plane's 32 pencils sit on 32 lanes (two planes per wave),
-// each holding the whole pencil: the x and y transforms are in-register
-// FFTs with no lane exchange.  This trades half the occupancy (64-value
+// k_prod1: the product stage with one lane per pencil (FP32, N = 24, 32).
+// The workgroup (slot, group, G) is the same as k_prod's, with the same
+// pruned z stage, but each half-wave (32 lanes) holds one plane, one pencil
+// per lane (lanes past N idle for N = 24): the x and y transforms are
+// in-register FFTs with no lane exchange.  This trades occupancy (whole
 // pencils need about twice the registers) for about a third fewer
 // instructions per transform.
 // ---------------------------------------------------------------------------
 template <int N> struct Prod1Geo {
     static constexpr int S = N / 4;
-    static constexpr int THREADS = S * N;          // one lane per pencil
+    static_assert(S % 2 == 0 && N <= 32, "two planes per wave");
+    static constexpr int THREADS = (S / 2) * 64;
 };
 
 template <int N, typename T>
-__device__ __forceinline__ void plane_inv1(cpx<T> *Lp, cpx<T> *v, int c) {
+__device__ __forceinline__ void plane_inv1(cpx<T> *Lp, cpx<T> *v, int c, bool act) {
     using PG = ProdGeo<N>;
     // column c over I -> vx (natural order)
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = Lp[i * PG::ROW + c];
+    for (int i = 0; i < N; ++i) v[i] = act ? Lp[i * PG::ROW + c] : cpx<T>{0, 0};
     fq::LFFT<N, 1>::run(v);
+    if (act) {
 #pragma unroll
-    for (int i = 0; i < N; ++i) Lp[i * PG::ROW + c] = v[i];
+        for (int i = 0; i < N; ++i) Lp[i * PG::ROW + c] = v[i];
+    }
     plane_sync<32>();                              // the plane lives in one wave
     // row vx = c over J -> vy
 #pragma unroll
-    for (int j = 0; j < N; ++j) v[j] = Lp[c * PG::ROW + j];
+    for (int j = 0; j < N; ++j) v[j] = act ? Lp[c * PG::ROW + j] : cpx<T>{0, 0};
     fq::LFFT<N, 1>::run(v);
 }
 
@@ -619,14 +622,14 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
     using P1 = Prod1Geo<N>;
     using G3 = Geo<N>;
     constexpr int ZC = 4 + PG::S;
-    static_assert(2 * N == 64, "two planes per wave");
     __shared__ cpx<T> lds[PG::S * PG::PLANE];
     const int64_t unit = blockIdx.x;
     const int G = int(unit % 4);
     const int g = int((unit / 4) % ngroups);
     const int slot = int(unit / (4 * int64_t(ngroups)));
-    const int k = threadIdx.x / N;                 // plane of this lane
-    const int c = threadIdx.x % N;                 // pencil (column, then row) of this lane
+    const int k = int(threadIdx.x) / 32;           // plane of this half-wave
+    const int c = int(threadIdx.x) % 32;           // pencil (column, then row) of this lane
+    const bool act = c < N;
     const int vz = G + 4 * k;
     cpx<T> *L1 = lds + k * PG::PLANE;
     cpx<T> acc[N];
@@ -643,11 +646,11 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
         __syncthreads();                                   // LDS free
         zstage<N, T, P1::THREADS>(src1, zc1, ph1, lds);
         __syncthreads();
-        plane_inv1<N, T>(L1, e1, c);
+        plane_inv1<N, T>(L1, e1, c, act);
         __syncthreads();
         zstage<N, T, P1::THREADS>(src2, zc2, ph2, lds);
         __syncthreads();
-        plane_inv1<N, T>(L1, e2, c);
+        plane_inv1<N, T>(L1, e2, c, act);
         const T cf = T(pr.coef);
         if constexpr (MULTI) {
 #pragma unroll
@@ -663,15 +666,19 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
     // forward transforms of the group sum: lane c holds row vx = c over vy
     fq::LFFT<N, -1>::run(acc);                            // -> J'
     plane_sync<32>();
+    if (act) {
 #pragma unroll
-    for (int j = 0; j < N; ++j) L1[c * PG::ROW + j] = acc[j];
+        for (int j = 0; j < N; ++j) L1[c * PG::ROW + j] = acc[j];
+    }
     plane_sync<32>();
 #pragma unroll
-    for (int i = 0; i < N; ++i) acc[i] = L1[i * PG::ROW + c];   // column J' = c over vx
+    for (int i = 0; i < N; ++i) acc[i] = act ? L1[i * PG::ROW + c] : cpx<T>{0, 0};   // column J' = c over vx
     fq::LFFT<N, -1>::run(acc);                            // -> I'
     cpx<T> *dst = P + (int64_t(slot) * ngroups + g) * G3::VOL + int64_t(vz) * G3::N2;
+    if (act) {
 #pragma unroll
-    for (int i = 0; i < N; ++i) dst[i * N + c] = acc[i];
+        for (int i = 0; i < N; ++i) dst[i * N + c] = acc[i];
+    }
 }
 
 // FP64 -> FP32 copy of the source spectra (mixed precision).
@@ -1299,7 +1306,7 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
 #ifndef FQ_PROD_ONE_LANE
 #define FQ_PROD_ONE_LANE 1
 #endif
-        if constexpr (FQ_PROD_ONE_LANE && N == 32 && sizeof(T) == 4) {
+        if constexpr (FQ_PROD_ONE_LANE && (N == 24 || N == 32) && sizeof(T) == 4) {
             bool multi1 = false;
             for (int g = 0; g < ngroups; ++g) multi1 |= P.group_np[g] > 1;
             const dim3 blk1(Prod1Geo<N>::THREADS);
@@ -1577,6 +1584,11 @@ int fq_chunk_dofs(const fq_handle *h, int64_t *chunk) {
 
 int fq_destroy(fq_handle *h) {
     if (!h) return FQ_SUCCESS;
+    // A handle may be released (e.g. by a garbage collector) while another
+    // stream is being captured into a graph; relaxed capture mode keeps these
+    // frees from invalidating that capture.
+    hipStreamCaptureMode mode = hipStreamCaptureModeRelaxed;
+    (void)hipThreadExchangeStreamCaptureMode(&mode);
     for (auto &r : h->ev_rec) { (void)hipEventDestroy(r.second.first); (void)hipEventDestroy(r.second.second); }
     for (auto e : h->ev_pool) (void)hipEventDestroy(e);
     for (void *p : {h->ph, (void *)h->item_src, (void *)h->prods, (void *)h->g_p0, (void *)h->g_np,
@@ -1584,6 +1596,7 @@ int fq_destroy(fq_handle *h) {
                     (void *)h->t_ln, (void *)h->W, (void *)h->ginv, (void *)h->pmass, (void *)h->F, h->Z,
                     h->Pg, h->zc, (void *)h->F32})
         if (p) (void)hipFree(p);
+    (void)hipThreadExchangeStreamCaptureMode(&mode);
     delete h;
     return FQ_SUCCESS;
 }

@@ -12,6 +12,8 @@ captured once into a HIP graph and replayed.
 advection term into bank ``fout`` (the FR spatial operator of the reference);
 without it the systems integrate the space-homogeneous equation df/dt = Q(f).
 """
+import gc
+
 import numpy as np
 
 from . import _lib
@@ -128,6 +130,9 @@ class BaseStepper:
         if g is None:
             # one-time schedule tuning runs eagerly, never inside a capture
             self.system.prepare()
+            # release unreachable handles now rather than from a collection
+            # that could run in the middle of the capture
+            gc.collect()
             # capture one full step (fixed banks, in-place update of r0); the
             # library allocates nothing and issues no memsets during a step
             torch.cuda.synchronize()
