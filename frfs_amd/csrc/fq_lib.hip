@@ -802,8 +802,8 @@ __global__ __launch_bounds__(256, accum_waves(N, sizeof(T))) void k_accum(const 
 }
 
 // ---------------------------------------------------------------------------
-// k_accum_lds: the same accumulation for grids whose pencil row fills one
-// wave (N = 24, 32).  One workgroup = four consecutive DOF slots of one
+// k_accum_lds: the same accumulation with FP32 pencils on the 32^3 grid
+// (a pencil row fills one wave).  One workgroup = four consecutive DOF slots of one
 // (target, row I'), one wave each.  The weight row block W[c][.][I'][.]
 // (N x N FP64 values) is the same for the four slots, so the workgroup loads
 // it once per contribution into a double-buffered LDS area, one contribution
@@ -813,7 +813,7 @@ __global__ __launch_bounds__(256, accum_waves(N, sizeof(T))) void k_accum(const 
 // only.
 // ---------------------------------------------------------------------------
 template <int N, typename T>
-__global__ __launch_bounds__(256, accum_waves(N, sizeof(T))) void k_accum_lds(
+__global__ __launch_bounds__(256, 2) void k_accum_lds(
     const cpx<T> *__restrict__ P, const cpx<double> *__restrict__ F, const cpx<double> *__restrict__ W,
     const int *__restrict__ c_group, const int *__restrict__ t_c0, const int *__restrict__ t_cn,
     const int *__restrict__ l_src, const int *__restrict__ t_l0, const int *__restrict__ t_ln, int ncontribs,
@@ -837,23 +837,10 @@ __global__ __launch_bounds__(256, accum_waves(N, sizeof(T))) void k_accum_lds(
     const bool act = J < N;
     const int64_t off = int64_t(I) * N + J;
     const int c0 = t_c0[t], cn = t_cn[t];
-    // weight copies: element idx = K' * N + J' of the row block
+    // weight copies: element idx = K' * N + J' of the row block (written out
+    // in place below; a lambda capturing the array would put it in scratch)
     cpx<double> wp[CP];
-    auto wload = [&](int ci) {
-        const cpx<double> *w = W + int64_t(ci) * G::VOL + int64_t(I) * N;
-#pragma unroll
-        for (int k = 0; k < CP; ++k) {
-            const int idx = int(threadIdx.x) + 256 * k;
-            if (WR % 256 == 0 || idx < WR) wp[k] = w[int64_t(idx / N) * G::N2 + idx % N];
-        }
-    };
-    auto wstore = [&](int buf) {
-#pragma unroll
-        for (int k = 0; k < CP; ++k) {
-            const int idx = int(threadIdx.x) + 256 * k;
-            if (WR % 256 == 0 || idx < WR) wl[buf][idx] = wp[k];
-        }
-    };
+    const cpx<double> *wrow = W + int64_t(I) * N;
     const cpx<T> *pbase = P + int64_t(live ? slot : 0) * ngroups * G::VOL + off;
     auto load_pencil = [&](int ci, cpx<T> *dst) {
         const cpx<T> *src = pbase + int64_t(c_group[ci]) * G::VOL;
@@ -867,14 +854,40 @@ __global__ __launch_bounds__(256, accum_waves(N, sizeof(T))) void k_accum_lds(
     // loaded in place (no register room for both)
     constexpr bool PF = sizeof(T) == 4;
     cpx<T> nxt[PF ? E : 1];
+    // compile-time copy loops (constant indices keep wp in registers)
     if (cn > 0) {
-        wload(c0);
+        fq::sfor<0, CP>([&](auto kk) {
+            constexpr int k = decltype(kk)::value;
+            const int idx = int(threadIdx.x) + 256 * k;
+            if (WR % 256 == 0 || idx < WR) {
+                const cpx<double> *q = wrow + int64_t(c0) * G::VOL + int64_t(idx / N) * G::N2 + idx % N;
+                wp[k] = {q->x, q->y};           // field copies: no aggregate memcpy
+            }
+        });
         if constexpr (PF) load_pencil(c0, nxt);
     }
     for (int ci = c0, it = 0; ci < c0 + cn; ++ci, ++it) {
         const int buf = it & 1;
-        wstore(buf);                                  // reads of buf ended before the last barrier
-        if (ci + 1 < c0 + cn) wload(ci + 1);
+        // reads of buf ended before the previous barrier
+        fq::sfor<0, CP>([&](auto kk) {
+            constexpr int k = decltype(kk)::value;
+            const int idx = int(threadIdx.x) + 256 * k;
+            if (WR % 256 == 0 || idx < WR) {
+                wl[buf][idx].x = wp[k].x;
+                wl[buf][idx].y = wp[k].y;
+            }
+        });
+        // next contribution's weights: the loads stay in flight across the
+        // barrier and the transform below
+        const int cn1 = ci + 1 < c0 + cn ? ci + 1 : ci;
+        fq::sfor<0, CP>([&](auto kk) {
+            constexpr int k = decltype(kk)::value;
+            const int idx = int(threadIdx.x) + 256 * k;
+            if (WR % 256 == 0 || idx < WR) {
+                const cpx<double> *q = wrow + int64_t(cn1) * G::VOL + int64_t(idx / N) * G::N2 + idx % N;
+                wp[k] = {q->x, q->y};
+            }
+        });
         __syncthreads();
         cpx<T> v[E];
         if constexpr (PF) {
