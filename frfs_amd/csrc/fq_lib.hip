@@ -802,8 +802,8 @@ __global__ __launch_bounds__(256, accum_waves(N, sizeof(T))) void k_accum(const 
 }
 
 // ---------------------------------------------------------------------------
-// k_accum_lds: the same accumulation with FP32 pencils on the 32^3 grid
-// (a pencil row fills one wave).  One workgroup = four consecutive DOF slots of one
+// k_accum_lds: the same accumulation on the 32^3 grid (a pencil row fills
+// one wave).  One workgroup = four consecutive DOF slots of one
 // (target, row I'), one wave each.  The weight row block W[c][.][I'][.]
 // (N x N FP64 values) is the same for the four slots, so the workgroup loads
 // it once per contribution into a double-buffered LDS area, one contribution
@@ -1331,7 +1331,10 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
                                    h->g_np, h->item_src, ngroups, nsrc);
             post(7);
         } else {
-        bool multi = false;
+        // the accumulating variant also for single-product groups in FP64:
+        // with its accumulator live across the item stages the compiler
+        // keeps the FP64 kernel within 256 VGPRs without spills
+        bool multi = sizeof(T) == 8;
         for (int g = 0; g < ngroups; ++g) multi |= P.group_np[g] > 1;
         if (multi)
             hipLaunchKernelGGL((k_prod<N, T, true>), grid, blk, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0,
@@ -1360,7 +1363,7 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     }
     u = int64_t(D) * ntgt * N;
     pre();
-    if constexpr (N == 32 && sizeof(T) == 4)
+    if constexpr (N == 32)
         hipLaunchKernelGGL((k_accum_lds<N, T>), dim3(unsigned(int64_t((D + 3) / 4) * ntgt * N)), dim3(256), 0, st,
                            Pg, h->F, h->W, h->c_group, h->t_c0, h->t_cn, h->l_src, h->t_l0, h->t_ln,
                            int(P.contribs.size()), Y, ngroups, nsrc, ntgt, D);
