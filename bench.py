@@ -27,7 +27,7 @@ HBM_PEAK = 8000.0                          # GB/s
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (1272 DOFs per launch),
 # scaled by what the same counters read for a 1 GiB device copy (FETCH x2.0,
 # WRITE x1.0); the summary with its command is profiles/k_prod_vss32_traffic.txt.
-K_PROD_TRAFFIC_PER_DOF = {'mixed': 58.42e6, 'fp64': 83.76e6}
+K_PROD_TRAFFIC_PER_DOF = {'mixed': 27.44e6, 'fp64': 57.70e6}
 
 
 def parse():
