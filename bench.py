@@ -33,9 +33,9 @@ K_PROD_TRAFFIC_PER_DOF = {'mixed': 27.44e6, 'fp64': 57.70e6}
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--workload', default='vss32', choices=['vss32', 'vss32x2', 'rhs24'],
-                    help='vss32: the Q(f,f) headline; vss32x2: two species, four pairs; '
-                         'rhs24: full SSP-RK2 step (FR advection + Q)')
+    ap.add_argument('--workload', default='vss32', choices=['vss32', 'vhs16', 'vss32x2', 'rhs24'],
+                    help='vss32: the Q(f,f) headline; vhs16: plain FP64 16^3, 64 DOFs; '
+                         'vss32x2: two species, four pairs; rhs24: full SSP-RK2 step (FR advection + Q)')
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--ndof', type=int, default=4096)
@@ -214,6 +214,67 @@ def bench_multi(a):
     print(json.dumps(out))
 
 
+def bench_vhs16(a):
+    """The survey's first configuration: plain FP64 vhs-gll (omega = 0.5),
+    16^3 grid on [-5,5]^3, M = 6 half-sphere directions, Nrho = 8, 64 DOFs of
+    the seeded two-Maxwellian mixture, checked against the oracle on every
+    DOF (the oracle also gives the CPU rate on the same DOFs)."""
+    import torch
+    from frfs_amd.backend import HIPBackend
+    from frfs_amd.inifile import Inifile
+    from frfs_amd.inputs import from_aosoa, mixture_device, mixture_params, to_aosoa
+    from frfs_amd.scattering import DGFSVHSGLLScatteringModel
+    from frfs_amd.velocitymesh import DGFSVelocityMesh
+    from oracle import fastspectral as fsp
+    ndof, N, Nrho, M = 64, 16, 8, 6
+    fpar = mixture_params(ndof, a.seed)
+    # oracle first (CPU, before any GPU work)
+    vm_o = fsp.velocity_mesh(N, Nrho, M, dev=5.0)
+    prm = fsp.vhs_parameters(vm_o, 0.5, 4.17e-10, 273.0)
+    tab = fsp.vhs_tables(vm_o, prm['gamma'])
+    fo = np.zeros((ndof, N ** 3))
+    for d in range(ndof):
+        for k in range(2):
+            nk, ux, uy, uz, Tk = fpar[d, k]
+            fo[d] += fsp.maxwellian(vm_o['cv'], nk, [ux, uy, uz], Tk)
+    t0 = time.perf_counter()
+    ref = np.stack([fsp.vhs_fs(fo[d], vm_o, tab, prm['prefac']) for d in range(ndof)])
+    cpu_s = time.perf_counter() - t0
+    be = HIPBackend()
+    cfg = Inifile.from_dict({
+        'non-dim': dict(T0=273.0, H0=1e-3, rho0=0.00011140244750324529, molarMass0=39.96e-3),
+        'constants': {'Nv': N, 'NvBatchSize': N ** 3, 'Nrho': Nrho},
+        'velocity-mesh': {'dev': 5.0, 'cmax': 0, 'Tmax': 273},
+        'spherical-design-rule': {'ssrule': 'womersley', 'M': M},
+        'scattering-model': {'type': 'vhs-gll', 'omega': 0.5, 'dRef': 4.17e-10, 'Tref': 273,
+                             'precision': 'double'}})
+    vm = DGFSVelocityMesh(be, cfg)
+    sm = DGFSVHSGLLScatteringModel(be, cfg, vm)
+    n = vm.vsize()
+    fin, qout = be.matrix((1, n, ndof)), be.matrix((1, n, ndof))
+    to_aosoa(mixture_device(vm.cv(), fpar), fin)
+    torch.cuda.synchronize()
+    for _ in range(a.warmup):
+        sm.fs_batch(fin, qout)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        sm.fs_batch(fin, qout)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
+    q = from_aosoa(qout).cpu().numpy()
+    err = float(np.max(np.linalg.norm(q - ref, axis=1) / np.linalg.norm(ref, axis=1)))
+    print(json.dumps({
+        'metric': 'plain FP64 Q(f,f) DOF-evals/s, vhs-gll 16^3, 64 DOFs', 'value': ndof / dt,
+        'unit': 'DOF-evals/s', 'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': dt * 1e3,
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic',
+        'config': {'workload': 'vhs-gll omega=0.5, 16^3 grid on [-5,5]^3, M=6 half sphere, Nrho=8, 64 DOFs of '
+                               'seeded two-Maxwellian f'},
+        'max_dof_rel_l2_vs_oracle': err, 'plan': sm.plan_info(),
+        'cpu_baseline': {'value': ndof / cpu_s, 'unit': 'DOF-evals/s', 'cores': 1, 'kind': 'port',
+                         'sample': f'all {ndof} DOFs, NumPy complex128 oracle, one process'}}))
+
+
 def bench_rhs(a):
     """Full explicit step: 32x32 periodic quads, p = 2 (9216 DOFs), 24^3 velocity
     grid on [-5,5]^3, vhs-gll M = 6 (the 12-point design as 6 antipodal pairs),
@@ -283,6 +344,9 @@ def main():
         sys.exit(2)
     if a.workload == 'rhs24':
         bench_rhs(a)
+        return
+    if a.workload == 'vhs16':
+        bench_vhs16(a)
         return
     if a.workload == 'vss32x2':
         bench_multi(a)
