@@ -62,16 +62,6 @@ template <int N> struct Geo {
 constexpr int kPencilThreads = 256;           // pencil kernels (no LDS)
 constexpr int kPlaneThreads = 128;            // plane kernels (LDS transpose per unit)
 
-template <typename T> __device__ __forceinline__ cpx<T> ld(const cpx<T> *p) { return *p; }
-
-// Element idx of a workgroup-uniform base.  The byte offset is formed in 32
-// bits, so the load takes the SGPR-base + 32-bit VGPR-offset form instead of a
-// 64-bit per-lane address.
-template <typename T> __device__ __forceinline__ cpx<T> ldu(const cpx<T> *base, uint32_t idx) {
-    return *reinterpret_cast<const cpx<T> *>(reinterpret_cast<const char *>(base) +
-                                             uint32_t(idx * uint32_t(sizeof(cpx<T>))));
-}
-
 // Buffer view of a workgroup-uniform array of `bytes` bytes.  Loads take a
 // per-lane 32-bit byte offset plus a uniform byte offset held in an SGPR, so
 // a run of loads at constant strides shares one address register.  An offset
@@ -841,11 +831,24 @@ __global__ __launch_bounds__(256, 2) void k_accum_lds(
     // in place below; a lambda capturing the array would put it in scratch)
     cpx<double> wp[CP];
     const cpx<double> *wrow = W + int64_t(I) * N;
-    const cpx<T> *pbase = P + int64_t(live ? slot : 0) * ngroups * G::VOL + off;
+    // FP64 pencils load through a buffer view of the (wave-uniform) group
+    // volume (one lane offset register, plane offsets in SGPRs), which frees
+    // the registers that keep the FP64 kernel at three waves per SIMD; the
+    // prefetching FP32 kernel measured faster with plain loads
+    const cpx<T> *pslot = P + int64_t(live ? slot : 0) * ngroups * G::VOL;
+    const uint32_t voP = uint32_t(off + int64_t(half) * G::N2) * uint32_t(sizeof(cpx<T>));
     auto load_pencil = [&](int ci, cpx<T> *dst) {
-        const cpx<T> *src = pbase + int64_t(c_group[ci]) * G::VOL;
+        if constexpr (sizeof(T) == 8) {
+            const BufView pv(pslot + int64_t(c_group[ci]) * G::VOL, uint32_t(G::VOL * sizeof(cpx<T>)));
 #pragma unroll
-        for (int e = 0; e < E; ++e) dst[e] = (live && act) ? src[int64_t(half + 2 * e) * G::N2] : cpx<T>{0, 0};
+            for (int e = 0; e < E; ++e)
+                dst[e] = (live && act) ? pv.ld<T>(voP, uint32_t(2 * e * G::N2 * sizeof(cpx<T>))) : cpx<T>{0, 0};
+        } else {
+            const cpx<T> *src = pslot + int64_t(c_group[ci]) * G::VOL + off;
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                dst[e] = (live && act) ? src[int64_t(half + 2 * e) * G::N2] : cpx<T>{0, 0};
+        }
     };
     cpx<double> H[E];
 #pragma unroll
@@ -959,8 +962,8 @@ __device__ __forceinline__ void plane_inv64(const cpx<double> *src, cpx<double> 
 }
 
 template <int N>
-__global__ __launch_bounds__(kPlaneThreads) void k_out(const cpx<double> *Y, AosoaPtrs ptr, Dof dof, int ntgt,
-                                             int64_t nunits) {
+__global__ __launch_bounds__(kPlaneThreads, 2) void k_out(const cpx<double> *Y, AosoaPtrs ptr, Dof dof, int ntgt,
+                                                int64_t nunits) {
     using G = Geo<N>;
     constexpr int E = G::E;
     constexpr int UPB = kPlaneThreads / G::LPP;
@@ -977,8 +980,15 @@ __global__ __launch_bounds__(kPlaneThreads) void k_out(const cpx<double> *Y, Aos
     const bool act = valid && c < N;
     cpx<double> *L = lds[u];
     const cpx<double> *yg = Y + ((int64_t(slot) * ntgt + t) * 2 + 0) * G::VOL + int64_t(vz) * G::N2;
-    cpx<double> gv[E], lv[E];
-    plane_inv64<N>(yg, L, gv, c, half, act);
+    // only the real parts of the gain and loss transforms are used
+    double gr[E];
+    {
+        cpx<double> gv[E];
+        plane_inv64<N>(yg, L, gv, c, half, act);
+#pragma unroll
+        for (int e = 0; e < E; ++e) gr[e] = gv[e].x;
+    }
+    cpx<double> lv[E];
     plane_inv64<N>(yg + G::VOL, L, lv, c, half, act);
     if (act) {
         const double *f = ptr.f[t];
@@ -989,7 +999,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_out(const cpx<double> *Y, Aos
             const int vy = half + 2 * e;
             const int64_t vv = (int64_t(c) * N + vy) * N + vz;
             const int64_t a = b + vv * dof.soasz;
-            q[a] = gv[e].x - f[a] * lv[e].x;
+            q[a] = gr[e] - f[a] * lv[e].x;
         }
     }
 }
