@@ -124,6 +124,32 @@ def test_quadratic_homogeneity(vhs16):
     assert np.linalg.norm(q3 - 9 * q1) / np.linalg.norm(9 * q1) < 1e-13
 
 
+def test_nodal_bilinear_form_reduces_to_quadratic(vhs16):
+    """Q(f, g) of vhs-gll-nodal with g = f is Q(f) of vhs-gll: the same
+    tables, weights and loss term, so the bilinear oracle is pinned by the
+    quadratic one (itself pinned by BKW and the identities below)."""
+    vm, prm, tab, f = vhs16
+    q = fsp.vhs_fs(f[0], vm, tab, prm['prefac'])
+    qn = fsp.vhs_nodal_fs(f[0], f[0], vm, tab, prm['prefac'])
+    assert np.linalg.norm(qn - q) / np.linalg.norm(q) < 1e-13
+
+
+def test_nodal_is_bilinear(vhs16):
+    """Gain and loss are linear in each argument separately."""
+    vm, prm, tab, f = vhs16
+    pf = prm['prefac']
+    a, b = 0.7, -1.9
+    g = f[1]
+    lhs = fsp.vhs_nodal_fs(a * f[0] + b * g, g, vm, tab, pf)
+    rhs = a * fsp.vhs_nodal_fs(f[0], g, vm, tab, pf) + b * fsp.vhs_nodal_fs(g, g, vm, tab, pf)
+    assert np.linalg.norm(lhs - rhs) / np.linalg.norm(rhs) < 1e-12
+    lhs = fsp.vhs_nodal_fs(f[0], a * f[0] + b * g, vm, tab, pf)
+    rhs = a * fsp.vhs_nodal_fs(f[0], f[0], vm, tab, pf) + b * fsp.vhs_nodal_fs(f[0], g, vm, tab, pf)
+    assert np.linalg.norm(lhs - rhs) / np.linalg.norm(rhs) < 1e-12
+    # and not symmetric: the loss term is f * L(g), so swapping changes it
+    assert np.linalg.norm(fsp.vhs_nodal_fs(f[0], g, vm, tab, pf) - fsp.vhs_nodal_fs(g, f[0], vm, tab, pf)) > 1e-3 * np.linalg.norm(rhs)
+
+
 def test_shift_equivariance(vhs16):
     vm, prm, tab, f = vhs16
     N = vm['Nv']
