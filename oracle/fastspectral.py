@@ -26,7 +26,8 @@ spherical-design invariants, FFT conventions vs a brute-force DFT, the sinc
 tables vs spherical quadrature of plane waves, the exact identities
 Q(cf)=c^2 Q(f) and shift equivariance, the half-sphere cos/sin form vs the
 full-sphere product form, VSS(alpha=1) vs VHS, the collision invariants,
-Q(Maxwellian) ~ 0, and the BKW exact solution's time derivative.
+Q(Maxwellian) ~ 0, the BKW exact solution's time derivative, and for the
+bilinear Q(f, g) its reduction to Q(f) at g = f and bilinearity.
 """
 from __future__ import annotations
 
