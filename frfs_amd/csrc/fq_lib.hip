@@ -4,19 +4,23 @@
 // [plane][I][J] (J fastest, plane = the z index K or vz):
 //   k_fwd_z    f (AoSoA, per source)  -> T0  : DFT- along z           (r2z + cuFFT fwd)
 //   k_fwd_xy   T0 -> F (FP64)                : DFT- along x, y, / n
-//   k_inv_z    F -> Z_i  (T)                 : e^{i th_z l_z} . F, DFT+ along z
-//   k_plane    Z -> P_g  (T)                 : e^{i(th_x l_x + th_y l_y)}, DFT+ x,y,
-//                                              products, sum in group, DFT- x,y
-//   k_accum    P, F -> Y (FP64)              : DFT- along z, H += W . Ghat (FP64),
+//   k_cvt      F -> F32                      : FP32 copy (mixed precision)
+//   k_prod / k_prod1  F -> P_g (T)           : per group: the items' pruned z
+//                                              DFTs straight into LDS, x,y DFT+,
+//                                              products, group sum, DFT- x,y
+//     (staged alternative, FQ_FLAG_UNFUSED: k_inv_z F -> Z_i through HBM,
+//      then k_plane Z -> P_g)
+//   k_accum / k_accum_lds  P, F -> Y (FP64)  : DFT- along z, H += W . Ghat (FP64),
 //                                              L += Lt . F, DFT+ along z of H and L
 //   k_out      Y -> Q (AoSoA, per target)    : DFT+ x,y; Q = Re gain - f Re loss
-//   k_proj_*   optional conservation projection (FP64)
+//   k_project  optional conservation projection (FP64)
 // T = double (FQ_PREC_FP64) or float (FQ_PREC_MIXED).
 //
 // Lane mapping: a "unit" (one plane or one row of N pencils) is LPP = 2*NP
 // lanes, NP = N rounded up to a power of two; lane = 2*J + half, the two
-// lanes of a pencil exchange through DPP (see fft.hpp).  Global loads and
-// stores walk J across lanes, so every access is a contiguous row segment.
+// lanes of a pencil exchange through DPP (see fft.hpp).  k_prod1 instead
+// puts one pencil on each lane (FP32, N = 24, 32).  Global loads and stores
+// walk J across lanes, so every access is a contiguous row segment.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
