@@ -86,20 +86,23 @@ def test_convergence_order():
 
 
 @pytest.mark.gpu
-def test_gpu_kernel_matches_oracle(backend):
+@pytest.mark.parametrize('order,pts', [(1, 'gauss-legendre'), (2, 'gauss-legendre'), (3, 'gauss-legendre'),
+                                       (2, 'gauss-legendre-lobatto'), (3, 'gauss-legendre-lobatto')])
+def test_gpu_kernel_matches_oracle(backend, order, pts):
     import torch
     from frfs_amd.fr import PeriodicQuadAdvection
     from frfs_amd.velocitymesh import DGFSVelocityMesh
     from tests.cases import vhs_cfg
     vm = DGFSVelocityMesh(backend, vhs_cfg(8, 2, 6, 4.0))
     nelx, nely, hx, hy = 6, 5, 0.25, 0.4
-    op = PeriodicQuadAdvection(backend, vm, nelx, nely, hx, hy, order=2)
+    op = PeriodicQuadAdvection(backend, vm, nelx, nely, hx, hy, order=order, upts=pts, fpts=pts)
     rng = np.random.default_rng(2)
-    u = rng.normal(size=(9, vm.vsize(), nelx * nely))
+    u = rng.normal(size=((order + 1) ** 2, vm.vsize(), nelx * nely))
     ub = backend.matrix(u.shape, u)
     ob = backend.matrix(u.shape)
+    print(f'[fr order={order} {pts}]', flush=True)
     op(ub, ob)
     torch.cuda.synchronize()
-    want = ofr.fr_rhs(u, vm.cv()[0], vm.cv()[1], nelx, nely, hx, hy, ofr.quad_operators(2))
+    want = ofr.fr_rhs(u, vm.cv()[0], vm.cv()[1], nelx, nely, hx, hy, ofr.quad_operators(order, pts, pts))
     got = ob.get()
     assert np.max(np.abs(got - want)) < 1e-12 * np.max(np.abs(want))
