@@ -25,6 +25,7 @@ MODEL_VHS, MODEL_VHS_NODAL, MODEL_VSS = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 FLAG_PROJECT = 1
 FLAG_UNFUSED = 2
+FLAG_TWO_LANE = 4
 
 _dp = C.POINTER(C.c_double)
 

@@ -1,8 +1,11 @@
 """Schedule autotuner for the collision operator.
 
-A schedule is the pair (product stage, DOFs per internal batch): the fused
-product kernel or the staged path that writes the inverse z transforms to
-HBM, and either the largest batch the scratch budget allows or a smaller one.
+A schedule is the triple (product stage, pencil layout, DOFs per internal
+batch): the fused product kernel or the staged path that writes the inverse z
+transforms to HBM; for FP32 transforms on the 24^3 and 32^3 grids, one lane
+per pencil (in-register radix-4/8 transforms) or two (radix-2 exchange
+between lanes); and the largest batch the scratch budget allows or a smaller
+one.
 ``tune`` builds one handle per candidate on the current device, times each on
 the caller's own inputs (one untimed call, then ``reps`` timed calls between
 device synchronisations), keeps the fastest handle and releases the others.
@@ -15,9 +18,10 @@ import time
 from . import _lib
 
 CANDIDATES = (
-    {'fused': 1, 'max_chunk': 0},
-    {'fused': 1, 'max_chunk': 512},
-    {'fused': 0, 'max_chunk': 0},
+    {'fused': 1, 'max_chunk': 0, 'two_lane': 0},
+    {'fused': 1, 'max_chunk': 512, 'two_lane': 0},
+    {'fused': 1, 'max_chunk': 0, 'two_lane': 1},
+    {'fused': 0, 'max_chunk': 0, 'two_lane': 0},
 )
 
 _CACHE = {}
@@ -30,7 +34,12 @@ def _sync():
 
 def _apply(params, cand):
     c = params.c
-    c.flags = (c.flags & ~_lib.FLAG_UNFUSED) | (0 if cand['fused'] else _lib.FLAG_UNFUSED)
+    fl = c.flags & ~(_lib.FLAG_UNFUSED | _lib.FLAG_TWO_LANE)
+    if not cand['fused']:
+        fl |= _lib.FLAG_UNFUSED
+    if cand.get('two_lane'):
+        fl |= _lib.FLAG_TWO_LANE
+    c.flags = fl
     c.max_chunk = cand['max_chunk']
 
 

@@ -1120,6 +1120,7 @@ struct fq_handle {
     double *pmass = nullptr;     // [ns] species mass ratios used by the projection
     void *zc = nullptr;          // [items][4][4 + N/4] k_prod z-stage coefficients in T
     bool fused = true;           // k_prod path (else k_inv_z + k_plane)
+    bool two_lane = false;       // FP32 24^3/32^3: two-lane k_prod instead of one-lane k_prod1
     cpx<float> *F32 = nullptr;   // FP32 copy of F (mixed, fused)
     // scratch
     cpx<double> *F = nullptr;
@@ -1330,19 +1331,18 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
         pre();
         const cpx<T> *zc = reinterpret_cast<const cpx<T> *>(h->zc);
         const dim3 grid(unsigned(int64_t(D) * ngroups * 4)), blk(ProdGeo<N>::THREADS);
-#ifndef FQ_PROD_ONE_LANE
-#define FQ_PROD_ONE_LANE 1
-#endif
-        if constexpr (FQ_PROD_ONE_LANE && (N == 24 || N == 32) && sizeof(T) == 4) {
+        bool one_lane = false;
+        if constexpr ((N == 24 || N == 32) && sizeof(T) == 4) one_lane = !h->two_lane;
+        if (one_lane) {
             bool multi1 = false;
             for (int g = 0; g < ngroups; ++g) multi1 |= P.group_np[g] > 1;
             const dim3 blk1(Prod1Geo<N>::THREADS);
             if (multi1)
-                hipLaunchKernelGGL((k_prod1<N, T, true>), grid, blk1, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0,
-                                   h->g_np, h->item_src, ngroups, nsrc);
+                    hipLaunchKernelGGL((k_prod1<N, T, true>), grid, blk1, 0, st, FT, Pg, ph, zc, h->prods,
+                                   h->g_p0, h->g_np, h->item_src, ngroups, nsrc);
             else
-                hipLaunchKernelGGL((k_prod1<N, T, false>), grid, blk1, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0,
-                                   h->g_np, h->item_src, ngroups, nsrc);
+                hipLaunchKernelGGL((k_prod1<N, T, false>), grid, blk1, 0, st, FT, Pg, ph, zc, h->prods,
+                                   h->g_p0, h->g_np, h->item_src, ngroups, nsrc);
             post(7);
         } else {
         // the accumulating variant also for single-product groups in FP64:
@@ -1512,6 +1512,7 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     h->precision = prm->precision;
     h->flags = prm->flags;
     h->fused = !(prm->flags & FQ_FLAG_UNFUSED);
+    h->two_lane = (prm->flags & FQ_FLAG_TWO_LANE) != 0;
     if (hipGetDevice(&h->device) != hipSuccess) {
         delete h;
         return fail(FQ_ERR_HIP, "no HIP device");

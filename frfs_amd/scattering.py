@@ -15,7 +15,9 @@ Extra configuration keys (absent from the reference, whose operator is
 always FP64): ``[scattering-model] precision = double | mixed`` and
 ``conservation-projection = 0 | 1``; ``max-chunk`` caps the DOFs per
 internal batch; ``fused = 0`` selects the three-kernel path that stages the
-inverse transforms through HBM (for comparison); ``autotune = 1`` makes the
+inverse transforms through HBM (for comparison); ``pencil-lanes = 2`` selects
+the two-lanes-per-pencil FP32 product kernel on the 24^3 and 32^3 grids
+(default 1: one lane per pencil); ``autotune = 1`` makes the
 first ``fs_batch`` call time the candidate schedules on its own inputs and
 keep the fastest (``frfs_amd.autotune``; ``autotune()`` does it on demand).
 """
@@ -55,6 +57,8 @@ def _flags(cfg):
     fl = _lib.FLAG_PROJECT if cfg.getint('scattering-model', 'conservation-projection', 0) else 0
     if not cfg.getint('scattering-model', 'fused', 1):
         fl |= _lib.FLAG_UNFUSED
+    if cfg.getint('scattering-model', 'pencil-lanes', 1) == 2:
+        fl |= _lib.FLAG_TWO_LANE
     return fl
 
 

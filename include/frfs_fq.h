@@ -65,8 +65,11 @@ enum fq_flags {
     FQ_FLAG_PROJECT = 1,      /* least-squares FP64 correction of Q that zeroes the
                                  collision invariants: the mass of every species
                                  and the total (mass-weighted) momentum and energy */
-    FQ_FLAG_UNFUSED = 2       /* stage every inverse transform through HBM
+    FQ_FLAG_UNFUSED = 2,      /* stage every inverse transform through HBM
                                  (three-kernel path; for comparison) */
+    FQ_FLAG_TWO_LANE = 4      /* FP32 on 24^3/32^3 grids: product kernel with two
+                                 lanes per pencil instead of one (a schedule
+                                 candidate of the autotuner) */
 };
 
 /*

@@ -13,7 +13,7 @@ class _FakeHandle:
         if params.c.max_chunk == 512 and fused:
             from frfs_amd._lib import ERR_ALLOC, FQError
             raise FQError(ERR_ALLOC, 'no memory')
-        self.cost = 0.004 if fused else 0.001
+        self.cost = 0.001 if not fused else (0.003 if params.c.flags & 4 else 0.005)
         self.closed = False
         _FakeHandle.made.append(self)
 
@@ -39,9 +39,9 @@ def test_tune_picks_fastest_and_caches(monkeypatch):
     _FakeHandle.made = []
     p = _P()
     h, rep = autotune.tune(p, lambda h: time.sleep(h.cost), key=('k', 1), reps=1)
-    assert rep['choice'] == {'fused': 0, 'max_chunk': 0}
+    assert rep['choice'] == {'fused': 0, 'max_chunk': 0, 'two_lane': 0}
     assert not h.closed
-    assert [x.closed for x in _FakeHandle.made] == [True, False]   # the slower fused one released
+    assert [x.closed for x in _FakeHandle.made] == [True, True, False]   # slower ones released
     assert any('error' in t for t in rep['timings'])                 # failed candidate reported
     assert p.c.flags == 1 | 2 and p.c.max_chunk == 0                 # params keep the choice
     h2, rep2 = autotune.tune(_P(), lambda h: 1 / 0, key=('k', 1))    # cached: nothing is run
@@ -84,7 +84,7 @@ def test_tuned_schedule_matches_default(backend):
     tuned = DGFSBiVSSGLLScatteringModel(backend, cfg, vm)
     rep = tuned.autotune(fin, q1)
     print('autotune report', rep)
-    assert not rep['cached'] and len(rep['timings']) == 3
+    assert not rep['cached'] and len(rep['timings']) == len(autotune.CANDIDATES)
     q1.set(np.zeros(q1.ioshape))
     tuned.fs_batch(fin, q1)
     torch.cuda.synchronize()

@@ -136,12 +136,13 @@ def test_vss_single_species(backend, prec, tol, fused):
     assert max(errs) <= tol
 
 
-@pytest.mark.parametrize('prec,tol', [('double', FP64_TOL), ('mixed', MIXED_TOL)])
-def test_vss_single_species_32(backend, prec, tol):
-    """32^3 grid, one product per group: the one-lane-per-pencil FP32 product
-    kernel and the FP64 two-lane kernel against the oracle."""
+@pytest.mark.parametrize('prec,tol,lanes', [('double', FP64_TOL, 1), ('mixed', MIXED_TOL, 1),
+                                            ('mixed', MIXED_TOL, 2)])
+def test_vss_single_species_32(backend, prec, tol, lanes):
+    """32^3 grid, one product per group: the FP32 product kernel with one and
+    with two lanes per pencil, and the FP64 kernel, against the oracle."""
     from oracle import fastspectral as fsp
-    cfg = vss_cfg(32, 4, 12, 6.0, precision=prec)
+    cfg = vss_cfg(32, 4, 12, 6.0, precision=prec, lanes=lanes)
     vm, prm, tab = oracle_vss(cfg, 1)
     f = fsp.two_maxwellian_batch(vm['cv'], 2, seed=32)
     print(f'[vss-1 32^3 {prec}]', flush=True)
