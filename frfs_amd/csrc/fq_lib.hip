@@ -3,8 +3,7 @@
 // Pipeline per batch ("chunk") of D DOFs, all volumes stored plane-major
 // [plane][I][J] (J fastest, plane = the z index K or vz):
 //   k_fwd_z    f (AoSoA, per source)  -> T0  : DFT- along z           (r2z + cuFFT fwd)
-//   k_fwd_xy   T0 -> F (FP64)                : DFT- along x, y, / n
-//   k_cvt      F -> F32                      : FP32 copy (mixed precision)
+//   k_fwd_xy   T0 -> F (FP64)                : DFT- along x, y, / n (+ FP32 copy, mixed)
 //   k_prod / k_prod1  F -> P_g (T)           : per group: the items' pruned z
 //                                              DFTs straight into LDS, x,y DFT+,
 //                                              products, group sum, DFT- x,y
@@ -214,9 +213,11 @@ __global__ __launch_bounds__(256) void k_fwd_z(AosoaPtrs ptr, Dof dof, cpx<doubl
 // column J in the column phase and the row (x index) in the row phase.
 // ---------------------------------------------------------------------------
 
-// k_fwd_xy: plane (slot, src, K'):  DFT- along x then y, divide by n.
+// k_fwd_xy: plane (slot, src, K'):  DFT- along x then y, divide by n; with
+// F32 non-null it also writes the FP32 copy the mixed-precision path reads.
 template <int N>
-__global__ __launch_bounds__(kPlaneThreads) void k_fwd_xy(const cpx<double> *T0, cpx<double> *F, int64_t nunits) {
+__global__ __launch_bounds__(kPlaneThreads) void k_fwd_xy(const cpx<double> *T0, cpx<double> *F,
+                                                           cpx<float> *F32, int64_t nunits) {
     using G = Geo<N>;
     constexpr int E = G::E;
     constexpr int UPB = kPlaneThreads / G::LPP;
@@ -253,7 +254,11 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_xy(const cpx<double> *T0,
     cpx<double> *dst = F + unit * G::N2;
     if (act) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) dst[(half * E + e) * N + c] = L[(half * E + e) * G::LDSROW + c];
+        for (int e = 0; e < E; ++e) {
+            const cpx<double> x = L[(half * E + e) * G::LDSROW + c];
+            dst[(half * E + e) * N + c] = x;
+            if (F32) F32[unit * G::N2 + (half * E + e) * N + c] = cvt<float>(x);   // mixed precision copy
+        }
     }
 }
 
@@ -675,11 +680,6 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
     }
 }
 
-// FP64 -> FP32 copy of the source spectra (mixed precision).
-__global__ __launch_bounds__(256) void k_cvt(const cpx<double> *F, cpx<float> *F32, int64_t n) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) F32[i] = cvt<float>(F[i]);
-}
 
 // Minimum waves per SIMD requested for k_accum: three for FP32 transforms
 // (the 24-point radix-3 variant needs more registers and keeps two), two for FP64.
@@ -1134,7 +1134,7 @@ struct fq_handle {
 };
 
 static const char *kKernelNames[FQ_NKERNELS] = {"k_fwd_z", "k_fwd_xy", "k_inv_z", "k_plane",
-                                                "k_accum", "k_out", "k_project", "k_prod", "k_cvt"};
+                                                "k_accum", "k_out", "k_project", "k_prod"};
 
 namespace {
 
@@ -1316,18 +1316,14 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     pre();
     hipLaunchKernelGGL(k_fwd_z<N>, dim3(blocks(u)), bt, 0, st, ptr, dof, T0, nsrc, u);
     post(0);
+    // the fused FP32 path reads an FP32 copy of F, written by k_fwd_xy itself
+    const bool want32 = h->fused && std::is_same<T, float>::value;
     pre();
-    hipLaunchKernelGGL(k_fwd_xy<N>, dim3(pblocks(u)), pt, 0, st, T0, h->F, u);
+    hipLaunchKernelGGL(k_fwd_xy<N>, dim3(pblocks(u)), pt, 0, st, T0, h->F, want32 ? h->F32 : nullptr, u);
     post(1);
     if (h->fused) {
-        const cpx<T> *FT = reinterpret_cast<const cpx<T> *>(h->F);
-        if constexpr (std::is_same<T, float>::value) {
-            const int64_t ne = int64_t(D) * nsrc * G::VOL;
-            pre();
-            hipLaunchKernelGGL(k_cvt, dim3(unsigned((ne + 255) / 256)), dim3(256), 0, st, h->F, h->F32, ne);
-            post(8);
-            FT = reinterpret_cast<const cpx<T> *>(h->F32);
-        }
+        const cpx<T> *FT = reinterpret_cast<const cpx<T> *>(want32 ? static_cast<void *>(h->F32)
+                                                                   : static_cast<void *>(h->F));
         pre();
         const cpx<T> *zc = reinterpret_cast<const cpx<T> *>(h->zc);
         const dim3 grid(unsigned(int64_t(D) * ngroups * 4)), blk(ProdGeo<N>::THREADS);

@@ -162,7 +162,7 @@ int fq_eval_bilinear(fq_handle *h, const double *f, const double *g, double *q,
 
 /* ---- instrumentation --------------------------------------------------- */
 
-#define FQ_NKERNELS 9
+#define FQ_NKERNELS 8
 /* Bracket every kernel launch with HIP events on the launch stream (on = 1)
  * or stop (on = 0); either call discards earlier records.  No reference
  * counterpart: the reference times only its precompute with cuda.Event
