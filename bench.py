@@ -50,9 +50,19 @@ def parse():
     ap.add_argument('--no-reference-proj', action='store_true',
                     help='skip the untimed FP64-with-projection reference evaluation')
     ap.add_argument('--modes', default='fp64,mixed,mixedproj')
+    ap.add_argument('--sm-opt', action='append', default=[], metavar='KEY=VALUE',
+                    help='extra [scattering-model] key, e.g. pair-pack=0 (repeatable)')
     ap.add_argument('--calib-copy-mib', type=int, default=0,
                     help='run one device-to-device copy of this size first (counter calibration)')
     return ap.parse_args()
+
+
+def sm_opts(a, sm):
+    """[scattering-model] section sm with the --sm-opt overrides applied."""
+    for kv in a.sm_opt:
+        k, v = kv.split('=', 1)
+        sm[k.strip()] = v.strip()
+    return sm
 
 
 def vss_cfg(a, precision, project=0):
@@ -62,10 +72,10 @@ def vss_cfg(a, precision, project=0):
         'constants': {'Nv': a.Nv, 'NvBatchSize': a.Nv ** 3, 'Nrho': a.Nrho},
         'velocity-mesh': {'dev': 6.0, 'cmax': 0, 'Tmax': 273, 'molarMass1': 39.96e-3},
         'spherical-design-rule': {'ssrule': 'womersley', 'M': a.M},
-        'scattering-model': {'type': 'vss-gll', 'ssrulepre': 'womersley', 'Mpre': a.Mpre,
-                             'alpha00': 1.4, 'omega00': 0.74, 'dRef00': 4.17e-10,
-                             'Tref00': 273, 'precision': precision,
-                             'conservation-projection': project},
+        'scattering-model': sm_opts(a, {'type': 'vss-gll', 'ssrulepre': 'womersley', 'Mpre': a.Mpre,
+                                        'alpha00': 1.4, 'omega00': 0.74, 'dRef00': 4.17e-10,
+                                        'Tref00': 273, 'precision': precision,
+                                        'conservation-projection': project}),
     })
 
 
@@ -166,7 +176,7 @@ def bench_multi(a):
             'velocity-mesh': {'dev': 6.0, 'cmax': 0, 'Tmax': 273, 'molarMass1': 39.96e-3,
                               'molarMass2': 2 * 39.96e-3},
             'spherical-design-rule': {'ssrule': 'womersley', 'M': a.M},
-            'scattering-model': sm})
+            'scattering-model': sm_opts(a, sm)})
         vm = DGFSBiVelocityMesh(be, cfg, 2)
         model = DGFSBiVSSGLLScatteringModel(be, cfg, vm)
         n = vm.vsize()
@@ -295,9 +305,9 @@ def bench_rhs(a):
             'constants': {'Nv': N, 'NvBatchSize': N ** 3, 'Nrho': 12},
             'velocity-mesh': {'dev': 5.0, 'cmax': 0, 'Tmax': 273},
             'spherical-design-rule': {'ssrule': 'womersley', 'M': 6},
-            'scattering-model': {'type': 'vhs-gll', 'omega': 0.5, 'dRef': 4.17e-10, 'Tref': 273,
-                                 'precision': 'mixed' if mode == 'mixed' else 'double',
-                                 'conservation-projection': 1 if mode == 'mixed' else 0}})
+            'scattering-model': sm_opts(a, {'type': 'vhs-gll', 'omega': 0.5, 'dRef': 4.17e-10, 'Tref': 273,
+                                            'precision': 'mixed' if mode == 'mixed' else 'double',
+                                            'conservation-projection': 1 if mode == 'mixed' else 0})})
         sysm = DGFSCollisionSystem(be, cfg, 9, nel * nel)
         adv = PeriodicQuadAdvection(be, sysm.vm, nel, nel, 1.0 / nel, 1.0 / nel, order=2)
         sysm.rhs = lambda t, uin, fout: adv(sysm.banks[uin], sysm.banks[fout])

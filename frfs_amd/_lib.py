@@ -18,7 +18,7 @@ LIBPATH = os.environ.get('FRFS_FQ_LIB') or os.path.join(
 ABI_VERSION = 1
 
 SUCCESS, ERR_INVALID, ERR_UNSUPPORTED, ERR_HIP, ERR_ALLOC = 0, -1, -2, -3, -4
-NKERNELS = 8
+NKERNELS = 9
 AXNPBY_MAX = 4
 NMOMENTS = 14
 MODEL_VHS, MODEL_VHS_NODAL, MODEL_VSS = 0, 1, 2
@@ -26,6 +26,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 FLAG_PROJECT = 1
 FLAG_UNFUSED = 2
 FLAG_TWO_LANE = 4
+FLAG_NO_PACK = 8
 
 _dp = C.POINTER(C.c_double)
 

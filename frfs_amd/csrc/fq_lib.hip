@@ -473,6 +473,90 @@ __device__ __forceinline__ void zstage(const cpx<T> *__restrict__ src, const cpx
     }
 }
 
+// Pruned z-DFT of the packed pair T_1 + i T_2 of one product (the
+// pair-packed k_prod1): both items' combinations as in zstage, with their
+// x,y phases applied before one shared S-point transform.  SH: both items
+// read the same source volume, whose 4S loads per pencil then serve both.
+template <int N, typename T, int THREADS, bool SH>
+__device__ __forceinline__ void zstage_pair(const cpx<T> *__restrict__ src1, const cpx<T> *__restrict__ src2,
+                                            const cpx<T> *__restrict__ zc1, const cpx<T> *__restrict__ zc2,
+                                            const cpx<T> *__restrict__ ph1, const cpx<T> *__restrict__ ph2,
+                                            cpx<T> *dst) {
+    using PG = ProdGeo<N>;
+    constexpr int S = PG::S;
+    static_assert((N * N) % THREADS == 0, "whole pencils per thread");
+    constexpr int ITER = N * N / THREADS;
+    cpx<T> d1[4], cc1[S], d2[4], cc2[S];
+#pragma unroll
+    for (int b = 1; b < 4; ++b) { d1[b] = zc1[b]; d2[b] = zc2[b]; }
+#pragma unroll
+    for (int a = 1; a < S; ++a) { cc1[a] = zc1[4 + a]; cc2[a] = zc2[4 + a]; }
+    constexpr uint32_t PB = uint32_t(N * N * sizeof(cpx<T>));
+    const BufView sv1(src1, uint32_t(N) * PB), sv2(src2, uint32_t(N) * PB);
+#pragma unroll 1
+    for (int it = 0; it < ITER; ++it) {
+        const uint32_t p = threadIdx.x + uint32_t(it * THREADS);
+        const uint32_t I = p / N, J = p % N;
+        const uint32_t vo = p * uint32_t(sizeof(cpx<T>));
+        cpx<T> t1[S], t2[S];
+        {
+            cpx<T> x[4][S];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+#pragma unroll
+                for (int a = 0; a < S; ++a) x[b][a] = sv1.ld<T>(vo, uint32_t(a + S * b) * PB);
+            }
+#pragma unroll
+            for (int a = 0; a < S; ++a) {
+                t1[a] = x[0][a];
+#pragma unroll
+                for (int b = 1; b < 4; ++b) fq::cmac(t1[a], x[b][a], d1[b]);
+            }
+            if constexpr (SH) {
+#pragma unroll
+                for (int a = 0; a < S; ++a) {
+                    t2[a] = x[0][a];
+#pragma unroll
+                    for (int b = 1; b < 4; ++b) fq::cmac(t2[a], x[b][a], d2[b]);
+                }
+            }
+        }
+        if constexpr (!SH) {
+            cpx<T> x[4][S];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+#pragma unroll
+                for (int a = 0; a < S; ++a) x[b][a] = sv2.ld<T>(vo, uint32_t(a + S * b) * PB);
+            }
+#pragma unroll
+            for (int a = 0; a < S; ++a) {
+                t2[a] = x[0][a];
+#pragma unroll
+                for (int b = 1; b < 4; ++b) fq::cmac(t2[a], x[b][a], d2[b]);
+            }
+        }
+#pragma unroll
+        for (int a = 1; a < S; ++a) {
+            t1[a] = fq::cmul(t1[a], cc1[a]);
+            t2[a] = fq::cmul(t2[a], cc2[a]);
+        }
+        // u = pxy1 t1 + i pxy2 t2, transformed once
+        const cpx<T> pxy1 = fq::cmul(ph1[I], ph1[N + J]);
+        const cpx<T> q2 = fq::cmul(ph2[I], ph2[N + J]);
+        const cpx<T> pxy2 = {-q2.y, q2.x};
+#pragma unroll
+        for (int a = 0; a < S; ++a) {
+            cpx<T> u = fq::cmul(t1[a], pxy1);
+            fq::cmac(u, t2[a], pxy2);
+            t1[a] = u;
+        }
+        fq::LFFT<S, 1>::run(t1);
+        cpx<T> *dd = dst + I * PG::ROW + J;
+#pragma unroll
+        for (int k = 0; k < S; ++k) dd[k * PG::PLANE] = t1[k];
+    }
+}
+
 // Synchronisation around the in-place transpose of one LDS plane.  When a
 // plane is exactly one wave (2N = 64 lanes) only that wave touches it: LDS
 // executes one wave's accesses in issue order, so a wave-level barrier that
@@ -612,16 +696,33 @@ __device__ __forceinline__ void plane_inv1(cpx<T> *Lp, cpx<T> *v, int c, bool ac
     fq::LFFT<N, 1>::run(v);
 }
 
-template <int N, typename T, bool MULTI>
+// Pair packing (PACK): every item volume T = phi(theta) . F_s is Hermitian,
+// T(-l) = conj T(l) (F_s is the spectrum of a real f), except on the Nyquist
+// set where a component of l is the unpaired index N/2.  So E = DFT+(T) =
+// R + i S with R real and S = -i DFT+(A) from the anti-Hermitian part
+// A(l) = T(l) (1 - e^{i N theta.nyq(l)}) / 2, nyq(l) the Nyquist components
+// of l, and one complex transform of T_1 + i T_2 carries both items of a
+// product: its real part is R_1 - S_2, its imaginary part S_1 + R_2.
+// k_nyq evaluates S through the three Nyquist planes (see there); with
+// X = DFT+ of the packed A over one plane (X = i a_1 - a_2, a_t real),
+//   U(vx,vy,vz) = (-1)^vz X_K(vx,vy) + (-1)^vy X_J(vx,vz) + (-1)^vx X_I(vy,vz)
+// gives S_1 = U.y and S_2 = -U.x, hence with R = E - U
+//   E_1 = (R.x, U.y),  E_2 = (R.y, -U.x).
+// This halves the z stage's transforms and the x,y transforms of the items.
+template <int N, typename T, bool MULTI, bool PACK, bool SH>
 __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
     const cpx<T> *__restrict__ FT, cpx<T> *__restrict__ P, const cpx<T> *__restrict__ ph,
     const cpx<T> *__restrict__ zc, const DevProd *__restrict__ prods, const int *__restrict__ g_p0,
-    const int *__restrict__ g_np, const int *__restrict__ item_src, int ngroups, int nsrc) {
+    const int *__restrict__ g_np, const int *__restrict__ item_src, int ngroups, int nsrc,
+    const cpx<T> *__restrict__ NY, int nprods) {
     using PG = ProdGeo<N>;
     using P1 = Prod1Geo<N>;
     using G3 = Geo<N>;
     constexpr int ZC = 4 + PG::S;
-    __shared__ cpx<T> lds[PG::S * PG::PLANE];
+    // LDS: the S planes, then (PACK) the product's Nyquist corrections:
+    // K piece [vy][vx], J piece [k][vx] and I piece [k][vy] for vz = G + 4k
+    constexpr int NYL = PACK ? N * N + 2 * PG::S * N : 0;
+    __shared__ cpx<T> lds[PG::S * PG::PLANE + NYL];
     const int64_t unit = blockIdx.x;
     const int G = int(unit % 4);
     const int g = int((unit / 4) % ngroups);
@@ -641,25 +742,76 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
         const cpx<T> *src2 = FT + (int64_t(slot) * nsrc + item_src[pr.i2]) * G3::VOL;
         const cpx<T> *zc1 = zc + (int64_t(pr.i1) * 4 + G) * ZC, *zc2 = zc + (int64_t(pr.i2) * 4 + G) * ZC;
         const cpx<T> *ph1 = ph + int64_t(pr.i1) * 3 * N, *ph2 = ph + int64_t(pr.i2) * 3 * N;
-        cpx<T> e1[N], e2[N];
-        __syncthreads();                                   // LDS free
-        zstage<N, T, P1::THREADS>(src1, zc1, ph1, lds);
-        __syncthreads();
-        plane_inv1<N, T>(L1, e1, c, act);
-        __syncthreads();
-        zstage<N, T, P1::THREADS>(src2, zc2, ph2, lds);
-        __syncthreads();
-        plane_inv1<N, T>(L1, e2, c, act);
         const T cf = T(pr.coef);
-        if constexpr (MULTI) {
+        if constexpr (PACK) {
+            cpx<T> e[N];
+            __syncthreads();                               // LDS free
+            zstage_pair<N, T, P1::THREADS, SH>(src1, src2, zc1, zc2, ph1, ph2, lds);
+            __syncthreads();
+            // the product's corrections: global loads in flight across the
+            // x,y transforms, then staged in LDS for the whole workgroup
+            constexpr int KPT = (N * N + P1::THREADS - 1) / P1::THREADS;
+            constexpr int SN = PG::S * N;
+            static_assert(SN <= P1::THREADS, "one J and one I element per thread");
+            const cpx<T> *ny = NY + (int64_t(slot) * nprods + pi) * 3 * G3::N2;
+            const int tid = int(threadIdx.x);
+            cpx<T> rk[KPT], rj = {0, 0}, ri = {0, 0};
 #pragma unroll
-            for (int e = 0; e < N; ++e) {
-                const cpx<T> pe = fq::cmul(e1[e], e2[e]);
-                acc[e] = {fq::fmadd(cf, pe.x, acc[e].x), fq::fmadd(cf, pe.y, acc[e].y)};
+            for (int r = 0; r < KPT; ++r) {
+                const int idx = tid + r * P1::THREADS;
+                rk[r] = (N * N % P1::THREADS == 0 || idx < N * N) ? ny[idx] : cpx<T>{0, 0};
+            }
+            if (tid < SN) {
+                const int kk = tid / N, w = tid % N, z = G + 4 * kk;
+                rj = ny[G3::N2 + z * N + w];                       // (vx = w, vz = z)
+                ri = ny[2 * G3::N2 + w * N + z];                   // (vy = w, vz = z)
+            }
+            plane_inv1<N, T>(L1, e, c, act);               // lane c: row vx = c over vy
+            cpx<T> *nyk = lds + PG::S * PG::PLANE, *nyj = nyk + N * N, *nyi = nyj + SN;
+#pragma unroll
+            for (int r = 0; r < KPT; ++r) {
+                const int idx = tid + r * P1::THREADS;
+                if (N * N % P1::THREADS == 0 || idx < N * N) nyk[idx] = rk[r];
+            }
+            if (tid < SN) { nyj[tid] = rj; nyi[tid] = ri; }
+            __syncthreads();
+            const T sz = (vz & 1) ? T(-1) : T(1);
+            const T sc = (c & 1) ? T(-1) : T(1);
+            const cpx<T> xj = act ? nyj[k * N + c] : cpx<T>{0, 0};
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const cpx<T> xk = act ? nyk[j * N + c] : cpx<T>{0, 0};
+                const cpx<T> xi = nyi[k * N + j];
+                const T jx = (j & 1) ? -xj.x : xj.x, jy = (j & 1) ? -xj.y : xj.y;
+                const T ux = fq::fmadd(sc, xi.x, fq::fmadd(sz, xk.x, jx));
+                const T uy = fq::fmadd(sc, xi.y, fq::fmadd(sz, xk.y, jy));
+                const T rx = e[j].x - ux, ry = e[j].y - uy;
+                // (rx + i uy)(ry - i ux)
+                const T px = fq::fmadd(rx, ry, uy * ux);
+                const T py = fq::fmadd(uy, ry, -(rx * ux));
+                if constexpr (MULTI) acc[j] = {fq::fmadd(cf, px, acc[j].x), fq::fmadd(cf, py, acc[j].y)};
+                else acc[j] = {cf * px, cf * py};
             }
         } else {
+            cpx<T> e1[N], e2[N];
+            __syncthreads();                               // LDS free
+            zstage<N, T, P1::THREADS>(src1, zc1, ph1, lds);
+            __syncthreads();
+            plane_inv1<N, T>(L1, e1, c, act);
+            __syncthreads();
+            zstage<N, T, P1::THREADS>(src2, zc2, ph2, lds);
+            __syncthreads();
+            plane_inv1<N, T>(L1, e2, c, act);
+            if constexpr (MULTI) {
 #pragma unroll
-            for (int e = 0; e < N; ++e) acc[e] = fq::cscale(fq::cmul(e1[e], e2[e]), cf);
+                for (int e = 0; e < N; ++e) {
+                    const cpx<T> pe = fq::cmul(e1[e], e2[e]);
+                    acc[e] = {fq::fmadd(cf, pe.x, acc[e].x), fq::fmadd(cf, pe.y, acc[e].y)};
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < N; ++e) acc[e] = fq::cscale(fq::cmul(e1[e], e2[e]), cf);
+            }
         }
     }
     // forward transforms of the group sum: lane c holds row vx = c over vy
@@ -677,6 +829,67 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
     if (act) {
 #pragma unroll
         for (int i = 0; i < N; ++i) dst[i * N + c] = acc[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_nyq: the Nyquist-plane corrections of the pair-packed k_prod1 (FP32).
+// One workgroup per (slot, product), one 32-lane group per plane piece:
+//   piece 0 (K = N/2):                 rows a = I, lanes = J   -> NY0[vy][vx]
+//   piece 1 (J = N/2, K != N/2):       rows a = I, lanes = K   -> NY1[vz][vx]
+//   piece 2 (I = N/2, J, K != N/2):    rows a = K, lanes = J   -> NY2[vy][vz]
+// Each lane gathers its column of A_1 + i A_2 (A_t = T_t (1 - e^{i N
+// theta_t.nyq}) / 2 on the piece) as F_s1 V_1 + F_s2 V_2 with the per-product
+// weights V_1 = phi_1 (1 - e^{..}) / 2 and V_2 = i phi_2 (1 - e^{..}) / 2
+// (zero off the piece), and the piece's 2D DFT+ runs as a column transform,
+// an LDS transpose and a row transform.  SH: both items read one source and
+// V_1 holds V_1 + V_2.
+// ---------------------------------------------------------------------------
+template <int N, bool SH>
+__global__ __launch_bounds__(96, 4) void k_nyq(const cpx<float> *__restrict__ FT, cpx<float> *__restrict__ NY,
+                                               const cpx<float> *__restrict__ V, const DevProd *__restrict__ prods,
+                                               const int *__restrict__ item_src, int nprods, int nsrc) {
+    using G3 = Geo<N>;
+    constexpr int H = N / 2, ROW = N + 1;
+    __shared__ cpx<float> lds[3 * N * ROW];
+    const int pi = int(blockIdx.x % unsigned(nprods));
+    const int slot = int(blockIdx.x / unsigned(nprods));
+    const int piece = int(threadIdx.x) / 32, c = int(threadIdx.x) % 32;
+    const bool act = c < N;
+    const DevProd pr = prods[pi];
+    const cpx<float> *F1 = FT + (int64_t(slot) * nsrc + item_src[pr.i1]) * G3::VOL;
+    const cpx<float> *F2 = FT + (int64_t(slot) * nsrc + item_src[pr.i2]) * G3::VOL;
+    const cpx<float> *v1 = V + (int64_t(pi) * 6 + piece) * G3::N2 + c;     // [product][item][piece][a][lane]
+    const cpx<float> *v2 = v1 + 3 * G3::N2;
+    // element a of this lane's column: flat index base + a * stride of the volume
+    const int base = piece == 0 ? H * G3::N2 + c : (piece == 1 ? c * G3::N2 + H : H * N + c);
+    const int stride = piece == 2 ? G3::N2 : N;
+    cpx<float> v[N];
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+        if (!act) { v[a] = {0, 0}; continue; }
+        const int off = base + a * stride;
+        if constexpr (SH) {
+            v[a] = fq::cmul(F1[off], v1[a * N]);             // V_1 holds V_1 + V_2
+        } else {
+            v[a] = fq::cmul(F1[off], v1[a * N]);
+            fq::cmac(v[a], F2[off], v2[a * N]);
+        }
+    }
+    fq::LFFT<N, 1>::run(v);                                // over a, lane c
+    cpx<float> *L = lds + piece * N * ROW;
+    if (act) {
+#pragma unroll
+        for (int a = 0; a < N; ++a) L[a * ROW + c] = v[a];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < N; ++b) v[b] = act ? L[c * ROW + b] : cpx<float>{0, 0};
+    fq::LFFT<N, 1>::run(v);                                // over the lane coordinate
+    cpx<float> *out = NY + ((int64_t(slot) * nprods + pi) * 3 + piece) * G3::N2;
+    if (act) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) out[j * N + c] = v[j];
     }
 }
 
@@ -1121,6 +1334,10 @@ struct fq_handle {
     void *zc = nullptr;          // [items][4][4 + N/4] k_prod z-stage coefficients in T
     bool fused = true;           // k_prod path (else k_inv_z + k_plane)
     bool two_lane = false;       // FP32 24^3/32^3: two-lane k_prod instead of one-lane k_prod1
+    bool pack = false;           // k_prod1 with pair packing (k_nyq corrections in NY)
+    void *NY = nullptr;          // [chunk][products][3][N^2] cpx<float> Nyquist-plane corrections
+    cpx<float> *nyv = nullptr;   // [products][2][3][N][N] k_nyq weights
+    bool pack_shared = false;    // every product's two items read one source
     cpx<float> *F32 = nullptr;   // FP32 copy of F (mixed, fused)
     // scratch
     cpx<double> *F = nullptr;
@@ -1133,8 +1350,8 @@ struct fq_handle {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_rec;
 };
 
-static const char *kKernelNames[FQ_NKERNELS] = {"k_fwd_z", "k_fwd_xy", "k_inv_z", "k_plane",
-                                                "k_accum", "k_out", "k_project", "k_prod"};
+static const char *kKernelNames[FQ_NKERNELS] = {"k_fwd_z", "k_fwd_xy", "k_inv_z", "k_plane", "k_accum",
+                                                "k_out",   "k_project", "k_prod",  "k_nyq"};
 
 namespace {
 
@@ -1176,6 +1393,54 @@ template <typename T> int upload_phases(fq_handle *h) {
     if (rc == FQ_SUCCESS) rc = upload(&dz, zc);
     h->zc = dz;
     return rc;
+}
+
+// Weights of k_nyq for the pair-packed product kernel: for product p and
+// item t (t = 0: i1, t = 1: i2) on plane piece q at (row a, lane c),
+//   V = i^t e^{i theta.l0(I,J,K)} (1 - e^{i N theta.nyq(I,J,K)}) / 2,
+// with (I,J,K) the piece's coordinates (k_nyq) and zero where the piece
+// excludes the point; nyq = the components at the unpaired index N/2.
+int upload_nyq(fq_handle *h) {
+    const fq::Plan &P = h->P;
+    const int N = P.N, H = N / 2;
+    const size_t N2 = size_t(N) * N;
+    std::vector<cpx<float>> v(P.prods.size() * 6 * N2);
+    bool shared = true;
+    auto l0 = [&](int k) { return double(k < H ? k : k - N); };
+    for (size_t p = 0; p < P.prods.size(); ++p) {
+        const int items[2] = {P.prods[p].i1, P.prods[p].i2};
+        shared = shared && P.items[items[0]].src == P.items[items[1]].src;
+        for (int t = 0; t < 2; ++t) {
+            const double *th = P.items[items[t]].th;
+            for (int q = 0; q < 3; ++q)
+                for (int a = 0; a < N; ++a)
+                    for (int c = 0; c < N; ++c) {
+                        const int I = q == 2 ? H : a, J = q == 1 ? H : c, K = q == 0 ? H : (q == 1 ? c : a);
+                        const bool keep = q == 0 || (q == 1 ? c != H : (a != H && c != H));
+                        double re = 0, im = 0;
+                        if (keep) {
+                            const double ang = th[0] * l0(I) + th[1] * l0(J) + th[2] * l0(K);
+                            const double nq = N * ((I == H ? th[0] : 0.) + (J == H ? th[1] : 0.) + (K == H ? th[2] : 0.));
+                            // e^{i ang} (1 - e^{i nq}) / 2
+                            const double hr = 0.5 * (1 - std::cos(nq)), hi = -0.5 * std::sin(nq);
+                            re = std::cos(ang) * hr - std::sin(ang) * hi;
+                            im = std::cos(ang) * hi + std::sin(ang) * hr;
+                            if (t == 1) { const double r = re; re = -im; im = r; }
+                        }
+                        v[((p * 2 + t) * 3 + q) * N2 + size_t(a) * N + c] = {float(re), float(im)};
+                    }
+        }
+    }
+    // one source for both items: k_nyq reads the sum V_1 + V_2 once
+    if (shared)
+        for (size_t p = 0; p < P.prods.size(); ++p)
+            for (size_t e = 0; e < 3 * N2; ++e) {
+                cpx<float> &a = v[p * 6 * N2 + e];
+                const cpx<float> b = v[(p * 6 + 3) * N2 + e];
+                a = {a.x + b.x, a.y + b.y};
+            }
+    h->pack_shared = shared;
+    return upload(&h->nyv, v);
 }
 
 int build_tables(fq_handle *h) {
@@ -1324,35 +1589,45 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     if (h->fused) {
         const cpx<T> *FT = reinterpret_cast<const cpx<T> *>(want32 ? static_cast<void *>(h->F32)
                                                                    : static_cast<void *>(h->F));
-        pre();
         const cpx<T> *zc = reinterpret_cast<const cpx<T> *>(h->zc);
-        const dim3 grid(unsigned(int64_t(D) * ngroups * 4)), blk(ProdGeo<N>::THREADS);
-        bool one_lane = false;
-        if constexpr ((N == 24 || N == 32) && sizeof(T) == 4) one_lane = !h->two_lane;
-        if (one_lane) {
-            bool multi1 = false;
-            for (int g = 0; g < ngroups; ++g) multi1 |= P.group_np[g] > 1;
-            const dim3 blk1(Prod1Geo<N>::THREADS);
-            if (multi1)
-                    hipLaunchKernelGGL((k_prod1<N, T, true>), grid, blk1, 0, st, FT, Pg, ph, zc, h->prods,
-                                   h->g_p0, h->g_np, h->item_src, ngroups, nsrc);
-            else
-                hipLaunchKernelGGL((k_prod1<N, T, false>), grid, blk1, 0, st, FT, Pg, ph, zc, h->prods,
-                                   h->g_p0, h->g_np, h->item_src, ngroups, nsrc);
-            post(7);
-        } else {
-        // the accumulating variant also for single-product groups in FP64:
-        // with its accumulator live across the item stages the compiler
-        // keeps the FP64 kernel within 256 VGPRs without spills
-        bool multi = sizeof(T) == 8;
+        const dim3 grid(unsigned(int64_t(D) * ngroups * 4));
+        bool multi = false;
         for (int g = 0; g < ngroups; ++g) multi |= P.group_np[g] > 1;
-        if (multi)
-            hipLaunchKernelGGL((k_prod<N, T, true>), grid, blk, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0,
+        bool one_lane = false;
+        if constexpr ((N == 24 || N == 32) && sizeof(T) == 4) {
+            // one lane per pencil, with pair packing unless switched off
+            one_lane = !h->two_lane;
+            if (one_lane) {
+                const dim3 blk1(Prod1Geo<N>::THREADS);
+                const int nprods = int(P.prods.size());
+                cpx<T> *NY = reinterpret_cast<cpx<T> *>(h->NY);
+                const bool sh = h->pack_shared;
+                if (h->pack) {
+                    auto knyq = sh ? k_nyq<N, true> : k_nyq<N, false>;
+                    pre();
+                    hipLaunchKernelGGL(knyq, dim3(unsigned(int64_t(D) * nprods)), dim3(96), 0, st, FT, NY, h->nyv,
+                                       h->prods, h->item_src, nprods, nsrc);
+                    post(8);
+                }
+                auto kern = !h->pack ? (multi ? k_prod1<N, T, true, false, false> : k_prod1<N, T, false, false, false>)
+                            : sh     ? (multi ? k_prod1<N, T, true, true, true> : k_prod1<N, T, false, true, true>)
+                                     : (multi ? k_prod1<N, T, true, true, false> : k_prod1<N, T, false, true, false>);
+                pre();
+                hipLaunchKernelGGL(kern, grid, blk1, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0, h->g_np,
+                                   h->item_src, ngroups, nsrc, NY, nprods);
+                post(7);
+            }
+        }
+        if (!one_lane) {
+            // the accumulating variant also for single-product groups in FP64:
+            // with its accumulator live across the item stages the compiler
+            // keeps the FP64 kernel within 256 VGPRs without spills
+            const bool m2 = multi || sizeof(T) == 8;
+            auto kern = m2 ? k_prod<N, T, true> : k_prod<N, T, false>;
+            pre();
+            hipLaunchKernelGGL(kern, grid, dim3(ProdGeo<N>::THREADS), 0, st, FT, Pg, ph, zc, h->prods, h->g_p0,
                                h->g_np, h->item_src, ngroups, nsrc);
-        else
-            hipLaunchKernelGGL((k_prod<N, T, false>), grid, blk, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0,
-                               h->g_np, h->item_src, ngroups, nsrc);
-        post(7);
+            post(7);
         }
     } else {
     u = int64_t(D) * nitems * N;
@@ -1509,6 +1784,8 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     h->flags = prm->flags;
     h->fused = !(prm->flags & FQ_FLAG_UNFUSED);
     h->two_lane = (prm->flags & FQ_FLAG_TWO_LANE) != 0;
+    h->pack = h->fused && !h->two_lane && h->precision == FQ_PREC_MIXED && (prm->Nv == 24 || prm->Nv == 32) &&
+              !(prm->flags & FQ_FLAG_NO_PACK);
     if (hipGetDevice(&h->device) != hipSuccess) {
         delete h;
         return fail(FQ_ERR_HIP, "no HIP device");
@@ -1539,6 +1816,7 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     chk(upload(&h->t_ln, tln));
     if (rc == FQ_SUCCESS) rc = build_tables(h);
     if (rc == FQ_SUCCESS) rc = build_projection(h);
+    if (rc == FQ_SUCCESS && h->pack) rc = upload_nyq(h);
     // scratch: F (FP64, nsrc) + max(Z (items, T), T0 (nsrc, FP64), Y (2 ntgt, FP64)) + P (groups, T)
     const size_t tsz = h->precision == FQ_PREC_MIXED ? sizeof(cpx<float>) : sizeof(cpx<double>);
     const size_t per_dof_F = size_t(P.nsrc) * n * sizeof(cpx<double>);
@@ -1548,7 +1826,9 @@ int fq_create(const fq_params *prm, fq_handle **out) {
                                        size_t(2 * P.ntgt) * n * sizeof(cpx<double>)});
     const size_t per_dof_P = size_t(P.ngroups) * n * tsz;
     const size_t per_dof_F32 = h->precision == FQ_PREC_MIXED ? size_t(P.nsrc) * n * sizeof(cpx<float>) : 0;
-    const size_t per_dof = per_dof_F + per_dof_Z + per_dof_P + per_dof_F32;
+    // Nyquist-plane corrections of the pair-packed product kernel
+    const size_t per_dof_NY = h->pack ? P.prods.size() * 3 * size_t(N) * N * sizeof(cpx<float>) : 0;
+    const size_t per_dof = per_dof_F + per_dof_Z + per_dof_P + per_dof_F32 + per_dof_NY;
     int64_t chunk = prm->max_chunk;
     if (chunk <= 0) {
         const size_t budget = size_t(32) << 30;   // 32 GiB default scratch budget (of 288 GB HBM)
@@ -1563,7 +1843,9 @@ int fq_create(const fq_params *prm, fq_handle **out) {
         hipError_t e4 = hipSuccess;
         if (h->precision == FQ_PREC_MIXED)
             e4 = hipMalloc(&h->F32, size_t(P.nsrc) * n * sizeof(cpx<float>) * chunk);
-        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess)
+        hipError_t e5 = hipSuccess;
+        if (h->pack) e5 = hipMalloc(&h->NY, per_dof_NY * chunk);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess || e5 != hipSuccess)
             rc = fail(FQ_ERR_ALLOC, "hipMalloc scratch failed (reduce max_chunk)");
         h->scratch = per_dof * chunk;
     }
@@ -1621,7 +1903,7 @@ int fq_destroy(fq_handle *h) {
     for (void *p : {h->ph, (void *)h->item_src, (void *)h->prods, (void *)h->g_p0, (void *)h->g_np,
                     (void *)h->c_group, (void *)h->t_c0, (void *)h->t_cn, (void *)h->l_src, (void *)h->t_l0,
                     (void *)h->t_ln, (void *)h->W, (void *)h->ginv, (void *)h->pmass, (void *)h->F, h->Z,
-                    h->Pg, h->zc, (void *)h->F32})
+                    h->Pg, h->zc, (void *)h->F32, h->NY, (void *)h->nyv})
         if (p) (void)hipFree(p);
     (void)hipThreadExchangeStreamCaptureMode(&mode);
     delete h;

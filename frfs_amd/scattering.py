@@ -17,7 +17,9 @@ always FP64): ``[scattering-model] precision = double | mixed`` and
 internal batch; ``fused = 0`` selects the three-kernel path that stages the
 inverse transforms through HBM (for comparison); ``pencil-lanes = 2`` selects
 the two-lanes-per-pencil FP32 product kernel on the 24^3 and 32^3 grids
-(default 1: one lane per pencil); ``autotune = 1`` makes the
+(default 1: one lane per pencil); ``pair-pack = 0`` makes the one-lane kernel
+transform the two items of every product separately instead of as one packed
+complex volume (default 1); ``autotune = 1`` makes the
 first ``fs_batch`` call time the candidate schedules on its own inputs and
 keep the fastest (``frfs_amd.autotune``; ``autotune()`` does it on demand).
 """
@@ -59,6 +61,8 @@ def _flags(cfg):
         fl |= _lib.FLAG_UNFUSED
     if cfg.getint('scattering-model', 'pencil-lanes', 1) == 2:
         fl |= _lib.FLAG_TWO_LANE
+    if not cfg.getint('scattering-model', 'pair-pack', 1):
+        fl |= _lib.FLAG_NO_PACK
     return fl
 
 

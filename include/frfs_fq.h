@@ -67,9 +67,13 @@ enum fq_flags {
                                  and the total (mass-weighted) momentum and energy */
     FQ_FLAG_UNFUSED = 2,      /* stage every inverse transform through HBM
                                  (three-kernel path; for comparison) */
-    FQ_FLAG_TWO_LANE = 4      /* FP32 on 24^3/32^3 grids: product kernel with two
+    FQ_FLAG_TWO_LANE = 4,     /* FP32 on 24^3/32^3 grids: product kernel with two
                                  lanes per pencil instead of one (a schedule
                                  candidate of the autotuner) */
+    FQ_FLAG_NO_PACK = 8       /* FP32 one-lane product kernel: transform the two
+                                 items of a product separately instead of as one
+                                 packed complex volume with Nyquist-plane
+                                 corrections (for comparison) */
 };
 
 /*
@@ -162,7 +166,7 @@ int fq_eval_bilinear(fq_handle *h, const double *f, const double *g, double *q,
 
 /* ---- instrumentation --------------------------------------------------- */
 
-#define FQ_NKERNELS 8
+#define FQ_NKERNELS 9
 /* Bracket every kernel launch with HIP events on the launch stream (on = 1)
  * or stop (on = 0); either call discards earlier records.  No reference
  * counterpart: the reference times only its precompute with cuda.Event

@@ -8,7 +8,7 @@ ARGON = dict(T0=273.0, H0=1e-3, rho0=0.00011140244750324529, molarMass0=39.96e-3
 
 
 def vhs_cfg(Nv=16, Nrho=8, M=6, dev=5.0, omega=0.5, precision='double', project=0,
-            model='vhs-gll', max_chunk=0, fused=1):
+            model='vhs-gll', max_chunk=0, fused=1, pack=1):
     return Inifile.from_dict({
         'non-dim': ARGON,
         'constants': {'Nv': Nv, 'NvBatchSize': Nv ** 3, 'Nrho': Nrho},
@@ -16,16 +16,16 @@ def vhs_cfg(Nv=16, Nrho=8, M=6, dev=5.0, omega=0.5, precision='double', project=
         'spherical-design-rule': {'ssrule': 'womersley', 'M': M},
         'scattering-model': {'type': model, 'omega': omega, 'dRef': 4.17e-10, 'Tref': 273,
                              'precision': precision, 'conservation-projection': project,
-                             'max-chunk': max_chunk, 'fused': fused},
+                             'max-chunk': max_chunk, 'fused': fused, 'pair-pack': pack},
     })
 
 
 def vss_cfg(Nv=16, Nrho=8, M=12, dev=5.0, masses=(39.96e-3,), alpha=None, omega=None,
-            dref=None, Mpre=32, precision='double', project=0, max_chunk=0, fused=1, lanes=1):
+            dref=None, Mpre=32, precision='double', project=0, max_chunk=0, fused=1, lanes=1, pack=1):
     ns = len(masses)
     sm = {'type': 'vss-gll', 'ssrulepre': 'womersley', 'Mpre': Mpre,
           'precision': precision, 'conservation-projection': project, 'max-chunk': max_chunk,
-          'fused': fused, 'pencil-lanes': lanes}
+          'fused': fused, 'pencil-lanes': lanes, 'pair-pack': pack}
     for p in range(ns):
         for q in range(ns):
             pq = f'{p}{q}'

@@ -80,18 +80,52 @@ def test_plan_counts():
     assert i['n_sources'] == 2 and i['n_fft_ref'] == 4 * 580 and i['n_fft'] == 1158
 
 
-def _emulate(plan, F, weight_of_contrib, loss_tables, f_src, N):
-    """NumPy execution of an exported plan (complex128)."""
+def _packed_pair(T1, T2, th1, th2, N):
+    """Both item transforms of a product the way the pair-packed product
+    kernel forms them: one DFT+ of T1 + i T2, and the Nyquist-plane
+    corrections of k_nyq (three 2D DFT+ of the packed anti-Hermitian part,
+    on the planes K = N/2; J = N/2, K != N/2; I = N/2, J, K != N/2)."""
+    h = N // 2
+    dft2 = lambda x: np.fft.ifft2(x) * x.size
+    E = np.fft.ifftn(T1 + 1j * T2) * T1.size
+    A = 0
+    for t, (T, th) in enumerate(((T1, th1), (T2, th2))):
+        w = np.ones((N, N, N), complex)
+        w[h, :, :] *= np.exp(1j * N * th[0])
+        w[:, h, :] *= np.exp(1j * N * th[1])
+        w[:, :, h] *= np.exp(1j * N * th[2])
+        A = A + (1j ** t) * T * (1 - w) / 2
+    xk = dft2(A[:, :, h])                                      # (vx, vy)
+    aj = A[:, h, :].copy(); aj[:, h] = 0
+    xj = dft2(aj)                                              # (vx, vz)
+    ai = A[h, :, :].copy(); ai[h, :] = 0; ai[:, h] = 0
+    xi = dft2(ai)                                              # (vy, vz)
+    sg = (-1.0) ** np.arange(N)
+    U = (sg[None, None, :] * xk[:, :, None] + sg[None, :, None] * xj[:, None, :]
+         + sg[:, None, None] * xi[None, :, :])
+    R = E - U
+    return R.real + 1j * U.imag, R.imag - 1j * U.real
+
+
+def _emulate(plan, F, weight_of_contrib, loss_tables, f_src, N, pack=False):
+    """NumPy execution of an exported plan (complex128); with pack=True the
+    item transforms of every product come from _packed_pair."""
     n = N ** 3
     l0 = np.concatenate((np.arange(0, N // 2), np.arange(-N // 2, 0))).astype(float)
-    E = []
+    T, E = [], []
     for s, th in zip(plan['item_src'], plan['item_theta']):
         ph = np.exp(1j * (th[0] * l0[:, None, None] + th[1] * l0[None, :, None] + th[2] * l0[None, None, :]))
-        E.append(np.fft.ifftn(ph * F[s].reshape(N, N, N)) * n)
+        T.append(ph * F[s].reshape(N, N, N))
+        E.append(np.fft.ifftn(T[-1]) * n)
     ngroups = plan['prod_group'].max() + 1
     G = [np.zeros((N, N, N), complex) for _ in range(ngroups)]
     for i1, i2, cf, g in zip(plan['prod_i1'], plan['prod_i2'], plan['prod_coef'], plan['prod_group']):
-        G[g] += cf * E[i1] * E[i2]
+        if pack:
+            e1, e2 = _packed_pair(T[i1], T[i2], plan['item_theta'][i1], plan['item_theta'][i2], N)
+            assert np.allclose(e1, E[i1], rtol=0, atol=1e-12) and np.allclose(e2, E[i2], rtol=0, atol=1e-12)
+            G[g] += cf * e1 * e2
+        else:
+            G[g] += cf * E[i1] * E[i2]
     Gh = [np.fft.fftn(g).ravel() for g in G]
     ntgt = plan['contrib_target'].max() + 1
     H = [np.zeros(n, complex) for _ in range(ntgt)]
@@ -108,7 +142,8 @@ def _emulate(plan, F, weight_of_contrib, loss_tables, f_src, N):
     return out
 
 
-def test_plan_execution_matches_oracle_vhs():
+@pytest.mark.parametrize('pack', [False, True])
+def test_plan_execution_matches_oracle_vhs(pack):
     cfg = vhs_cfg(8, 4, 6, 4.0)
     p, _ = _vhs_params(cfg)
     plan = _lib.Plan(p).export()
@@ -117,12 +152,13 @@ def test_plan_execution_matches_oracle_vhs():
     f = fsp.two_maxwellian_batch(vm['cv'], 1, seed=8)[0]
     F = [np.fft.fftn(f.reshape(N, N, N)).ravel() / n]
     w = lambda c: prm['prefac'] * (2 * vm['qw'][c] * vm['sw'] / n) * tab['bb1'][c]
-    q = _emulate(plan, F, w, [prm['prefac'] * tab['bb2']], [f], N)[0]
+    q = _emulate(plan, F, w, [prm['prefac'] * tab['bb2']], [f], N, pack)[0]
     assert rel_l2(q, fsp.vhs_fs(f, vm, tab, prm['prefac'])) < 1e-13
 
 
+@pytest.mark.parametrize('pack', [False, True])
 @pytest.mark.parametrize('ns', [1, 2])
-def test_plan_execution_matches_oracle_vss(ns):
+def test_plan_execution_matches_oracle_vss(ns, pack):
     m0 = ARGON['molarMass0']
     cfg = vss_cfg(8, 3, 12, 4.0, masses=(m0, 2 * m0)[:ns], Mpre=32)
     p, _ = _vss_params(cfg, ns)
@@ -154,7 +190,7 @@ def test_plan_execution_matches_oracle_vss(ns):
                     pq = f'{pp}{qq}'
                     W[cidx] = W.get(cidx, 0) + prm['prefac'][pq] * (vm['qw'][x] * vm['sw'] / n) * tab['b'][pq][x, y]
     Lt = [prm['prefac'][f'{t}{s}'] * tab['c'][f'{t}{s}'] for s, t in zip(plan['loss_src'], plan['loss_target'])]
-    q = _emulate(plan, F, lambda c: W[c], Lt, fs, N)
+    q = _emulate(plan, F, lambda c: W[c], Lt, fs, N, pack)
     ref = fsp.vss_fs(fs, vm, tab, prm)
     for t in range(ns):
         assert rel_l2(q[t], ref[t]) < 1e-13
