@@ -474,10 +474,9 @@ __device__ __forceinline__ void zstage(const cpx<T> *__restrict__ src, const cpx
 }
 
 // Pruned z-DFT of the packed pair T_1 + i T_2 of one product (the
-// pair-packed k_prod1): both items' combinations as in zstage, with their
-// x,y phases applied before one shared S-point transform.  SH: both items
-// read the same source volume, whose 4S loads per pencil then serve both.
-template <int N, typename T, int THREADS, bool SH>
+// pair-packed k_prod1, general pairs): both items' combinations as in
+// zstage, with their x,y phases applied before one shared S-point transform.
+template <int N, typename T, int THREADS>
 __device__ __forceinline__ void zstage_pair(const cpx<T> *__restrict__ src1, const cpx<T> *__restrict__ src2,
                                             const cpx<T> *__restrict__ zc1, const cpx<T> *__restrict__ zc2,
                                             const cpx<T> *__restrict__ ph1, const cpx<T> *__restrict__ ph2,
@@ -512,16 +511,8 @@ __device__ __forceinline__ void zstage_pair(const cpx<T> *__restrict__ src1, con
 #pragma unroll
                 for (int b = 1; b < 4; ++b) fq::cmac(t1[a], x[b][a], d1[b]);
             }
-            if constexpr (SH) {
-#pragma unroll
-                for (int a = 0; a < S; ++a) {
-                    t2[a] = x[0][a];
-#pragma unroll
-                    for (int b = 1; b < 4; ++b) fq::cmac(t2[a], x[b][a], d2[b]);
-                }
-            }
         }
-        if constexpr (!SH) {
+        {
             cpx<T> x[4][S];
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
@@ -554,6 +545,68 @@ __device__ __forceinline__ void zstage_pair(const cpx<T> *__restrict__ src1, con
         cpx<T> *dd = dst + I * PG::ROW + J;
 #pragma unroll
         for (int k = 0; k < S; ++k) dd[k * PG::PLANE] = t1[k];
+    }
+}
+
+// Pruned z-DFT of a conjugate pair: both items read one source and
+// theta_2 = -theta_1 (every VHS product, and the same-species VSS pairs).
+// With y_b = w_4^{bG} x_b, beta_b = p_b + i q_b, P = sum_b p_b y_b and
+// Q = sum_b q_b y_b the two combinations are P + iQ and P - iQ, and with
+// gamma = alpha_a pxy (item 1's phases)
+//   gamma (P + iQ) + i conj(gamma) (P - iQ)
+//       = (1 + i) [ Re((1 - i) gamma) P + Re((1 + i) gamma) Q ],
+// so one pencil costs two real-coefficient sums and no per-item complex
+// products.  zq[(prod*4 + G)*(3 + 3S) + .] holds {(p_b, q_b) for b = 1..3
+// with the sign of w_4^{bG} folded in}, (1 - i) alpha_a, (1 + i) alpha_a and
+// (1 + i) w_N^{aG}.  ODD (G odd): w_4^{bG} = +-i for odd b, applied as a
+// rotation of x_b.
+template <int N, typename T, int THREADS, bool ODD>
+__device__ __forceinline__ void zstage_conj(const cpx<T> *__restrict__ src, const cpx<T> *__restrict__ zq,
+                                            const cpx<T> *__restrict__ ph1, cpx<T> *dst) {
+    using PG = ProdGeo<N>;
+    constexpr int S = PG::S;
+    static_assert((N * N) % THREADS == 0, "whole pencils per thread");
+    constexpr int ITER = N * N / THREADS;
+    cpx<T> pq[4], am[S], ap[S], cw[S];
+#pragma unroll
+    for (int b = 1; b < 4; ++b) pq[b] = zq[b - 1];
+#pragma unroll
+    for (int a = 0; a < S; ++a) { am[a] = zq[3 + a]; ap[a] = zq[3 + S + a]; cw[a] = zq[3 + 2 * S + a]; }
+    constexpr uint32_t PB = uint32_t(N * N * sizeof(cpx<T>));
+    const BufView sv(src, uint32_t(N) * PB);
+#pragma unroll 1
+    for (int it = 0; it < ITER; ++it) {
+        const uint32_t p = threadIdx.x + uint32_t(it * THREADS);
+        const uint32_t I = p / N, J = p % N;
+        const uint32_t vo = p * uint32_t(sizeof(cpx<T>));
+        cpx<T> x[4][S];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+#pragma unroll
+            for (int a = 0; a < S; ++a) x[b][a] = sv.ld<T>(vo, uint32_t(a + S * b) * PB);
+        }
+        const cpx<T> pxy = fq::cmul(ph1[I], ph1[N + J]);
+        cpx<T> u[S];
+#pragma unroll
+        for (int a = 0; a < S; ++a) {
+            cpx<T> P = x[0][a], Q;
+#pragma unroll
+            for (int b = 1; b < 4; ++b) {
+                cpx<T> y = x[b][a];
+                if (ODD && (b & 1)) y = {-y.y, y.x};
+                P = {fq::fmadd(pq[b].x, y.x, P.x), fq::fmadd(pq[b].x, y.y, P.y)};
+                if (b == 1) Q = {pq[b].y * y.x, pq[b].y * y.y};
+                else Q = {fq::fmadd(pq[b].y, y.x, Q.x), fq::fmadd(pq[b].y, y.y, Q.y)};
+            }
+            const T A = fq::fmadd(am[a].x, pxy.x, -(am[a].y * pxy.y));
+            const T B = fq::fmadd(ap[a].x, pxy.x, -(ap[a].y * pxy.y));
+            const cpx<T> z = {fq::fmadd(A, P.x, B * Q.x), fq::fmadd(A, P.y, B * Q.y)};
+            u[a] = fq::cmul(z, cw[a]);
+        }
+        fq::LFFT<S, 1>::run(u);
+        cpx<T> *dd = dst + I * PG::ROW + J;
+#pragma unroll
+        for (int k = 0; k < S; ++k) dd[k * PG::PLANE] = u[k];
     }
 }
 
@@ -709,12 +762,12 @@ __device__ __forceinline__ void plane_inv1(cpx<T> *Lp, cpx<T> *v, int c, bool ac
 // gives S_1 = U.y and S_2 = -U.x, hence with R = E - U
 //   E_1 = (R.x, U.y),  E_2 = (R.y, -U.x).
 // This halves the z stage's transforms and the x,y transforms of the items.
-template <int N, typename T, bool MULTI, bool PACK, bool SH>
+template <int N, typename T, bool MULTI, bool PACK, bool CONJ>
 __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
     const cpx<T> *__restrict__ FT, cpx<T> *__restrict__ P, const cpx<T> *__restrict__ ph,
     const cpx<T> *__restrict__ zc, const DevProd *__restrict__ prods, const int *__restrict__ g_p0,
     const int *__restrict__ g_np, const int *__restrict__ item_src, int ngroups, int nsrc,
-    const cpx<T> *__restrict__ NY, int nprods) {
+    const cpx<T> *__restrict__ NY, int nprods, const cpx<T> *__restrict__ zq) {
     using PG = ProdGeo<N>;
     using P1 = Prod1Geo<N>;
     using G3 = Geo<N>;
@@ -746,7 +799,13 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
         if constexpr (PACK) {
             cpx<T> e[N];
             __syncthreads();                               // LDS free
-            zstage_pair<N, T, P1::THREADS, SH>(src1, src2, zc1, zc2, ph1, ph2, lds);
+            if constexpr (CONJ) {
+                const cpx<T> *zqp = zq + (int64_t(pi) * 4 + G) * (3 + 3 * PG::S);
+                if (G & 1) zstage_conj<N, T, P1::THREADS, true>(src1, zqp, ph1, lds);
+                else zstage_conj<N, T, P1::THREADS, false>(src1, zqp, ph1, lds);
+            } else {
+                zstage_pair<N, T, P1::THREADS>(src1, src2, zc1, zc2, ph1, ph2, lds);
+            }
             __syncthreads();
             // the product's corrections: global loads in flight across the
             // x,y transforms, then staged in LDS for the whole workgroup
@@ -842,8 +901,8 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
 // theta_t.nyq}) / 2 on the piece) as F_s1 V_1 + F_s2 V_2 with the per-product
 // weights V_1 = phi_1 (1 - e^{..}) / 2 and V_2 = i phi_2 (1 - e^{..}) / 2
 // (zero off the piece), and the piece's 2D DFT+ runs as a column transform,
-// an LDS transpose and a row transform.  SH: both items read one source and
-// V_1 holds V_1 + V_2.
+// an LDS transpose and a row transform.  SH (conjugate pairs): both items
+// read one source and V_1 holds V_1 + V_2.
 // ---------------------------------------------------------------------------
 template <int N, bool SH>
 __global__ __launch_bounds__(96, 4) void k_nyq(const cpx<float> *__restrict__ FT, cpx<float> *__restrict__ NY,
@@ -1337,7 +1396,8 @@ struct fq_handle {
     bool pack = false;           // k_prod1 with pair packing (k_nyq corrections in NY)
     void *NY = nullptr;          // [chunk][products][3][N^2] cpx<float> Nyquist-plane corrections
     cpx<float> *nyv = nullptr;   // [products][2][3][N][N] k_nyq weights
-    bool pack_shared = false;    // every product's two items read one source
+    bool pack_conj = false;      // every product: one source, conjugate phases (zstage_conj)
+    cpx<float> *zq = nullptr;    // [products][4][3 + 3 N/4] zstage_conj coefficients
     cpx<float> *F32 = nullptr;   // FP32 copy of F (mixed, fused)
     // scratch
     cpx<double> *F = nullptr;
@@ -1431,16 +1491,47 @@ int upload_nyq(fq_handle *h) {
                     }
         }
     }
-    // one source for both items: k_nyq reads the sum V_1 + V_2 once
-    if (shared)
+    bool conj = shared;
+    for (auto &pr : P.prods)
+        for (int d = 0; d < 3; ++d) conj = conj && P.items[pr.i2].th[d] == -P.items[pr.i1].th[d];
+    h->pack_conj = conj;
+    // conjugate pairs read one source: k_nyq reads the sum V_1 + V_2 once
+    if (conj)
         for (size_t p = 0; p < P.prods.size(); ++p)
             for (size_t e = 0; e < 3 * N2; ++e) {
                 cpx<float> &a = v[p * 6 * N2 + e];
                 const cpx<float> b = v[(p * 6 + 3) * N2 + e];
                 a = {a.x + b.x, a.y + b.y};
             }
-    h->pack_shared = shared;
-    return upload(&h->nyv, v);
+    int rc = upload(&h->nyv, v);
+    if (rc != FQ_SUCCESS || !conj) return rc;
+    // zstage_conj coefficients of item 1 of every product, per vz-group G:
+    // beta_b = e^{i th S b'} (b' = b, or b - 4 for b >= 2) times the sign of
+    // w_4^{bG} (its factor i, for odd bG, is applied in the kernel);
+    // alpha_a = e^{i th a}; the twiddle w_N^{aG}
+    const int S = N / 4, ZQ = 3 + 3 * S;
+    std::vector<cpx<float>> zq(P.prods.size() * 4 * ZQ);
+    for (size_t p = 0; p < P.prods.size(); ++p) {
+        const double th = P.items[P.prods[p].i1].th[2];
+        for (int G = 0; G < 4; ++G) {
+            cpx<float> *z = zq.data() + (p * 4 + G) * ZQ;
+            for (int b = 1; b < 4; ++b) {
+                const int q4 = (b * G) % 4;               // w_4^{bG} = i^q4
+                const double sg = (q4 == 0 || q4 == 1) ? 1. : -1.;
+                const double bp = b < 2 ? b : b - 4;
+                z[b - 1] = {float(sg * std::cos(th * S * bp)), float(sg * std::sin(th * S * bp))};
+            }
+            for (int a = 0; a < S; ++a) {
+                const double ca = std::cos(th * a), sa = std::sin(th * a);
+                z[3 + a] = {float(ca + sa), float(sa - ca)};            // (1 - i) alpha_a
+                z[3 + S + a] = {float(ca - sa), float(ca + sa)};        // (1 + i) alpha_a
+                const double tw = 2 * M_PI * double((a * G) % N) / N;
+                const double cr = std::cos(tw), ci = std::sin(tw);
+                z[3 + 2 * S + a] = {float(cr - ci), float(cr + ci)};    // (1 + i) w_N^{aG}
+            }
+        }
+    }
+    return upload(&h->zq, zq);
 }
 
 int build_tables(fq_handle *h) {
@@ -1601,20 +1692,23 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
                 const dim3 blk1(Prod1Geo<N>::THREADS);
                 const int nprods = int(P.prods.size());
                 cpx<T> *NY = reinterpret_cast<cpx<T> *>(h->NY);
-                const bool sh = h->pack_shared;
+                const bool cj = h->pack_conj;
                 if (h->pack) {
-                    auto knyq = sh ? k_nyq<N, true> : k_nyq<N, false>;
+                    auto knyq = cj ? k_nyq<N, true> : k_nyq<N, false>;
                     pre();
                     hipLaunchKernelGGL(knyq, dim3(unsigned(int64_t(D) * nprods)), dim3(96), 0, st, FT, NY, h->nyv,
                                        h->prods, h->item_src, nprods, nsrc);
                     post(8);
                 }
-                auto kern = !h->pack ? (multi ? k_prod1<N, T, true, false, false> : k_prod1<N, T, false, false, false>)
-                            : sh     ? (multi ? k_prod1<N, T, true, true, true> : k_prod1<N, T, false, true, true>)
-                                     : (multi ? k_prod1<N, T, true, true, false> : k_prod1<N, T, false, true, false>);
+                using KP = decltype(&k_prod1<N, T, true, true, false>);
+                KP kern;
+                if (!h->pack) kern = multi ? k_prod1<N, T, true, false, false> : k_prod1<N, T, false, false, false>;
+                else if (cj) kern = multi ? k_prod1<N, T, true, true, true> : k_prod1<N, T, false, true, true>;
+                else kern = multi ? k_prod1<N, T, true, true, false> : k_prod1<N, T, false, true, false>;
+                const cpx<T> *zq = reinterpret_cast<const cpx<T> *>(h->zq);
                 pre();
                 hipLaunchKernelGGL(kern, grid, blk1, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0, h->g_np,
-                                   h->item_src, ngroups, nsrc, NY, nprods);
+                                   h->item_src, ngroups, nsrc, NY, nprods, zq);
                 post(7);
             }
         }
@@ -1903,7 +1997,7 @@ int fq_destroy(fq_handle *h) {
     for (void *p : {h->ph, (void *)h->item_src, (void *)h->prods, (void *)h->g_p0, (void *)h->g_np,
                     (void *)h->c_group, (void *)h->t_c0, (void *)h->t_cn, (void *)h->l_src, (void *)h->t_l0,
                     (void *)h->t_ln, (void *)h->W, (void *)h->ginv, (void *)h->pmass, (void *)h->F, h->Z,
-                    h->Pg, h->zc, (void *)h->F32, h->NY, (void *)h->nyv})
+                    h->Pg, h->zc, (void *)h->F32, h->NY, (void *)h->nyv, (void *)h->zq})
         if (p) (void)hipFree(p);
     (void)hipThreadExchangeStreamCaptureMode(&mode);
     delete h;
