@@ -78,6 +78,11 @@ struct BufView {
 template <> __device__ __forceinline__ cpx<float> BufView::ld<float>(uint32_t voff, uint32_t soff) const {
     return __builtin_bit_cast(cpx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, int(voff), int(soff), 0));
 }
+// Two consecutive FP32 complex values in one 16-byte load.
+struct cpx2f { cpx<float> a, b; };
+__device__ __forceinline__ cpx2f ld2f(const BufView &v, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(cpx2f, __builtin_amdgcn_raw_buffer_load_b128(v.r, int(voff), int(soff), 0));
+}
 template <> __device__ __forceinline__ cpx<double> BufView::ld<double>(uint32_t voff, uint32_t soff) const {
     return __builtin_bit_cast(cpx<double>, __builtin_amdgcn_raw_buffer_load_b128(r, int(voff), int(soff), 0));
 }
@@ -476,6 +481,8 @@ __device__ __forceinline__ void zstage(const cpx<T> *__restrict__ src, const cpx
 // Pruned z-DFT of the packed pair T_1 + i T_2 of one product (the
 // pair-packed k_prod1, general pairs): both items' combinations as in
 // zstage, with their x,y phases applied before one shared S-point transform.
+// Each lane takes two neighbouring pencils (J, J + 1) per step with 16-byte
+// loads, one source after the other.
 template <int N, typename T, int THREADS>
 __device__ __forceinline__ void zstage_pair(const cpx<T> *__restrict__ src1, const cpx<T> *__restrict__ src2,
                                             const cpx<T> *__restrict__ zc1, const cpx<T> *__restrict__ zc2,
@@ -483,8 +490,7 @@ __device__ __forceinline__ void zstage_pair(const cpx<T> *__restrict__ src1, con
                                             cpx<T> *dst) {
     using PG = ProdGeo<N>;
     constexpr int S = PG::S;
-    static_assert((N * N) % THREADS == 0, "whole pencils per thread");
-    constexpr int ITER = N * N / THREADS;
+    constexpr int ITER = (N * N / 2 + THREADS - 1) / THREADS;
     cpx<T> d1[4], cc1[S], d2[4], cc2[S];
 #pragma unroll
     for (int b = 1; b < 4; ++b) { d1[b] = zc1[b]; d2[b] = zc2[b]; }
@@ -494,57 +500,53 @@ __device__ __forceinline__ void zstage_pair(const cpx<T> *__restrict__ src1, con
     const BufView sv1(src1, uint32_t(N) * PB), sv2(src2, uint32_t(N) * PB);
 #pragma unroll 1
     for (int it = 0; it < ITER; ++it) {
-        const uint32_t p = threadIdx.x + uint32_t(it * THREADS);
-        const uint32_t I = p / N, J = p % N;
-        const uint32_t vo = p * uint32_t(sizeof(cpx<T>));
-        cpx<T> t1[S], t2[S];
-        {
-            cpx<T> x[4][S];
+        const uint32_t p2 = threadIdx.x + uint32_t(it * THREADS);
+        if ((N * N / 2) % THREADS != 0 && p2 >= uint32_t(N * N / 2)) break;
+        const uint32_t I = p2 / (N / 2), J0 = 2 * (p2 % (N / 2));
+        const uint32_t vo = (I * N + J0) * uint32_t(sizeof(cpx<T>));
+        // t[h][item][a]
+        cpx<T> t[2][2][S];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const BufView &sv = m ? sv2 : sv1;
+            const cpx<T> *dm = m ? d2 : d1;
+            cpx2f x[4][S];
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
 #pragma unroll
-                for (int a = 0; a < S; ++a) x[b][a] = sv1.ld<T>(vo, uint32_t(a + S * b) * PB);
+                for (int a = 0; a < S; ++a) x[b][a] = ld2f(sv, vo, uint32_t(a + S * b) * PB);
             }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+#pragma unroll
+                for (int a = 0; a < S; ++a) {
+                    cpx<T> v = h ? x[0][a].b : x[0][a].a;
+#pragma unroll
+                    for (int b = 1; b < 4; ++b) fq::cmac(v, h ? x[b][a].b : x[b][a].a, dm[b]);
+                    t[h][m][a] = v;
+                }
+            }
+        }
+        const cpx<T> px1 = ph1[I], px2 = ph2[I];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            // u = pxy1 cc1 t1 + i pxy2 cc2 t2, transformed once
+            const cpx<T> pxy1 = fq::cmul(px1, ph1[N + J0 + h]);
+            const cpx<T> q2 = fq::cmul(px2, ph2[N + J0 + h]);
+            const cpx<T> pxy2 = {-q2.y, q2.x};
+            cpx<T> u[S];
 #pragma unroll
             for (int a = 0; a < S; ++a) {
-                t1[a] = x[0][a];
-#pragma unroll
-                for (int b = 1; b < 4; ++b) fq::cmac(t1[a], x[b][a], d1[b]);
+                const cpx<T> v1 = a ? fq::cmul(t[h][0][a], cc1[a]) : t[h][0][a];
+                const cpx<T> v2 = a ? fq::cmul(t[h][1][a], cc2[a]) : t[h][1][a];
+                u[a] = fq::cmul(v1, pxy1);
+                fq::cmac(u[a], v2, pxy2);
             }
+            fq::LFFT<S, 1>::run(u);
+            cpx<T> *dd = dst + I * PG::ROW + J0 + h;
+#pragma unroll
+            for (int k = 0; k < S; ++k) dd[k * PG::PLANE] = u[k];
         }
-        {
-            cpx<T> x[4][S];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-#pragma unroll
-                for (int a = 0; a < S; ++a) x[b][a] = sv2.ld<T>(vo, uint32_t(a + S * b) * PB);
-            }
-#pragma unroll
-            for (int a = 0; a < S; ++a) {
-                t2[a] = x[0][a];
-#pragma unroll
-                for (int b = 1; b < 4; ++b) fq::cmac(t2[a], x[b][a], d2[b]);
-            }
-        }
-#pragma unroll
-        for (int a = 1; a < S; ++a) {
-            t1[a] = fq::cmul(t1[a], cc1[a]);
-            t2[a] = fq::cmul(t2[a], cc2[a]);
-        }
-        // u = pxy1 t1 + i pxy2 t2, transformed once
-        const cpx<T> pxy1 = fq::cmul(ph1[I], ph1[N + J]);
-        const cpx<T> q2 = fq::cmul(ph2[I], ph2[N + J]);
-        const cpx<T> pxy2 = {-q2.y, q2.x};
-#pragma unroll
-        for (int a = 0; a < S; ++a) {
-            cpx<T> u = fq::cmul(t1[a], pxy1);
-            fq::cmac(u, t2[a], pxy2);
-            t1[a] = u;
-        }
-        fq::LFFT<S, 1>::run(t1);
-        cpx<T> *dd = dst + I * PG::ROW + J;
-#pragma unroll
-        for (int k = 0; k < S; ++k) dd[k * PG::PLANE] = t1[k];
     }
 }
 
@@ -559,14 +561,15 @@ __device__ __forceinline__ void zstage_pair(const cpx<T> *__restrict__ src1, con
 // products.  zq[(prod*4 + G)*(3 + 3S) + .] holds {(p_b, q_b) for b = 1..3
 // with the sign of w_4^{bG} folded in}, (1 - i) alpha_a, (1 + i) alpha_a and
 // (1 + i) w_N^{aG}.  ODD (G odd): w_4^{bG} = +-i for odd b, applied as a
-// rotation of x_b.
+// rotation of x_b.  Each lane takes two neighbouring pencils (J, J + 1)
+// per step with 16-byte loads: half the load instructions of one pencil
+// per lane, which the vector memory pipeline, not the arithmetic, limits.
 template <int N, typename T, int THREADS, bool ODD>
 __device__ __forceinline__ void zstage_conj(const cpx<T> *__restrict__ src, const cpx<T> *__restrict__ zq,
                                             const cpx<T> *__restrict__ ph1, cpx<T> *dst) {
     using PG = ProdGeo<N>;
     constexpr int S = PG::S;
-    static_assert((N * N) % THREADS == 0, "whole pencils per thread");
-    constexpr int ITER = N * N / THREADS;
+    constexpr int ITER = (N * N / 2 + THREADS - 1) / THREADS;
     cpx<T> pq[4], am[S], ap[S], cw[S];
 #pragma unroll
     for (int b = 1; b < 4; ++b) pq[b] = zq[b - 1];
@@ -576,37 +579,42 @@ __device__ __forceinline__ void zstage_conj(const cpx<T> *__restrict__ src, cons
     const BufView sv(src, uint32_t(N) * PB);
 #pragma unroll 1
     for (int it = 0; it < ITER; ++it) {
-        const uint32_t p = threadIdx.x + uint32_t(it * THREADS);
-        const uint32_t I = p / N, J = p % N;
-        const uint32_t vo = p * uint32_t(sizeof(cpx<T>));
-        cpx<T> x[4][S];
+        const uint32_t p2 = threadIdx.x + uint32_t(it * THREADS);
+        if ((N * N / 2) % THREADS != 0 && p2 >= uint32_t(N * N / 2)) break;
+        const uint32_t I = p2 / (N / 2), J0 = 2 * (p2 % (N / 2));
+        const uint32_t vo = (I * N + J0) * uint32_t(sizeof(cpx<T>));
+        cpx2f x[4][S];
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
 #pragma unroll
-            for (int a = 0; a < S; ++a) x[b][a] = sv.ld<T>(vo, uint32_t(a + S * b) * PB);
+            for (int a = 0; a < S; ++a) x[b][a] = ld2f(sv, vo, uint32_t(a + S * b) * PB);
         }
-        const cpx<T> pxy = fq::cmul(ph1[I], ph1[N + J]);
-        cpx<T> u[S];
+        const cpx<T> phx = ph1[I];
 #pragma unroll
-        for (int a = 0; a < S; ++a) {
-            cpx<T> P = x[0][a], Q;
+        for (int h = 0; h < 2; ++h) {
+            const cpx<T> pxy = fq::cmul(phx, ph1[N + J0 + h]);
+            cpx<T> u[S];
 #pragma unroll
-            for (int b = 1; b < 4; ++b) {
-                cpx<T> y = x[b][a];
-                if (ODD && (b & 1)) y = {-y.y, y.x};
-                P = {fq::fmadd(pq[b].x, y.x, P.x), fq::fmadd(pq[b].x, y.y, P.y)};
-                if (b == 1) Q = {pq[b].y * y.x, pq[b].y * y.y};
-                else Q = {fq::fmadd(pq[b].y, y.x, Q.x), fq::fmadd(pq[b].y, y.y, Q.y)};
+            for (int a = 0; a < S; ++a) {
+                cpx<T> P = h ? x[0][a].b : x[0][a].a, Q;
+#pragma unroll
+                for (int b = 1; b < 4; ++b) {
+                    cpx<T> y = h ? x[b][a].b : x[b][a].a;
+                    if (ODD && (b & 1)) y = {-y.y, y.x};
+                    P = {fq::fmadd(pq[b].x, y.x, P.x), fq::fmadd(pq[b].x, y.y, P.y)};
+                    if (b == 1) Q = {pq[b].y * y.x, pq[b].y * y.y};
+                    else Q = {fq::fmadd(pq[b].y, y.x, Q.x), fq::fmadd(pq[b].y, y.y, Q.y)};
+                }
+                const T A = fq::fmadd(am[a].x, pxy.x, -(am[a].y * pxy.y));
+                const T B = fq::fmadd(ap[a].x, pxy.x, -(ap[a].y * pxy.y));
+                const cpx<T> z = {fq::fmadd(A, P.x, B * Q.x), fq::fmadd(A, P.y, B * Q.y)};
+                u[a] = fq::cmul(z, cw[a]);
             }
-            const T A = fq::fmadd(am[a].x, pxy.x, -(am[a].y * pxy.y));
-            const T B = fq::fmadd(ap[a].x, pxy.x, -(ap[a].y * pxy.y));
-            const cpx<T> z = {fq::fmadd(A, P.x, B * Q.x), fq::fmadd(A, P.y, B * Q.y)};
-            u[a] = fq::cmul(z, cw[a]);
-        }
-        fq::LFFT<S, 1>::run(u);
-        cpx<T> *dd = dst + I * PG::ROW + J;
+            fq::LFFT<S, 1>::run(u);
+            cpx<T> *dd = dst + I * PG::ROW + J0 + h;
 #pragma unroll
-        for (int k = 0; k < S; ++k) dd[k * PG::PLANE] = u[k];
+            for (int k = 0; k < S; ++k) dd[k * PG::PLANE] = u[k];
+        }
     }
 }
 
