@@ -202,6 +202,11 @@ __device__ __forceinline__ double swap_adj(double v) {
     return __builtin_bit_cast(double, r);
 }
 template <typename T> __device__ __forceinline__ cpx<T> swap_adj(cpx<T> v) { return {swap_adj(v.x), swap_adj(v.y)}; }
+// Value of lane ^ 2 (DPP quad_perm [2,3,0,1]).
+__device__ __forceinline__ float swap_q2(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, true));
+}
+__device__ __forceinline__ cpx<float> swap_q2(cpx<float> v) { return {swap_q2(v.x), swap_q2(v.y)}; }
 // partner's value + (hi ? -v : v).  FP32: a sign select feeding an add whose
 // other operand is the DPP swap, which folds into one v_add_f32_dpp.  FP64
 // has no DPP form of its add, so the sign enters one FMA.

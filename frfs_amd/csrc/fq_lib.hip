@@ -1117,10 +1117,14 @@ __global__ __launch_bounds__(256, 2) void k_accum_lds(
     const cpx<double> *wrow = W + int64_t(I) * N;
     // FP64 pencils load through a buffer view of the (wave-uniform) group
     // volume (one lane offset register, plane offsets in SGPRs), which frees
-    // the registers that keep the FP64 kernel at three waves per SIMD; the
-    // prefetching FP32 kernel measured faster with plain loads
+    // the registers that keep the FP64 kernel at three waves per SIMD.  FP32
+    // pencils load as 16-byte pairs: lane (J, half) reads columns J & ~1 and
+    // J | 1 at the planes half + 2e with e = jo + 2f (jo = J & 1), and the
+    // value of the other column goes to the lane of J ^ 1 (lane ^ 2) in
+    // pair_swap, so a pencil costs E/2 load instructions instead of E.
     const cpx<T> *pslot = P + int64_t(live ? slot : 0) * ngroups * G::VOL;
     const uint32_t voP = uint32_t(off + int64_t(half) * G::N2) * uint32_t(sizeof(cpx<T>));
+    const int jo = J & 1;
     auto load_pencil = [&](int ci, cpx<T> *dst) {
         if constexpr (sizeof(T) == 8) {
             const BufView pv(pslot + int64_t(c_group[ci]) * G::VOL, uint32_t(G::VOL * sizeof(cpx<T>)));
@@ -1128,10 +1132,26 @@ __global__ __launch_bounds__(256, 2) void k_accum_lds(
             for (int e = 0; e < E; ++e)
                 dst[e] = (live && act) ? pv.ld<T>(voP, uint32_t(2 * e * G::N2 * sizeof(cpx<T>))) : cpx<T>{0, 0};
         } else {
-            const cpx<T> *src = pslot + int64_t(c_group[ci]) * G::VOL + off;
+            const cpx<T> *src = pslot + int64_t(c_group[ci]) * G::VOL + int64_t(I) * N + (J & ~1);
 #pragma unroll
-            for (int e = 0; e < E; ++e)
-                dst[e] = (live && act) ? src[int64_t(half + 2 * e) * G::N2] : cpx<T>{0, 0};
+            for (int f = 0; f < E / 2; ++f) {
+                cpx2f w = {{0, 0}, {0, 0}};
+                if (live && act) w = *reinterpret_cast<const cpx2f *>(src + int64_t(half + 2 * (jo + 2 * f)) * G::N2);
+                dst[2 * f] = w.a;
+                dst[2 * f + 1] = w.b;
+            }
+        }
+    };
+    // FP32 pairs as loaded -> this lane's pencil at planes half + 2e
+    auto pair_swap = [&](cpx<T> *v) {
+        if constexpr (sizeof(T) == 4) {
+#pragma unroll
+            for (int f = 0; f < E / 2; ++f) {
+                const cpx<T> a = v[2 * f], b = v[2 * f + 1];
+                const cpx<T> recv = fq::swap_q2(jo ? a : b);
+                v[2 * f] = jo ? recv : a;
+                v[2 * f + 1] = jo ? b : recv;
+            }
         }
     };
     cpx<double> H[E];
@@ -1185,6 +1205,7 @@ __global__ __launch_bounds__(256, 2) void k_accum_lds(
             load_pencil(ci, v);
         }
         if (live) {
+            pair_swap(v);
             fq::fft2_dit<N, -1>(v, half);             // -> K' = half*E + e
 #pragma unroll
             for (int e = 0; e < E; ++e) {
