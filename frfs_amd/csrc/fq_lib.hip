@@ -561,15 +561,38 @@ __device__ __forceinline__ void zstage_pair(const cpx<T> *__restrict__ src1, con
 // products.  zq[(prod*4 + G)*(3 + 3S) + .] holds {(p_b, q_b) for b = 1..3
 // with the sign of w_4^{bG} folded in}, (1 - i) alpha_a, (1 + i) alpha_a and
 // (1 + i) w_N^{aG}.  ODD (G odd): w_4^{bG} = +-i for odd b, applied as a
-// rotation of x_b.  Each lane takes two neighbouring pencils (J, J + 1)
-// per step with 16-byte loads: half the load instructions of one pencil
-// per lane, which the vector memory pipeline, not the arithmetic, limits.
+// rotation of x_b.  In FP32 each lane takes two neighbouring pencils
+// (J, J + 1) per step with 16-byte loads: half the load instructions of one
+// pencil per lane, which the vector memory pipeline, not the arithmetic,
+// limits.
+template <int S, typename T, bool ODD>
+__device__ __forceinline__ void conj_pencil(const cpx<T> (&x)[4][S], cpx<T> pxy, const cpx<T> (&pq)[4],
+                                            const cpx<T> (&am)[S], const cpx<T> (&ap)[S], const cpx<T> (&cw)[S],
+                                            cpx<T> (&u)[S]) {
+#pragma unroll
+    for (int a = 0; a < S; ++a) {
+        cpx<T> P = x[0][a], Q;
+#pragma unroll
+        for (int b = 1; b < 4; ++b) {
+            cpx<T> y = x[b][a];
+            if (ODD && (b & 1)) y = {-y.y, y.x};
+            P = {fq::fmadd(pq[b].x, y.x, P.x), fq::fmadd(pq[b].x, y.y, P.y)};
+            if (b == 1) Q = {pq[b].y * y.x, pq[b].y * y.y};
+            else Q = {fq::fmadd(pq[b].y, y.x, Q.x), fq::fmadd(pq[b].y, y.y, Q.y)};
+        }
+        const T A = fq::fmadd(am[a].x, pxy.x, -(am[a].y * pxy.y));
+        const T B = fq::fmadd(ap[a].x, pxy.x, -(ap[a].y * pxy.y));
+        const cpx<T> z = {fq::fmadd(A, P.x, B * Q.x), fq::fmadd(A, P.y, B * Q.y)};
+        u[a] = fq::cmul(z, cw[a]);
+    }
+    fq::LFFT<S, 1>::run(u);
+}
+
 template <int N, typename T, int THREADS, bool ODD>
 __device__ __forceinline__ void zstage_conj(const cpx<T> *__restrict__ src, const cpx<T> *__restrict__ zq,
                                             const cpx<T> *__restrict__ ph1, cpx<T> *dst) {
     using PG = ProdGeo<N>;
     constexpr int S = PG::S;
-    constexpr int ITER = (N * N / 2 + THREADS - 1) / THREADS;
     cpx<T> pq[4], am[S], ap[S], cw[S];
 #pragma unroll
     for (int b = 1; b < 4; ++b) pq[b] = zq[b - 1];
@@ -577,41 +600,53 @@ __device__ __forceinline__ void zstage_conj(const cpx<T> *__restrict__ src, cons
     for (int a = 0; a < S; ++a) { am[a] = zq[3 + a]; ap[a] = zq[3 + S + a]; cw[a] = zq[3 + 2 * S + a]; }
     constexpr uint32_t PB = uint32_t(N * N * sizeof(cpx<T>));
     const BufView sv(src, uint32_t(N) * PB);
+    if constexpr (sizeof(T) == 4) {
+        // two neighbouring pencils (J, J + 1) per lane and step
+        constexpr int ITER = (N * N / 2 + THREADS - 1) / THREADS;
 #pragma unroll 1
-    for (int it = 0; it < ITER; ++it) {
-        const uint32_t p2 = threadIdx.x + uint32_t(it * THREADS);
-        if ((N * N / 2) % THREADS != 0 && p2 >= uint32_t(N * N / 2)) break;
-        const uint32_t I = p2 / (N / 2), J0 = 2 * (p2 % (N / 2));
-        const uint32_t vo = (I * N + J0) * uint32_t(sizeof(cpx<T>));
-        cpx2f x[4][S];
+        for (int it = 0; it < ITER; ++it) {
+            const uint32_t p2 = threadIdx.x + uint32_t(it * THREADS);
+            if ((N * N / 2) % THREADS != 0 && p2 >= uint32_t(N * N / 2)) break;
+            const uint32_t I = p2 / (N / 2), J0 = 2 * (p2 % (N / 2));
+            const uint32_t vo = (I * N + J0) * uint32_t(sizeof(cpx<T>));
+            cpx2f x2[4][S];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
+            for (int b = 0; b < 4; ++b) {
 #pragma unroll
-            for (int a = 0; a < S; ++a) x[b][a] = ld2f(sv, vo, uint32_t(a + S * b) * PB);
-        }
-        const cpx<T> phx = ph1[I];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const cpx<T> pxy = fq::cmul(phx, ph1[N + J0 + h]);
-            cpx<T> u[S];
-#pragma unroll
-            for (int a = 0; a < S; ++a) {
-                cpx<T> P = h ? x[0][a].b : x[0][a].a, Q;
-#pragma unroll
-                for (int b = 1; b < 4; ++b) {
-                    cpx<T> y = h ? x[b][a].b : x[b][a].a;
-                    if (ODD && (b & 1)) y = {-y.y, y.x};
-                    P = {fq::fmadd(pq[b].x, y.x, P.x), fq::fmadd(pq[b].x, y.y, P.y)};
-                    if (b == 1) Q = {pq[b].y * y.x, pq[b].y * y.y};
-                    else Q = {fq::fmadd(pq[b].y, y.x, Q.x), fq::fmadd(pq[b].y, y.y, Q.y)};
-                }
-                const T A = fq::fmadd(am[a].x, pxy.x, -(am[a].y * pxy.y));
-                const T B = fq::fmadd(ap[a].x, pxy.x, -(ap[a].y * pxy.y));
-                const cpx<T> z = {fq::fmadd(A, P.x, B * Q.x), fq::fmadd(A, P.y, B * Q.y)};
-                u[a] = fq::cmul(z, cw[a]);
+                for (int a = 0; a < S; ++a) x2[b][a] = ld2f(sv, vo, uint32_t(a + S * b) * PB);
             }
-            fq::LFFT<S, 1>::run(u);
-            cpx<T> *dd = dst + I * PG::ROW + J0 + h;
+            const cpx<T> phx = ph1[I];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                cpx<T> x[4][S], u[S];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+#pragma unroll
+                    for (int a = 0; a < S; ++a) x[b][a] = h ? x2[b][a].b : x2[b][a].a;
+                }
+                conj_pencil<S, T, ODD>(x, fq::cmul(phx, ph1[N + J0 + h]), pq, am, ap, cw, u);
+                cpx<T> *dd = dst + I * PG::ROW + J0 + h;
+#pragma unroll
+                for (int k = 0; k < S; ++k) dd[k * PG::PLANE] = u[k];
+            }
+        }
+    } else {
+        // one pencil per lane and step (16-byte FP64 values)
+        constexpr int ITER = (N * N + THREADS - 1) / THREADS;
+#pragma unroll 1
+        for (int it = 0; it < ITER; ++it) {
+            const uint32_t p = threadIdx.x + uint32_t(it * THREADS);
+            if ((N * N) % THREADS != 0 && p >= uint32_t(N * N)) break;
+            const uint32_t I = p / N, J = p % N;
+            const uint32_t vo = p * uint32_t(sizeof(cpx<T>));
+            cpx<T> x[4][S], u[S];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+#pragma unroll
+                for (int a = 0; a < S; ++a) x[b][a] = sv.ld<T>(vo, uint32_t(a + S * b) * PB);
+            }
+            conj_pencil<S, T, ODD>(x, fq::cmul(ph1[I], ph1[N + J]), pq, am, ap, cw, u);
+            cpx<T> *dd = dst + I * PG::ROW + J;
 #pragma unroll
             for (int k = 0; k < S; ++k) dd[k * PG::PLANE] = u[k];
         }
@@ -654,16 +689,21 @@ __device__ __forceinline__ void plane_inv_lds(cpx<T> *Lp, cpx<T> *v, int c, int 
     fq::fft2_dif<N, 1>(v, half);
 }
 
-template <int N, typename T, bool MULTI>
+template <int N, typename T, bool MULTI, bool PACK>
 __global__ __launch_bounds__(ProdGeo<N>::THREADS, (sizeof(T) == 4 ? 4 : 2)) void k_prod(const cpx<T> *__restrict__ FT, cpx<T> *__restrict__ P, const cpx<T> *__restrict__ ph,
                                                               const cpx<T> *__restrict__ zc, const DevProd *__restrict__ prods,
                                                               const int *__restrict__ g_p0, const int *__restrict__ g_np,
-                                                              const int *__restrict__ item_src, int ngroups, int nsrc) {
+                                                              const int *__restrict__ item_src, int ngroups, int nsrc,
+                                                              const cpx<T> *__restrict__ NY, int nprods,
+                                                              const cpx<T> *__restrict__ zq) {
     using PG = ProdGeo<N>;
     using G3 = Geo<N>;
     constexpr int E = PG::E;
     constexpr int ZC = 4 + PG::S;
-    __shared__ cpx<T> lds[PG::S * PG::PLANE];
+    // LDS: the S planes, then (PACK, conjugate pairs only) the product's
+    // Nyquist corrections as in k_prod1
+    constexpr int NYL = PACK ? N * N + 2 * PG::S * N : 0;
+    __shared__ cpx<T> lds[PG::S * PG::PLANE + NYL];
     const int64_t unit = blockIdx.x;
     const int G = int(unit % 4);
     const int g = int((unit / 4) % ngroups);
@@ -681,29 +721,82 @@ __global__ __launch_bounds__(ProdGeo<N>::THREADS, (sizeof(T) == 4 ? 4 : 2)) void
     for (int pi = p0; pi < p0 + np; ++pi) {
         const DevProd pr = prods[pi];
         const cpx<T> *src1 = FT + (int64_t(slot) * nsrc + item_src[pr.i1]) * G3::VOL;
-        const cpx<T> *src2 = FT + (int64_t(slot) * nsrc + item_src[pr.i2]) * G3::VOL;
-        const cpx<T> *zc1 = zc + (int64_t(pr.i1) * 4 + G) * ZC, *zc2 = zc + (int64_t(pr.i2) * 4 + G) * ZC;
-        const cpx<T> *ph1 = ph + int64_t(pr.i1) * 3 * N, *ph2 = ph + int64_t(pr.i2) * 3 * N;
-        cpx<T> e1[E], e2[E];
-        __syncthreads();                                   // LDS free
-        zstage<N, T>(src1, zc1, ph1, lds);
-        __syncthreads();
-        plane_inv_lds<N, T>(L1, e1, c, half, act);
-        __syncthreads();
-        zstage<N, T>(src2, zc2, ph2, lds);
-        __syncthreads();
-        plane_inv_lds<N, T>(L1, e2, c, half, act);
+        const cpx<T> *ph1 = ph + int64_t(pr.i1) * 3 * N;
         const T cf = T(pr.coef);
-        if constexpr (MULTI) {
+        if constexpr (PACK) {
+            static_assert(PG::LPP == 64, "one plane per wave");
+            cpx<T> ev[E];
+            __syncthreads();                                   // LDS free
+            const cpx<T> *zqp = zq + (int64_t(pi) * 4 + G) * (3 + 3 * PG::S);
+            if (G & 1) zstage_conj<N, T, PG::THREADS, true>(src1, zqp, ph1, lds);
+            else zstage_conj<N, T, PG::THREADS, false>(src1, zqp, ph1, lds);
+            __syncthreads();
+            constexpr int KPT = (N * N + PG::THREADS - 1) / PG::THREADS;
+            constexpr int SN = PG::S * N;
+            static_assert(SN <= PG::THREADS, "one J and one I element per thread");
+            const cpx<T> *ny = NY + (int64_t(slot) * nprods + pi) * 3 * G3::N2;
+            const int tid = int(threadIdx.x);
+            cpx<T> rk[KPT], rj = {0, 0}, ri = {0, 0};
+#pragma unroll
+            for (int r = 0; r < KPT; ++r) {
+                const int idx = tid + r * PG::THREADS;
+                rk[r] = (N * N % PG::THREADS == 0 || idx < N * N) ? ny[idx] : cpx<T>{0, 0};
+            }
+            if (tid < SN) {
+                const int kk = tid / N, w = tid % N, z = G + 4 * kk;
+                rj = ny[G3::N2 + z * N + w];
+                ri = ny[2 * G3::N2 + w * N + z];
+            }
+            plane_inv_lds<N, T>(L1, ev, c, half, act);        // lane (c, half): vx = c, vy = half + 2e
+            cpx<T> *nyk = lds + PG::S * PG::PLANE, *nyj = nyk + N * N, *nyi = nyj + SN;
+#pragma unroll
+            for (int r = 0; r < KPT; ++r) {
+                const int idx = tid + r * PG::THREADS;
+                if (N * N % PG::THREADS == 0 || idx < N * N) nyk[idx] = rk[r];
+            }
+            if (tid < SN) { nyj[tid] = rj; nyi[tid] = ri; }
+            __syncthreads();
+            const T sz = (vz & 1) ? T(-1) : T(1);
+            const T sc = (c & 1) ? T(-1) : T(1);
+            cpx<T> xj = act ? nyj[k * N + c] : cpx<T>{0, 0};
+            if (half) xj = {-xj.x, -xj.y};                  // (-1)^vy, vy = half + 2e
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                const cpx<T> pe = fq::cmul(e1[e], e2[e]);
-                acc[e] = {fq::fmadd(cf, pe.x, acc[e].x), fq::fmadd(cf, pe.y, acc[e].y)};
+                const int vy = half + 2 * e;
+                const cpx<T> xk = act ? nyk[vy * N + c] : cpx<T>{0, 0};
+                const cpx<T> xi = nyi[k * N + vy];
+                const T ux = fq::fmadd(sc, xi.x, fq::fmadd(sz, xk.x, xj.x));
+                const T uy = fq::fmadd(sc, xi.y, fq::fmadd(sz, xk.y, xj.y));
+                const T rx = ev[e].x - ux, ry = ev[e].y - uy;
+                const T px = fq::fmadd(rx, ry, uy * ux);
+                const T py = fq::fmadd(uy, ry, -(rx * ux));
+                if constexpr (MULTI) acc[e] = {fq::fmadd(cf, px, acc[e].x), fq::fmadd(cf, py, acc[e].y)};
+                else acc[e] = {cf * px, cf * py};
             }
         } else {
-            // one product per group: acc is not live across the item stages
+            const cpx<T> *src2 = FT + (int64_t(slot) * nsrc + item_src[pr.i2]) * G3::VOL;
+            const cpx<T> *zc1 = zc + (int64_t(pr.i1) * 4 + G) * ZC, *zc2 = zc + (int64_t(pr.i2) * 4 + G) * ZC;
+            const cpx<T> *ph2 = ph + int64_t(pr.i2) * 3 * N;
+            cpx<T> e1[E], e2[E];
+            __syncthreads();                                   // LDS free
+            zstage<N, T>(src1, zc1, ph1, lds);
+            __syncthreads();
+            plane_inv_lds<N, T>(L1, e1, c, half, act);
+            __syncthreads();
+            zstage<N, T>(src2, zc2, ph2, lds);
+            __syncthreads();
+            plane_inv_lds<N, T>(L1, e2, c, half, act);
+            if constexpr (MULTI) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) acc[e] = fq::cscale(fq::cmul(e1[e], e2[e]), cf);
+                for (int e = 0; e < E; ++e) {
+                    const cpx<T> pe = fq::cmul(e1[e], e2[e]);
+                    acc[e] = {fq::fmadd(cf, pe.x, acc[e].x), fq::fmadd(cf, pe.y, acc[e].y)};
+                }
+            } else {
+                // one product per group: acc is not live across the item stages
+#pragma unroll
+                for (int e = 0; e < E; ++e) acc[e] = fq::cscale(fq::cmul(e1[e], e2[e]), cf);
+            }
         }
     }
     // forward transforms of the group sum: acc is row vx = c, cyclic over vy
@@ -900,7 +993,8 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
 }
 
 // ---------------------------------------------------------------------------
-// k_nyq: the Nyquist-plane corrections of the pair-packed k_prod1 (FP32).
+// k_nyq: the Nyquist-plane corrections of the pair-packed product kernels
+// (k_prod1 in FP32; the two-lane k_prod for conjugate pairs in FP64).
 // One workgroup per (slot, product), one 32-lane group per plane piece:
 //   piece 0 (K = N/2):                 rows a = I, lanes = J   -> NY0[vy][vx]
 //   piece 1 (J = N/2, K != N/2):       rows a = I, lanes = K   -> NY1[vz][vx]
@@ -912,26 +1006,26 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
 // an LDS transpose and a row transform.  SH (conjugate pairs): both items
 // read one source and V_1 holds V_1 + V_2.
 // ---------------------------------------------------------------------------
-template <int N, bool SH>
-__global__ __launch_bounds__(96, 4) void k_nyq(const cpx<float> *__restrict__ FT, cpx<float> *__restrict__ NY,
-                                               const cpx<float> *__restrict__ V, const DevProd *__restrict__ prods,
+template <int N, typename T, bool SH>
+__global__ __launch_bounds__(96, (sizeof(T) == 4 ? 4 : 2)) void k_nyq(const cpx<T> *__restrict__ FT, cpx<T> *__restrict__ NY,
+                                               const cpx<T> *__restrict__ V, const DevProd *__restrict__ prods,
                                                const int *__restrict__ item_src, int nprods, int nsrc) {
     using G3 = Geo<N>;
     constexpr int H = N / 2, ROW = N + 1;
-    __shared__ cpx<float> lds[3 * N * ROW];
+    __shared__ cpx<T> lds[3 * N * ROW];
     const int pi = int(blockIdx.x % unsigned(nprods));
     const int slot = int(blockIdx.x / unsigned(nprods));
     const int piece = int(threadIdx.x) / 32, c = int(threadIdx.x) % 32;
     const bool act = c < N;
     const DevProd pr = prods[pi];
-    const cpx<float> *F1 = FT + (int64_t(slot) * nsrc + item_src[pr.i1]) * G3::VOL;
-    const cpx<float> *F2 = FT + (int64_t(slot) * nsrc + item_src[pr.i2]) * G3::VOL;
-    const cpx<float> *v1 = V + (int64_t(pi) * 6 + piece) * G3::N2 + c;     // [product][item][piece][a][lane]
-    const cpx<float> *v2 = v1 + 3 * G3::N2;
+    const cpx<T> *F1 = FT + (int64_t(slot) * nsrc + item_src[pr.i1]) * G3::VOL;
+    const cpx<T> *F2 = FT + (int64_t(slot) * nsrc + item_src[pr.i2]) * G3::VOL;
+    const cpx<T> *v1 = V + (int64_t(pi) * 6 + piece) * G3::N2 + c;     // [product][item][piece][a][lane]
+    const cpx<T> *v2 = v1 + 3 * G3::N2;
     // element a of this lane's column: flat index base + a * stride of the volume
     const int base = piece == 0 ? H * G3::N2 + c : (piece == 1 ? c * G3::N2 + H : H * N + c);
     const int stride = piece == 2 ? G3::N2 : N;
-    cpx<float> v[N];
+    cpx<T> v[N];
 #pragma unroll
     for (int a = 0; a < N; ++a) {
         if (!act) { v[a] = {0, 0}; continue; }
@@ -944,16 +1038,16 @@ __global__ __launch_bounds__(96, 4) void k_nyq(const cpx<float> *__restrict__ FT
         }
     }
     fq::LFFT<N, 1>::run(v);                                // over a, lane c
-    cpx<float> *L = lds + piece * N * ROW;
+    cpx<T> *L = lds + piece * N * ROW;
     if (act) {
 #pragma unroll
         for (int a = 0; a < N; ++a) L[a * ROW + c] = v[a];
     }
     __syncthreads();
 #pragma unroll
-    for (int b = 0; b < N; ++b) v[b] = act ? L[c * ROW + b] : cpx<float>{0, 0};
+    for (int b = 0; b < N; ++b) v[b] = act ? L[c * ROW + b] : cpx<T>{0, 0};
     fq::LFFT<N, 1>::run(v);                                // over the lane coordinate
-    cpx<float> *out = NY + ((int64_t(slot) * nprods + pi) * 3 + piece) * G3::N2;
+    cpx<T> *out = NY + ((int64_t(slot) * nprods + pi) * 3 + piece) * G3::N2;
     if (act) {
 #pragma unroll
         for (int j = 0; j < N; ++j) out[j * N + c] = v[j];
@@ -1423,10 +1517,10 @@ struct fq_handle {
     bool fused = true;           // k_prod path (else k_inv_z + k_plane)
     bool two_lane = false;       // FP32 24^3/32^3: two-lane k_prod instead of one-lane k_prod1
     bool pack = false;           // k_prod1 with pair packing (k_nyq corrections in NY)
-    void *NY = nullptr;          // [chunk][products][3][N^2] cpx<float> Nyquist-plane corrections
-    cpx<float> *nyv = nullptr;   // [products][2][3][N][N] k_nyq weights
+    void *NY = nullptr;          // [chunk][products][3][N^2] cpx<T> Nyquist-plane corrections
+    void *nyv = nullptr;         // [products][2][3][N][N] cpx<T> k_nyq weights
     bool pack_conj = false;      // every product: one source, conjugate phases (zstage_conj)
-    cpx<float> *zq = nullptr;    // [products][4][3 + 3 N/4] zstage_conj coefficients
+    void *zq = nullptr;          // [products][4][3 + 3 N/4] cpx<T> zstage_conj coefficients
     cpx<float> *F32 = nullptr;   // FP32 copy of F (mixed, fused)
     // scratch
     cpx<double> *F = nullptr;
@@ -1489,16 +1583,25 @@ template <typename T> int upload_phases(fq_handle *h) {
 //   V = i^t e^{i theta.l0(I,J,K)} (1 - e^{i N theta.nyq(I,J,K)}) / 2,
 // with (I,J,K) the piece's coordinates (k_nyq) and zero where the piece
 // excludes the point; nyq = the components at the unpaired index N/2.
-int upload_nyq(fq_handle *h) {
+template <typename T> int upload_nyq(fq_handle *h) {
     const fq::Plan &P = h->P;
     const int N = P.N, H = N / 2;
     const size_t N2 = size_t(N) * N;
-    std::vector<cpx<float>> v(P.prods.size() * 6 * N2);
-    bool shared = true;
+    bool conj = true;
+    for (auto &pr : P.prods) {
+        conj = conj && P.items[pr.i1].src == P.items[pr.i2].src;
+        for (int d = 0; d < 3; ++d) conj = conj && P.items[pr.i2].th[d] == -P.items[pr.i1].th[d];
+    }
+    h->pack_conj = conj;
+    // the FP64 (two-lane) product kernel packs conjugate pairs only
+    if (sizeof(T) == 8 && !conj) {
+        h->pack = false;
+        return FQ_SUCCESS;
+    }
+    std::vector<cpx<T>> v(P.prods.size() * 6 * N2);
     auto l0 = [&](int k) { return double(k < H ? k : k - N); };
     for (size_t p = 0; p < P.prods.size(); ++p) {
         const int items[2] = {P.prods[p].i1, P.prods[p].i2};
-        shared = shared && P.items[items[0]].src == P.items[items[1]].src;
         for (int t = 0; t < 2; ++t) {
             const double *th = P.items[items[t]].th;
             for (int q = 0; q < 3; ++q)
@@ -1516,51 +1619,52 @@ int upload_nyq(fq_handle *h) {
                             im = std::cos(ang) * hi + std::sin(ang) * hr;
                             if (t == 1) { const double r = re; re = -im; im = r; }
                         }
-                        v[((p * 2 + t) * 3 + q) * N2 + size_t(a) * N + c] = {float(re), float(im)};
+                        v[((p * 2 + t) * 3 + q) * N2 + size_t(a) * N + c] = {T(re), T(im)};
                     }
         }
     }
-    bool conj = shared;
-    for (auto &pr : P.prods)
-        for (int d = 0; d < 3; ++d) conj = conj && P.items[pr.i2].th[d] == -P.items[pr.i1].th[d];
-    h->pack_conj = conj;
     // conjugate pairs read one source: k_nyq reads the sum V_1 + V_2 once
     if (conj)
         for (size_t p = 0; p < P.prods.size(); ++p)
             for (size_t e = 0; e < 3 * N2; ++e) {
-                cpx<float> &a = v[p * 6 * N2 + e];
-                const cpx<float> b = v[(p * 6 + 3) * N2 + e];
+                cpx<T> &a = v[p * 6 * N2 + e];
+                const cpx<T> b = v[(p * 6 + 3) * N2 + e];
                 a = {a.x + b.x, a.y + b.y};
             }
-    int rc = upload(&h->nyv, v);
+    cpx<T> *dv = nullptr;
+    int rc = upload(&dv, v);
+    h->nyv = dv;
     if (rc != FQ_SUCCESS || !conj) return rc;
     // zstage_conj coefficients of item 1 of every product, per vz-group G:
     // beta_b = e^{i th S b'} (b' = b, or b - 4 for b >= 2) times the sign of
     // w_4^{bG} (its factor i, for odd bG, is applied in the kernel);
     // alpha_a = e^{i th a}; the twiddle w_N^{aG}
     const int S = N / 4, ZQ = 3 + 3 * S;
-    std::vector<cpx<float>> zq(P.prods.size() * 4 * ZQ);
+    std::vector<cpx<T>> zq(P.prods.size() * 4 * ZQ);
     for (size_t p = 0; p < P.prods.size(); ++p) {
         const double th = P.items[P.prods[p].i1].th[2];
         for (int G = 0; G < 4; ++G) {
-            cpx<float> *z = zq.data() + (p * 4 + G) * ZQ;
+            cpx<T> *z = zq.data() + (p * 4 + G) * ZQ;
             for (int b = 1; b < 4; ++b) {
                 const int q4 = (b * G) % 4;               // w_4^{bG} = i^q4
                 const double sg = (q4 == 0 || q4 == 1) ? 1. : -1.;
                 const double bp = b < 2 ? b : b - 4;
-                z[b - 1] = {float(sg * std::cos(th * S * bp)), float(sg * std::sin(th * S * bp))};
+                z[b - 1] = {T(sg * std::cos(th * S * bp)), T(sg * std::sin(th * S * bp))};
             }
             for (int a = 0; a < S; ++a) {
                 const double ca = std::cos(th * a), sa = std::sin(th * a);
-                z[3 + a] = {float(ca + sa), float(sa - ca)};            // (1 - i) alpha_a
-                z[3 + S + a] = {float(ca - sa), float(ca + sa)};        // (1 + i) alpha_a
+                z[3 + a] = {T(ca + sa), T(sa - ca)};                    // (1 - i) alpha_a
+                z[3 + S + a] = {T(ca - sa), T(ca + sa)};                // (1 + i) alpha_a
                 const double tw = 2 * M_PI * double((a * G) % N) / N;
                 const double cr = std::cos(tw), ci = std::sin(tw);
-                z[3 + 2 * S + a] = {float(cr - ci), float(cr + ci)};    // (1 + i) w_N^{aG}
+                z[3 + 2 * S + a] = {T(cr - ci), T(cr + ci)};            // (1 + i) w_N^{aG}
             }
         }
     }
-    return upload(&h->zq, zq);
+    cpx<T> *dz = nullptr;
+    rc = upload(&dz, zq);
+    h->zq = dz;
+    return rc;
 }
 
 int build_tables(fq_handle *h) {
@@ -1723,10 +1827,10 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
                 cpx<T> *NY = reinterpret_cast<cpx<T> *>(h->NY);
                 const bool cj = h->pack_conj;
                 if (h->pack) {
-                    auto knyq = cj ? k_nyq<N, true> : k_nyq<N, false>;
+                    auto knyq = cj ? k_nyq<N, T, true> : k_nyq<N, T, false>;
                     pre();
-                    hipLaunchKernelGGL(knyq, dim3(unsigned(int64_t(D) * nprods)), dim3(96), 0, st, FT, NY, h->nyv,
-                                       h->prods, h->item_src, nprods, nsrc);
+                    hipLaunchKernelGGL(knyq, dim3(unsigned(int64_t(D) * nprods)), dim3(96), 0, st, FT, NY,
+                                       reinterpret_cast<const cpx<T> *>(h->nyv), h->prods, h->item_src, nprods, nsrc);
                     post(8);
                 }
                 using KP = decltype(&k_prod1<N, T, true, true, false>);
@@ -1746,10 +1850,26 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
             // with its accumulator live across the item stages the compiler
             // keeps the FP64 kernel within 256 VGPRs without spills
             const bool m2 = multi || sizeof(T) == 8;
-            auto kern = m2 ? k_prod<N, T, true> : k_prod<N, T, false>;
+            const int nprods = int(P.prods.size());
+            cpx<T> *NY = reinterpret_cast<cpx<T> *>(h->NY);
+            const cpx<T> *zq = reinterpret_cast<const cpx<T> *>(h->zq);
+            using KP = decltype(&k_prod<N, T, true, false>);
+            KP kern = m2 ? k_prod<N, T, true, false> : k_prod<N, T, false, false>;
+            if constexpr ((N == 24 || N == 32) && sizeof(T) == 8) {
+                if (h->pack) {
+                    pre();
+                    hipLaunchKernelGGL((k_nyq<N, T, true>), dim3(unsigned(int64_t(D) * nprods)), dim3(96), 0, st, FT,
+                                       NY, reinterpret_cast<const cpx<T> *>(h->nyv), h->prods, h->item_src, nprods,
+                                       nsrc);
+                    post(8);
+                    // packed: one item stage per product, so the single-product
+                    // variant stays within 256 VGPRs
+                    kern = multi ? k_prod<N, T, true, true> : k_prod<N, T, false, true>;
+                }
+            }
             pre();
             hipLaunchKernelGGL(kern, grid, dim3(ProdGeo<N>::THREADS), 0, st, FT, Pg, ph, zc, h->prods, h->g_p0,
-                               h->g_np, h->item_src, ngroups, nsrc);
+                               h->g_np, h->item_src, ngroups, nsrc, NY, nprods, zq);
             post(7);
         }
     } else {
@@ -1907,7 +2027,7 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     h->flags = prm->flags;
     h->fused = !(prm->flags & FQ_FLAG_UNFUSED);
     h->two_lane = (prm->flags & FQ_FLAG_TWO_LANE) != 0;
-    h->pack = h->fused && !h->two_lane && h->precision == FQ_PREC_MIXED && (prm->Nv == 24 || prm->Nv == 32) &&
+    h->pack = h->fused && !(h->two_lane && h->precision == FQ_PREC_MIXED) && (prm->Nv == 24 || prm->Nv == 32) &&
               !(prm->flags & FQ_FLAG_NO_PACK);
     if (hipGetDevice(&h->device) != hipSuccess) {
         delete h;
@@ -1939,7 +2059,8 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     chk(upload(&h->t_ln, tln));
     if (rc == FQ_SUCCESS) rc = build_tables(h);
     if (rc == FQ_SUCCESS) rc = build_projection(h);
-    if (rc == FQ_SUCCESS && h->pack) rc = upload_nyq(h);
+    if (rc == FQ_SUCCESS && h->pack)
+        rc = h->precision == FQ_PREC_MIXED ? upload_nyq<float>(h) : upload_nyq<double>(h);
     // scratch: F (FP64, nsrc) + max(Z (items, T), T0 (nsrc, FP64), Y (2 ntgt, FP64)) + P (groups, T)
     const size_t tsz = h->precision == FQ_PREC_MIXED ? sizeof(cpx<float>) : sizeof(cpx<double>);
     const size_t per_dof_F = size_t(P.nsrc) * n * sizeof(cpx<double>);
@@ -1950,7 +2071,7 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     const size_t per_dof_P = size_t(P.ngroups) * n * tsz;
     const size_t per_dof_F32 = h->precision == FQ_PREC_MIXED ? size_t(P.nsrc) * n * sizeof(cpx<float>) : 0;
     // Nyquist-plane corrections of the pair-packed product kernel
-    const size_t per_dof_NY = h->pack ? P.prods.size() * 3 * size_t(N) * N * sizeof(cpx<float>) : 0;
+    const size_t per_dof_NY = h->pack ? P.prods.size() * 3 * size_t(N) * N * tsz : 0;
     const size_t per_dof = per_dof_F + per_dof_Z + per_dof_P + per_dof_F32 + per_dof_NY;
     int64_t chunk = prm->max_chunk;
     if (chunk <= 0) {
