@@ -986,9 +986,15 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
     for (int i = 0; i < N; ++i) acc[i] = act ? L1[i * PG::ROW + c] : cpx<T>{0, 0};   // column J' = c over vx
     fq::LFFT<N, -1>::run(acc);                            // -> I'
     cpx<T> *dst = P + (int64_t(slot) * ngroups + g) * G3::VOL + int64_t(vz) * G3::N2;
-    if (act) {
+    // 16-byte stores: lanes c and c ^ 1 swap one value per row pair, the
+    // even lane then writes row 2m and the odd lane row 2m + 1 of columns
+    // (c & ~1, c | 1)
+    const bool odd = c & 1;
 #pragma unroll
-        for (int i = 0; i < N; ++i) dst[i * N + c] = acc[i];
+    for (int m = 0; m < N / 2; ++m) {
+        const cpx<T> recv = fq::swap_adj(odd ? acc[2 * m] : acc[2 * m + 1]);
+        const cpx2f w = odd ? cpx2f{recv, acc[2 * m + 1]} : cpx2f{acc[2 * m], recv};
+        if (act) *reinterpret_cast<cpx2f *>(dst + (2 * m + (odd ? 1 : 0)) * N + (c & ~1)) = w;
     }
 }
 
