@@ -1006,39 +1006,71 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
 //   piece 1 (J = N/2, K != N/2):       rows a = I, lanes = K   -> NY1[vz][vx]
 //   piece 2 (I = N/2, J, K != N/2):    rows a = K, lanes = J   -> NY2[vy][vz]
 // Each lane gathers its column of A_1 + i A_2 (A_t = T_t (1 - e^{i N
-// theta_t.nyq}) / 2 on the piece) as F_s1 V_1 + F_s2 V_2 with the per-product
-// weights V_1 = phi_1 (1 - e^{..}) / 2 and V_2 = i phi_2 (1 - e^{..}) / 2
-// (zero off the piece), and the piece's 2D DFT+ runs as a column transform,
-// an LDS transpose and a row transform.  SH (conjugate pairs): both items
-// read one source and V_1 holds V_1 + V_2.
+// theta_t.nyq}) / 2 on the piece), and the piece's 2D DFT+ runs as a column
+// transform, an LDS transpose and a row transform.  General pairs weight the
+// two sources with the per-product table V_1 = phi_1 (1 - e^{..}) / 2,
+// V_2 = i phi_2 (1 - e^{..}) / 2 (zero off the piece); conjugate pairs (SH)
+// compute a real weight per point from item 1's phase table.
 // ---------------------------------------------------------------------------
 template <int N, typename T, bool SH>
 __global__ __launch_bounds__(96, (sizeof(T) == 4 ? 4 : 2)) void k_nyq(const cpx<T> *__restrict__ FT, cpx<T> *__restrict__ NY,
-                                               const cpx<T> *__restrict__ V, const DevProd *__restrict__ prods,
+                                               const cpx<T> *__restrict__ V, const cpx<T> *__restrict__ ph,
+                                               const DevProd *__restrict__ prods,
                                                const int *__restrict__ item_src, int nprods, int nsrc) {
     using G3 = Geo<N>;
     constexpr int H = N / 2, ROW = N + 1;
-    __shared__ cpx<T> lds[3 * N * ROW];
+    __shared__ cpx<T> lds[3 * N * ROW + (SH ? 2 * N : 0)];
     const int pi = int(blockIdx.x % unsigned(nprods));
     const int slot = int(blockIdx.x / unsigned(nprods));
     const int piece = int(threadIdx.x) / 32, c = int(threadIdx.x) % 32;
     const bool act = c < N;
     const DevProd pr = prods[pi];
     const cpx<T> *F1 = FT + (int64_t(slot) * nsrc + item_src[pr.i1]) * G3::VOL;
-    const cpx<T> *F2 = FT + (int64_t(slot) * nsrc + item_src[pr.i2]) * G3::VOL;
-    const cpx<T> *v1 = V + (int64_t(pi) * 6 + piece) * G3::N2 + c;     // [product][item][piece][a][lane]
-    const cpx<T> *v2 = v1 + 3 * G3::N2;
     // element a of this lane's column: flat index base + a * stride of the volume
     const int base = piece == 0 ? H * G3::N2 + c : (piece == 1 ? c * G3::N2 + H : H * N + c);
     const int stride = piece == 2 ? G3::N2 : N;
     cpx<T> v[N];
+    if constexpr (SH) {
+        // conjugate pair: with psi = phi_1 on the piece and h = (1 - e^{i N
+        // theta_1.nyq}) / 2, A_1 + i A_2 = F (psi h + i conj(psi h))
+        // = (1 + i) F Re((1 - i) psi h): a real weight per point, and the
+        // factor (1 + i) is applied to the transform.  psi = phi_row(a) q
+        // with q = phi_lane(c) phi_fixed(N/2) per lane; phi_row staged in LDS.
+        const cpx<T> *p1 = ph + int64_t(pr.i1) * 3 * N;
+        cpx<T> *prow = lds + 3 * N * ROW;                  // [0, N): phi_x, [N, 2N): phi_z
+        if (threadIdx.x < 2 * N) prow[threadIdx.x] = p1[threadIdx.x < N ? threadIdx.x : threadIdx.x + N];
+        // dims (x = 0, y = 1, z = 2) of the row, lane and fixed coordinate
+        const int drow = piece == 2 ? 2 : 0, dlane = piece == 1 ? 2 : 1, dfix = piece == 0 ? 2 : (piece == 1 ? 1 : 0);
+        auto nq = [&](int d) { const cpx<T> u = {p1[d * N + H].x, -p1[d * N + H].y}; return fq::cmul(u, u); };
+        const cpx<T> q = fq::cmul(p1[dlane * N + (act ? c : 0)], p1[dfix * N + H]);
+        cpx<T> w0 = nq(dfix);
+        if (c == H) w0 = fq::cmul(w0, nq(dlane));
+        const cpx<T> wH = fq::cmul(w0, nq(drow));
+        auto qh = [&](cpx<T> w) {                       // (1 - i) q (1 - w) / 2
+            const cpx<T> hq = fq::cmul(q, cpx<T>{T(0.5) * (T(1) - w.x), T(-0.5) * w.y});
+            return cpx<T>{hq.x + hq.y, hq.y - hq.x};
+        };
+        const bool drop = !act || (piece == 1 && c == H) || (piece == 2 && c == H);
+        const cpx<T> Q0 = drop ? cpx<T>{0, 0} : qh(w0);
+        const cpx<T> QH = (drop || piece == 2) ? cpx<T>{0, 0} : qh(wH);
+        __syncthreads();
+        const cpx<T> *pr0 = prow + (drow == 2 ? N : 0);
 #pragma unroll
-    for (int a = 0; a < N; ++a) {
-        if (!act) { v[a] = {0, 0}; continue; }
-        const int off = base + a * stride;
-        if constexpr (SH) {
-            v[a] = fq::cmul(F1[off], v1[a * N]);             // V_1 holds V_1 + V_2
-        } else {
+        for (int a = 0; a < N; ++a) {
+            const cpx<T> Q = a == H ? QH : Q0;
+            const cpx<T> r = pr0[a];
+            const T sw = fq::fmadd(r.x, Q.x, -(r.y * Q.y));
+            const cpx<T> x = act ? F1[base + a * stride] : cpx<T>{0, 0};
+            v[a] = {x.x * sw, x.y * sw};
+        }
+    } else {
+        const cpx<T> *F2 = FT + (int64_t(slot) * nsrc + item_src[pr.i2]) * G3::VOL;
+        const cpx<T> *v1 = V + (int64_t(pi) * 6 + piece) * G3::N2 + c;     // [product][item][piece][a][lane]
+        const cpx<T> *v2 = v1 + 3 * G3::N2;
+#pragma unroll
+        for (int a = 0; a < N; ++a) {
+            if (!act) { v[a] = {0, 0}; continue; }
+            const int off = base + a * stride;
             v[a] = fq::cmul(F1[off], v1[a * N]);
             fq::cmac(v[a], F2[off], v2[a * N]);
         }
@@ -1053,10 +1085,25 @@ __global__ __launch_bounds__(96, (sizeof(T) == 4 ? 4 : 2)) void k_nyq(const cpx<
 #pragma unroll
     for (int b = 0; b < N; ++b) v[b] = act ? L[c * ROW + b] : cpx<T>{0, 0};
     fq::LFFT<N, 1>::run(v);                                // over the lane coordinate
-    cpx<T> *out = NY + ((int64_t(slot) * nprods + pi) * 3 + piece) * G3::N2;
-    if (act) {
+    if constexpr (SH) {
 #pragma unroll
-        for (int j = 0; j < N; ++j) out[j * N + c] = v[j];
+        for (int j = 0; j < N; ++j) v[j] = {v[j].x - v[j].y, v[j].x + v[j].y};   // (1 + i) X
+    }
+    cpx<T> *out = NY + ((int64_t(slot) * nprods + pi) * 3 + piece) * G3::N2;
+    if constexpr (sizeof(T) == 4) {
+        // 16-byte stores after a swap with lane c ^ 1 (as in k_prod1)
+        const bool odd = c & 1;
+#pragma unroll
+        for (int m = 0; m < N / 2; ++m) {
+            const cpx<T> recv = fq::swap_adj(odd ? v[2 * m] : v[2 * m + 1]);
+            const cpx2f w = odd ? cpx2f{recv, v[2 * m + 1]} : cpx2f{v[2 * m], recv};
+            if (act) *reinterpret_cast<cpx2f *>(out + (2 * m + (odd ? 1 : 0)) * N + (c & ~1)) = w;
+        }
+    } else {
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < N; ++j) out[j * N + c] = v[j];
+        }
     }
 }
 
@@ -1584,7 +1631,8 @@ template <typename T> int upload_phases(fq_handle *h) {
     return rc;
 }
 
-// Weights of k_nyq for the pair-packed product kernel: for product p and
+// Weights of k_nyq for general pairs of the pair-packed product kernel (for
+// conjugate pairs k_nyq computes them from the phase tables): for product p and
 // item t (t = 0: i1, t = 1: i2) on plane piece q at (row a, lane c),
 //   V = i^t e^{i theta.l0(I,J,K)} (1 - e^{i N theta.nyq(I,J,K)}) / 2,
 // with (I,J,K) the piece's coordinates (k_nyq) and zero where the piece
@@ -1604,6 +1652,9 @@ template <typename T> int upload_nyq(fq_handle *h) {
         h->pack = false;
         return FQ_SUCCESS;
     }
+    int rc = FQ_SUCCESS;
+    if (!conj) {
+    // general pairs: k_nyq reads these weights (conjugate pairs compute theirs)
     std::vector<cpx<T>> v(P.prods.size() * 6 * N2);
     auto l0 = [&](int k) { return double(k < H ? k : k - N); };
     for (size_t p = 0; p < P.prods.size(); ++p) {
@@ -1629,17 +1680,10 @@ template <typename T> int upload_nyq(fq_handle *h) {
                     }
         }
     }
-    // conjugate pairs read one source: k_nyq reads the sum V_1 + V_2 once
-    if (conj)
-        for (size_t p = 0; p < P.prods.size(); ++p)
-            for (size_t e = 0; e < 3 * N2; ++e) {
-                cpx<T> &a = v[p * 6 * N2 + e];
-                const cpx<T> b = v[(p * 6 + 3) * N2 + e];
-                a = {a.x + b.x, a.y + b.y};
-            }
     cpx<T> *dv = nullptr;
-    int rc = upload(&dv, v);
+    rc = upload(&dv, v);
     h->nyv = dv;
+    }
     if (rc != FQ_SUCCESS || !conj) return rc;
     // zstage_conj coefficients of item 1 of every product, per vz-group G:
     // beta_b = e^{i th S b'} (b' = b, or b - 4 for b >= 2) times the sign of
@@ -1836,7 +1880,8 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
                     auto knyq = cj ? k_nyq<N, T, true> : k_nyq<N, T, false>;
                     pre();
                     hipLaunchKernelGGL(knyq, dim3(unsigned(int64_t(D) * nprods)), dim3(96), 0, st, FT, NY,
-                                       reinterpret_cast<const cpx<T> *>(h->nyv), h->prods, h->item_src, nprods, nsrc);
+                                       reinterpret_cast<const cpx<T> *>(h->nyv), ph, h->prods, h->item_src, nprods,
+                                       nsrc);
                     post(8);
                 }
                 using KP = decltype(&k_prod1<N, T, true, true, false>);
@@ -1865,8 +1910,7 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
                 if (h->pack) {
                     pre();
                     hipLaunchKernelGGL((k_nyq<N, T, true>), dim3(unsigned(int64_t(D) * nprods)), dim3(96), 0, st, FT,
-                                       NY, reinterpret_cast<const cpx<T> *>(h->nyv), h->prods, h->item_src, nprods,
-                                       nsrc);
+                                       NY, nullptr, ph, h->prods, h->item_src, nprods, nsrc);
                     post(8);
                     // packed: one item stage per product, so the single-product
                     // variant stays within 256 VGPRs
