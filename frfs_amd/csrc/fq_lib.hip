@@ -220,9 +220,12 @@ __global__ __launch_bounds__(256) void k_fwd_z(AosoaPtrs ptr, Dof dof, cpx<doubl
 
 // k_fwd_xy: plane (slot, src, K'):  DFT- along x then y, divide by n; with
 // F32 non-null it also writes the FP32 copy the mixed-precision path reads.
+// FJ64 / FJ32 (pair packing) receive the plane J = N/2 of the spectrum as
+// [slot][src][I][K], so k_nyq reads that strided slice contiguously.
 template <int N>
 __global__ __launch_bounds__(kPlaneThreads) void k_fwd_xy(const cpx<double> *T0, cpx<double> *F,
-                                                           cpx<float> *F32, int64_t nunits) {
+                                                           cpx<float> *F32, cpx<double> *FJ64, cpx<float> *FJ32,
+                                                           int64_t nunits) {
     using G = Geo<N>;
     constexpr int E = G::E;
     constexpr int UPB = kPlaneThreads / G::LPP;
@@ -263,6 +266,11 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_xy(const cpx<double> *T0,
             const cpx<double> x = L[(half * E + e) * G::LDSROW + c];
             dst[(half * E + e) * N + c] = x;
             if (F32) F32[unit * G::N2 + (half * E + e) * N + c] = cvt<float>(x);   // mixed precision copy
+            if (c == N / 2) {
+                const int64_t fj = (unit / N) * G::N2 + int64_t(half * E + e) * N + unit % N;
+                if (FJ64) FJ64[fj] = x;
+                if (FJ32) FJ32[fj] = cvt<float>(x);
+            }
         }
     }
 }
@@ -1013,7 +1021,8 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
 // compute a real weight per point from item 1's phase table.
 // ---------------------------------------------------------------------------
 template <int N, typename T, bool SH>
-__global__ __launch_bounds__(96, (sizeof(T) == 4 ? 4 : 2)) void k_nyq(const cpx<T> *__restrict__ FT, cpx<T> *__restrict__ NY,
+__global__ __launch_bounds__(96, (sizeof(T) == 4 ? 4 : 2)) void k_nyq(const cpx<T> *__restrict__ FT, const cpx<T> *__restrict__ FJ,
+                                               cpx<T> *__restrict__ NY,
                                                const cpx<T> *__restrict__ V, const cpx<T> *__restrict__ ph,
                                                const DevProd *__restrict__ prods,
                                                const int *__restrict__ item_src, int nprods, int nsrc) {
@@ -1025,9 +1034,14 @@ __global__ __launch_bounds__(96, (sizeof(T) == 4 ? 4 : 2)) void k_nyq(const cpx<
     const int piece = int(threadIdx.x) / 32, c = int(threadIdx.x) % 32;
     const bool act = c < N;
     const DevProd pr = prods[pi];
-    const cpx<T> *F1 = FT + (int64_t(slot) * nsrc + item_src[pr.i1]) * G3::VOL;
-    // element a of this lane's column: flat index base + a * stride of the volume
-    const int base = piece == 0 ? H * G3::N2 + c : (piece == 1 ? c * G3::N2 + H : H * N + c);
+    // element a of this lane's column: base + a * stride in the source volume
+    // (pieces 0, 2) or in its J = N/2 slice [I][K] (piece 1)
+    auto col = [&](int it) {
+        const int64_t s = int64_t(slot) * nsrc + item_src[it];
+        return piece == 1 ? FJ + s * G3::N2 : FT + s * G3::VOL;
+    };
+    const cpx<T> *F1 = col(pr.i1);
+    const int base = piece == 0 ? H * G3::N2 + c : (piece == 1 ? c : H * N + c);
     const int stride = piece == 2 ? G3::N2 : N;
     cpx<T> v[N];
     if constexpr (SH) {
@@ -1064,7 +1078,7 @@ __global__ __launch_bounds__(96, (sizeof(T) == 4 ? 4 : 2)) void k_nyq(const cpx<
             v[a] = {x.x * sw, x.y * sw};
         }
     } else {
-        const cpx<T> *F2 = FT + (int64_t(slot) * nsrc + item_src[pr.i2]) * G3::VOL;
+        const cpx<T> *F2 = col(pr.i2);
         const cpx<T> *v1 = V + (int64_t(pi) * 6 + piece) * G3::N2 + c;     // [product][item][piece][a][lane]
         const cpx<T> *v2 = v1 + 3 * G3::N2;
 #pragma unroll
@@ -1571,6 +1585,7 @@ struct fq_handle {
     bool two_lane = false;       // FP32 24^3/32^3: two-lane k_prod instead of one-lane k_prod1
     bool pack = false;           // k_prod1 with pair packing (k_nyq corrections in NY)
     void *NY = nullptr;          // [chunk][products][3][N^2] cpx<T> Nyquist-plane corrections
+    void *FJ = nullptr;          // [chunk][sources][N^2] cpx<T> spectrum slice J = N/2 for k_nyq
     void *nyv = nullptr;         // [products][2][3][N][N] cpx<T> k_nyq weights
     bool pack_conj = false;      // every product: one source, conjugate phases (zstage_conj)
     void *zq = nullptr;          // [products][4][3 + 3 N/4] cpx<T> zstage_conj coefficients
@@ -1858,7 +1873,9 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     // the fused FP32 path reads an FP32 copy of F, written by k_fwd_xy itself
     const bool want32 = h->fused && std::is_same<T, float>::value;
     pre();
-    hipLaunchKernelGGL(k_fwd_xy<N>, dim3(pblocks(u)), pt, 0, st, T0, h->F, want32 ? h->F32 : nullptr, u);
+    cpx<double> *FJ64 = h->pack && sizeof(T) == 8 ? reinterpret_cast<cpx<double> *>(h->FJ) : nullptr;
+    cpx<float> *FJ32 = h->pack && sizeof(T) == 4 ? reinterpret_cast<cpx<float> *>(h->FJ) : nullptr;
+    hipLaunchKernelGGL(k_fwd_xy<N>, dim3(pblocks(u)), pt, 0, st, T0, h->F, want32 ? h->F32 : nullptr, FJ64, FJ32, u);
     post(1);
     if (h->fused) {
         const cpx<T> *FT = reinterpret_cast<const cpx<T> *>(want32 ? static_cast<void *>(h->F32)
@@ -1879,7 +1896,8 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
                 if (h->pack) {
                     auto knyq = cj ? k_nyq<N, T, true> : k_nyq<N, T, false>;
                     pre();
-                    hipLaunchKernelGGL(knyq, dim3(unsigned(int64_t(D) * nprods)), dim3(96), 0, st, FT, NY,
+                    hipLaunchKernelGGL(knyq, dim3(unsigned(int64_t(D) * nprods)), dim3(96), 0, st, FT,
+                                       reinterpret_cast<const cpx<T> *>(h->FJ), NY,
                                        reinterpret_cast<const cpx<T> *>(h->nyv), ph, h->prods, h->item_src, nprods,
                                        nsrc);
                     post(8);
@@ -1910,7 +1928,8 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
                 if (h->pack) {
                     pre();
                     hipLaunchKernelGGL((k_nyq<N, T, true>), dim3(unsigned(int64_t(D) * nprods)), dim3(96), 0, st, FT,
-                                       NY, nullptr, ph, h->prods, h->item_src, nprods, nsrc);
+                                       reinterpret_cast<const cpx<T> *>(h->FJ), NY, nullptr, ph, h->prods,
+                                       h->item_src, nprods, nsrc);
                     post(8);
                     // packed: one item stage per product, so the single-product
                     // variant stays within 256 VGPRs
@@ -2121,7 +2140,7 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     const size_t per_dof_P = size_t(P.ngroups) * n * tsz;
     const size_t per_dof_F32 = h->precision == FQ_PREC_MIXED ? size_t(P.nsrc) * n * sizeof(cpx<float>) : 0;
     // Nyquist-plane corrections of the pair-packed product kernel
-    const size_t per_dof_NY = h->pack ? P.prods.size() * 3 * size_t(N) * N * tsz : 0;
+    const size_t per_dof_NY = h->pack ? (P.prods.size() * 3 + P.nsrc) * size_t(N) * N * tsz : 0;
     const size_t per_dof = per_dof_F + per_dof_Z + per_dof_P + per_dof_F32 + per_dof_NY;
     int64_t chunk = prm->max_chunk;
     if (chunk <= 0) {
@@ -2138,7 +2157,10 @@ int fq_create(const fq_params *prm, fq_handle **out) {
         if (h->precision == FQ_PREC_MIXED)
             e4 = hipMalloc(&h->F32, size_t(P.nsrc) * n * sizeof(cpx<float>) * chunk);
         hipError_t e5 = hipSuccess;
-        if (h->pack) e5 = hipMalloc(&h->NY, per_dof_NY * chunk);
+        if (h->pack) {
+            e5 = hipMalloc(&h->NY, P.prods.size() * 3 * size_t(N) * N * tsz * chunk);
+            if (e5 == hipSuccess) e5 = hipMalloc(&h->FJ, size_t(P.nsrc) * N * N * tsz * chunk);
+        }
         if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess || e5 != hipSuccess)
             rc = fail(FQ_ERR_ALLOC, "hipMalloc scratch failed (reduce max_chunk)");
         h->scratch = per_dof * chunk;
@@ -2197,7 +2219,7 @@ int fq_destroy(fq_handle *h) {
     for (void *p : {h->ph, (void *)h->item_src, (void *)h->prods, (void *)h->g_p0, (void *)h->g_np,
                     (void *)h->c_group, (void *)h->t_c0, (void *)h->t_cn, (void *)h->l_src, (void *)h->t_l0,
                     (void *)h->t_ln, (void *)h->W, (void *)h->ginv, (void *)h->pmass, (void *)h->F, h->Z,
-                    h->Pg, h->zc, (void *)h->F32, h->NY, (void *)h->nyv, (void *)h->zq})
+                    h->Pg, h->zc, (void *)h->F32, h->NY, h->FJ, (void *)h->nyv, (void *)h->zq})
         if (p) (void)hipFree(p);
     (void)hipThreadExchangeStreamCaptureMode(&mode);
     delete h;
