@@ -536,11 +536,13 @@ __device__ __forceinline__ void zstage_pair(const cpx<T> *__restrict__ src1, con
             }
         }
         const cpx<T> px1 = ph1[I], px2 = ph2[I];
+        const cpx2f py1 = *reinterpret_cast<const cpx2f *>(ph1 + N + J0);
+        const cpx2f py2 = *reinterpret_cast<const cpx2f *>(ph2 + N + J0);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             // u = pxy1 cc1 t1 + i pxy2 cc2 t2, transformed once
-            const cpx<T> pxy1 = fq::cmul(px1, ph1[N + J0 + h]);
-            const cpx<T> q2 = fq::cmul(px2, ph2[N + J0 + h]);
+            const cpx<T> pxy1 = fq::cmul(px1, h ? py1.b : py1.a);
+            const cpx<T> q2 = fq::cmul(px2, h ? py2.b : py2.a);
             const cpx<T> pxy2 = {-q2.y, q2.x};
             cpx<T> u[S];
 #pragma unroll
@@ -624,6 +626,7 @@ __device__ __forceinline__ void zstage_conj(const cpx<T> *__restrict__ src, cons
                 for (int a = 0; a < S; ++a) x2[b][a] = ld2f(sv, vo, uint32_t(a + S * b) * PB);
             }
             const cpx<T> phx = ph1[I];
+            const cpx2f phy = *reinterpret_cast<const cpx2f *>(ph1 + N + J0);
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 cpx<T> x[4][S], u[S];
@@ -632,7 +635,7 @@ __device__ __forceinline__ void zstage_conj(const cpx<T> *__restrict__ src, cons
 #pragma unroll
                     for (int a = 0; a < S; ++a) x[b][a] = h ? x2[b][a].b : x2[b][a].a;
                 }
-                conj_pencil<S, T, ODD>(x, fq::cmul(phx, ph1[N + J0 + h]), pq, am, ap, cw, u);
+                conj_pencil<S, T, ODD>(x, fq::cmul(phx, h ? phy.b : phy.a), pq, am, ap, cw, u);
                 cpx<T> *dd = dst + I * PG::ROW + J0 + h;
 #pragma unroll
                 for (int k = 0; k < S; ++k) dd[k * PG::PLANE] = u[k];
