@@ -566,19 +566,17 @@ __device__ __forceinline__ void zstage_pair(const cpx<T> *__restrict__ src1, con
 // Q = sum_b q_b y_b the two combinations are P + iQ and P - iQ, and with
 // gamma = alpha_a pxy (item 1's phases)
 //   gamma (P + iQ) + i conj(gamma) (P - iQ)
-//       = (1 + i) [ Re((1 - i) gamma) P + Re((1 + i) gamma) Q ],
+//       = (1 + i) [ Re((1 - i) gamma) P - Im((1 - i) gamma) Q ],
 // so one pencil costs two real-coefficient sums and no per-item complex
-// products.  zq[(prod*4 + G)*(3 + 3S) + .] holds {(p_b, q_b) for b = 1..3
-// with the sign of w_4^{bG} folded in}, (1 - i) alpha_a, (1 + i) alpha_a and
-// (1 + i) w_N^{aG}.  ODD (G odd): w_4^{bG} = +-i for odd b, applied as a
+// products.  zq[(prod*4 + G)*(3 + 2S) + .] holds {(p_b, q_b) for b = 1..3
+// with the sign of w_4^{bG} folded in}, (1 - i) alpha_a and (1 + i) w_N^{aG}.  ODD (G odd): w_4^{bG} = +-i for odd b, applied as a
 // rotation of x_b.  In FP32 each lane takes two neighbouring pencils
 // (J, J + 1) per step with 16-byte loads: half the load instructions of one
 // pencil per lane, which the vector memory pipeline, not the arithmetic,
 // limits.
 template <int S, typename T, bool ODD>
 __device__ __forceinline__ void conj_pencil(const cpx<T> (&x)[4][S], cpx<T> pxy, const cpx<T> (&pq)[4],
-                                            const cpx<T> (&am)[S], const cpx<T> (&ap)[S], const cpx<T> (&cw)[S],
-                                            cpx<T> (&u)[S]) {
+                                            const cpx<T> (&am)[S], const cpx<T> (&cw)[S], cpx<T> (&u)[S]) {
 #pragma unroll
     for (int a = 0; a < S; ++a) {
         cpx<T> P = x[0][a], Q;
@@ -591,7 +589,7 @@ __device__ __forceinline__ void conj_pencil(const cpx<T> (&x)[4][S], cpx<T> pxy,
             else Q = {fq::fmadd(pq[b].y, y.x, Q.x), fq::fmadd(pq[b].y, y.y, Q.y)};
         }
         const T A = fq::fmadd(am[a].x, pxy.x, -(am[a].y * pxy.y));
-        const T B = fq::fmadd(ap[a].x, pxy.x, -(ap[a].y * pxy.y));
+        const T B = -fq::fmadd(am[a].x, pxy.y, am[a].y * pxy.x);
         const cpx<T> z = {fq::fmadd(A, P.x, B * Q.x), fq::fmadd(A, P.y, B * Q.y)};
         u[a] = fq::cmul(z, cw[a]);
     }
@@ -603,11 +601,11 @@ __device__ __forceinline__ void zstage_conj(const cpx<T> *__restrict__ src, cons
                                             const cpx<T> *__restrict__ ph1, cpx<T> *dst) {
     using PG = ProdGeo<N>;
     constexpr int S = PG::S;
-    cpx<T> pq[4], am[S], ap[S], cw[S];
+    cpx<T> pq[4], am[S], cw[S];
 #pragma unroll
     for (int b = 1; b < 4; ++b) pq[b] = zq[b - 1];
 #pragma unroll
-    for (int a = 0; a < S; ++a) { am[a] = zq[3 + a]; ap[a] = zq[3 + S + a]; cw[a] = zq[3 + 2 * S + a]; }
+    for (int a = 0; a < S; ++a) { am[a] = zq[3 + a]; cw[a] = zq[3 + S + a]; }
     constexpr uint32_t PB = uint32_t(N * N * sizeof(cpx<T>));
     const BufView sv(src, uint32_t(N) * PB);
     if constexpr (sizeof(T) == 4) {
@@ -635,7 +633,7 @@ __device__ __forceinline__ void zstage_conj(const cpx<T> *__restrict__ src, cons
 #pragma unroll
                     for (int a = 0; a < S; ++a) x[b][a] = h ? x2[b][a].b : x2[b][a].a;
                 }
-                conj_pencil<S, T, ODD>(x, fq::cmul(phx, h ? phy.b : phy.a), pq, am, ap, cw, u);
+                conj_pencil<S, T, ODD>(x, fq::cmul(phx, h ? phy.b : phy.a), pq, am, cw, u);
                 cpx<T> *dd = dst + I * PG::ROW + J0 + h;
 #pragma unroll
                 for (int k = 0; k < S; ++k) dd[k * PG::PLANE] = u[k];
@@ -656,7 +654,7 @@ __device__ __forceinline__ void zstage_conj(const cpx<T> *__restrict__ src, cons
 #pragma unroll
                 for (int a = 0; a < S; ++a) x[b][a] = sv.ld<T>(vo, uint32_t(a + S * b) * PB);
             }
-            conj_pencil<S, T, ODD>(x, fq::cmul(ph1[I], ph1[N + J]), pq, am, ap, cw, u);
+            conj_pencil<S, T, ODD>(x, fq::cmul(ph1[I], ph1[N + J]), pq, am, cw, u);
             cpx<T> *dd = dst + I * PG::ROW + J;
 #pragma unroll
             for (int k = 0; k < S; ++k) dd[k * PG::PLANE] = u[k];
@@ -738,7 +736,7 @@ __global__ __launch_bounds__(ProdGeo<N>::THREADS, (sizeof(T) == 4 ? 4 : 2)) void
             static_assert(PG::LPP == 64, "one plane per wave");
             cpx<T> ev[E];
             __syncthreads();                                   // LDS free
-            const cpx<T> *zqp = zq + (int64_t(pi) * 4 + G) * (3 + 3 * PG::S);
+            const cpx<T> *zqp = zq + (int64_t(pi) * 4 + G) * (3 + 2 * PG::S);
             if (G & 1) zstage_conj<N, T, PG::THREADS, true>(src1, zqp, ph1, lds);
             else zstage_conj<N, T, PG::THREADS, false>(src1, zqp, ph1, lds);
             __syncthreads();
@@ -912,7 +910,7 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
             cpx<T> e[N];
             __syncthreads();                               // LDS free
             if constexpr (CONJ) {
-                const cpx<T> *zqp = zq + (int64_t(pi) * 4 + G) * (3 + 3 * PG::S);
+                const cpx<T> *zqp = zq + (int64_t(pi) * 4 + G) * (3 + 2 * PG::S);
                 if (G & 1) zstage_conj<N, T, P1::THREADS, true>(src1, zqp, ph1, lds);
                 else zstage_conj<N, T, P1::THREADS, false>(src1, zqp, ph1, lds);
             } else {
@@ -1591,7 +1589,7 @@ struct fq_handle {
     void *FJ = nullptr;          // [chunk][sources][N^2] cpx<T> spectrum slice J = N/2 for k_nyq
     void *nyv = nullptr;         // [products][2][3][N][N] cpx<T> k_nyq weights
     bool pack_conj = false;      // every product: one source, conjugate phases (zstage_conj)
-    void *zq = nullptr;          // [products][4][3 + 3 N/4] cpx<T> zstage_conj coefficients
+    void *zq = nullptr;          // [products][4][3 + 2 N/4] cpx<T> zstage_conj coefficients
     cpx<float> *F32 = nullptr;   // FP32 copy of F (mixed, fused)
     // scratch
     cpx<double> *F = nullptr;
@@ -1707,7 +1705,7 @@ template <typename T> int upload_nyq(fq_handle *h) {
     // beta_b = e^{i th S b'} (b' = b, or b - 4 for b >= 2) times the sign of
     // w_4^{bG} (its factor i, for odd bG, is applied in the kernel);
     // alpha_a = e^{i th a}; the twiddle w_N^{aG}
-    const int S = N / 4, ZQ = 3 + 3 * S;
+    const int S = N / 4, ZQ = 3 + 2 * S;
     std::vector<cpx<T>> zq(P.prods.size() * 4 * ZQ);
     for (size_t p = 0; p < P.prods.size(); ++p) {
         const double th = P.items[P.prods[p].i1].th[2];
@@ -1722,10 +1720,9 @@ template <typename T> int upload_nyq(fq_handle *h) {
             for (int a = 0; a < S; ++a) {
                 const double ca = std::cos(th * a), sa = std::sin(th * a);
                 z[3 + a] = {T(ca + sa), T(sa - ca)};                    // (1 - i) alpha_a
-                z[3 + S + a] = {T(ca - sa), T(ca + sa)};                // (1 + i) alpha_a
                 const double tw = 2 * M_PI * double((a * G) % N) / N;
                 const double cr = std::cos(tw), ci = std::sin(tw);
-                z[3 + 2 * S + a] = {T(cr - ci), T(cr + ci)};            // (1 + i) w_N^{aG}
+                z[3 + S + a] = {T(cr - ci), T(cr + ci)};                // (1 + i) w_N^{aG}
             }
         }
     }
