@@ -24,10 +24,10 @@ sys.path.insert(0, ROOT)
 PEAK = {'fp64': 78.6, 'fp32': 157.3}     # MI355X vector TFLOP/s (published)
 HBM_PEAK = 8000.0                          # GB/s
 # HBM bytes per DOF of the fused product kernel at the 32^3 workload, from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (1272 DOFs per launch),
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (1272 DOFs, 2-3 launches),
 # scaled by what the same counters read for a 1 GiB device copy (FETCH x2.0,
 # WRITE x1.0); the summary with its command is profiles/k_prod_vss32_traffic.txt.
-K_PROD_TRAFFIC_PER_DOF = {'mixed': 27.44e6, 'fp64': 57.70e6}
+K_PROD_TRAFFIC_PER_DOF = {'mixed': 34.47e6, 'fp64': 84.67e6}
 
 
 def parse():

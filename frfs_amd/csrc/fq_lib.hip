@@ -3,10 +3,15 @@
 // Pipeline per batch ("chunk") of D DOFs, all volumes stored plane-major
 // [plane][I][J] (J fastest, plane = the z index K or vz):
 //   k_fwd_z    f (AoSoA, per source)  -> T0  : DFT- along z           (r2z + cuFFT fwd)
-//   k_fwd_xy   T0 -> F (FP64)                : DFT- along x, y, / n (+ FP32 copy, mixed)
-//   k_prod / k_prod1  F -> P_g (T)           : per group: the items' pruned z
+//   k_fwd_xy   T0 -> F (FP64)                : DFT- along x, y, / n (+ FP32 copy, mixed;
+//                                              + the slice J = N/2 for k_nyq)
+//   k_nyq      F -> NY                       : (pair packing) per product, the 2D
+//                                              transforms of the Nyquist planes
+//   k_prod / k_prod1  F, NY -> P_g (T)       : per group: the items' pruned z
 //                                              DFTs straight into LDS, x,y DFT+,
-//                                              products, group sum, DFT- x,y
+//                                              products, group sum, DFT- x,y; with
+//                                              pair packing both items of a product
+//                                              as one complex volume
 //     (staged alternative, FQ_FLAG_UNFUSED: k_inv_z F -> Z_i through HBM,
 //      then k_plane Z -> P_g)
 //   k_accum / k_accum_lds  P, F -> Y (FP64)  : DFT- along z, H += W . Ghat (FP64),
