@@ -785,7 +785,7 @@ __global__ __launch_bounds__(ProdGeo<N>::THREADS, (sizeof(T) == 4 ? 4 : 2)) void
                 const T px = fq::fmadd(rx, ry, uy * ux);
                 const T py = fq::fmadd(uy, ry, -(rx * ux));
                 if constexpr (MULTI) acc[e] = {fq::fmadd(cf, px, acc[e].x), fq::fmadd(cf, py, acc[e].y)};
-                else acc[e] = {cf * px, cf * py};
+                else acc[e] = {px, py};                      // coefficient applied below
             }
         } else {
             const cpx<T> *src2 = FT + (int64_t(slot) * nsrc + item_src[pr.i2]) * G3::VOL;
@@ -811,6 +811,14 @@ __global__ __launch_bounds__(ProdGeo<N>::THREADS, (sizeof(T) == 4 ? 4 : 2)) void
 #pragma unroll
                 for (int e = 0; e < E; ++e) acc[e] = fq::cscale(fq::cmul(e1[e], e2[e]), cf);
             }
+        }
+    }
+    if constexpr (PACK && !MULTI) {
+        // the single product's coefficient (1 for VHS and VSS), applied once
+        const T cf = np > 0 ? T(prods[p0].coef) : T(1);
+        if (cf != T(1)) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) acc[e] = fq::cscale(acc[e], cf);
         }
     }
     // forward transforms of the group sum: acc is row vx = c, cyclic over vy
@@ -964,7 +972,7 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
                 const T px = fq::fmadd(rx, ry, uy * ux);
                 const T py = fq::fmadd(uy, ry, -(rx * ux));
                 if constexpr (MULTI) acc[j] = {fq::fmadd(cf, px, acc[j].x), fq::fmadd(cf, py, acc[j].y)};
-                else acc[j] = {cf * px, cf * py};
+                else acc[j] = {px, py};                      // coefficient applied below
             }
         } else {
             cpx<T> e1[N], e2[N];
@@ -986,6 +994,14 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
 #pragma unroll
                 for (int e = 0; e < N; ++e) acc[e] = fq::cscale(fq::cmul(e1[e], e2[e]), cf);
             }
+        }
+    }
+    if constexpr (PACK && !MULTI) {
+        // the single product's coefficient (1 for VHS and VSS), applied once
+        const T cf = np > 0 ? T(prods[p0].coef) : T(1);
+        if (cf != T(1)) {
+#pragma unroll
+            for (int e = 0; e < N; ++e) acc[e] = fq::cscale(acc[e], cf);
         }
     }
     // forward transforms of the group sum: lane c holds row vx = c over vy
