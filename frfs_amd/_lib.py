@@ -51,6 +51,15 @@ class fq_params(C.Structure):
     ]
 
 
+SUR_ABI_VERSION = 1
+SUR_MAX_LAYERS = 8
+SUR_MAX_BATCH = 65536
+
+
+class fq_sur_shape(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ('abi_version', 'Nv', 'width', 'modes', 'layers')]
+
+
 class fq_plan_info(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         'n_items', 'n_products', 'n_groups', 'n_contribs', 'n_loss',
@@ -85,6 +94,17 @@ SIGNATURES = {
     'fq_plan_destroy': (C.c_int, [_vp]),
     'fq_plan_get_info': (C.c_int, [_vp, C.POINTER(fq_plan_info)]),
     'fq_plan_export': (C.c_int, [_vp] + [_vp] * 10),
+    # include/frfs_sur.h
+    'fq_sur_sizes': (C.c_int, [_vp, _i64, _i32, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]),
+    'fq_sur_forward': (C.c_int, [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp]),
+    'fq_sur_backward': (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp]),
+    'fq_sur_loss': (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
+    'fq_sur_adam': (C.c_int, [_i64, _vp, _vp, _vp, _vp, C.c_float, C.c_float, C.c_float, C.c_float,
+                              _i32, _vp]),
+    'fq_sur_gather': (C.c_int, [_i32, C.c_double, C.c_double, _i32, _vp, _i32, _i64, _i32, _i64, _i64,
+                                _vp, _vp, _vp, _vp]),
+    'fq_sur_finish': (C.c_int, [_i32, C.c_double, C.c_double, C.c_double, C.c_double, _vp, _vp, _vp, _vp,
+                                _vp, _i32, _i64, _i32, _i64, _i64, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libfrfs_fq.so')
 
-SOURCES = ['fq_lib.hip', 'fq_aux.hip']
+SOURCES = ['fq_lib.hip', 'fq_aux.hip', 'sur_lib.hip']
 HEADERS = ['fft.hpp', 'plan.hpp']
 
 # Packed FP32 VALU forms are disabled for every HIP translation unit.

@@ -1,0 +1,682 @@
+// Fourier-layer surrogate of the collision operator: forward, backward,
+// loss, Adam and the AoSoA gather / conservation-check / scatter kernels.
+//
+// The network (16^3 grid, width 32, 8^3 retained modes, L layers):
+//   v_0      = P g + p0                                 (lift, per point)
+//   X_l(k)   = (1/n) sum_x v_l(x) e^{-2 pi i k.x/16}   (k_sur_dft, per channel)
+//   Y_l(o,k) = sum_i X_l(i,k) R_l(i,o,k)               (k_sur_mix, per mode)
+//   s_l(x)   = Re sum_k Y_l(k) e^{+2 pi i k.x/16}       (k_sur_idft, per channel)
+//   z_l      = Wm_l v_l + b_l + s_l,  v_{l+1} = gelu(z_l) (last layer: z_l)
+//   out      = Qw . v_L + q0                            (projection)
+// Backward passes reuse the same transforms: the adjoint of Re(IDFT) is the
+// unscaled forward DFT and the adjoint of (1/n) DFT is (1/n) Re(IDFT).
+// Weight gradients are per-workgroup sums over 512 grid points, added with
+// vector-memory atomics into a caller-zeroed gradient vector.
+//
+// Layouts: activations are [batch][channel][x] FP32 with x = (I*16+J)*16+K;
+// mode arrays are [batch][channel][k] complex with k = (px*8+py)*8+pz and
+// wavenumber index kidx(p) per axis; theta holds P, p0, per layer
+// (R (i,o,k) complex interleaved, Wm (o,i), b), Qw, q0.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/frfs_sur.h"
+
+int fq_internal_fail(int code, const std::string &msg);   // fq_lib.hip
+
+namespace {
+
+constexpr int SN = 16, SN2 = SN * SN, SN3 = SN * SN * SN;
+constexpr int SW = 32, SM = 8, SM3 = SM * SM * SM;
+constexpr int64_t SR = 2LL * SW * SW * SM3;          // floats of one layer's R
+constexpr int64_t SLAYER = SR + SW * SW + SW;        // floats of one layer
+
+__device__ __forceinline__ int kidx(int p) { return p < SM / 2 ? p : p + SN - SM; }
+
+// tw[j] = (cos, sin)(2 pi j / 16)
+__device__ __forceinline__ void load_tw(float2 *tw) {
+    if (threadIdx.x < SN) {
+        float s, c;
+        sincospif(float(threadIdx.x) / (SN / 2), &s, &c);
+        tw[threadIdx.x] = make_float2(c, s);
+    }
+}
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
+}
+
+// X[slice] = scale * sum_x v[slice](x) e^{-i k.x} over the retained modes.
+// One workgroup per (batch, channel) slice; separable z, y, x stages in LDS.
+__global__ __launch_bounds__(256) void k_sur_dft(const float *__restrict__ v, float scale, float2 *__restrict__ X) {
+    __shared__ float vin[SN2 * 17];
+    __shared__ float2 A[SN2 * 9];            // [I][J][pz]
+    __shared__ float2 Bs[SN * SM * SM];      // [I][py][pz]
+    __shared__ float2 tw[SN];
+    const int t = threadIdx.x;
+    const int64_t slice = blockIdx.x;
+    const float *src = v + slice * SN3;
+    load_tw(tw);
+#pragma unroll
+    for (int r = 0; r < SN3 / 256; ++r) {
+        const int e = t + 256 * r;
+        vin[(e >> 4) * 17 + (e & 15)] = src[e];
+    }
+    __syncthreads();
+    {
+        float a[SN];
+#pragma unroll
+        for (int K = 0; K < SN; ++K) a[K] = vin[t * 17 + K];
+#pragma unroll
+        for (int p = 0; p < SM; ++p) {
+            const int k = kidx(p);
+            float re = 0.f, im = 0.f;
+#pragma unroll
+            for (int K = 0; K < SN; ++K) {
+                const float2 w = tw[(k * K) & (SN - 1)];
+                re += a[K] * w.x;
+                im -= a[K] * w.y;
+            }
+            A[t * 9 + p] = make_float2(re, im);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int o = t + 256 * r;
+        const int I = o >> 6, py = (o >> 3) & 7, pz = o & 7;
+        const int k = kidx(py);
+        float re = 0.f, im = 0.f;
+#pragma unroll
+        for (int J = 0; J < SN; ++J) {
+            const float2 a = A[(I * SN + J) * 9 + pz];
+            const float2 w = tw[(k * J) & (SN - 1)];
+            re += a.x * w.x + a.y * w.y;
+            im += a.y * w.x - a.x * w.y;
+        }
+        Bs[o] = make_float2(re, im);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int o = t + 256 * r;
+        const int px = o >> 6, py = (o >> 3) & 7, pz = o & 7;
+        const int k = kidx(px);
+        float re = 0.f, im = 0.f;
+#pragma unroll
+        for (int I = 0; I < SN; ++I) {
+            const float2 a = Bs[(I * SM + py) * SM + pz];
+            const float2 w = tw[(k * I) & (SN - 1)];
+            re += a.x * w.x + a.y * w.y;
+            im += a.y * w.x - a.x * w.y;
+        }
+        X[slice * SM3 + o] = make_float2(scale * re, scale * im);
+    }
+}
+
+// out[slice](x) = (accumulate ? out : 0) + scale * Re sum_k Y[slice](k) e^{+i k.x}.
+__global__ __launch_bounds__(256) void k_sur_idft(const float2 *__restrict__ Y, float scale, int accumulate,
+                                                  float *__restrict__ out) {
+    __shared__ float2 Ys[SM3];               // [px][py][pz]
+    __shared__ float2 T1[SN * SM * SM];      // [I][py][pz]
+    __shared__ float2 T2[SN2 * 9];           // [I][J][pz]
+    __shared__ float2 tw[SN];
+    const int t = threadIdx.x;
+    const int64_t slice = blockIdx.x;
+    load_tw(tw);
+    Ys[t] = Y[slice * SM3 + t];
+    Ys[t + 256] = Y[slice * SM3 + t + 256];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int o = t + 256 * r;
+        const int I = o >> 6, py = (o >> 3) & 7, pz = o & 7;
+        float re = 0.f, im = 0.f;
+#pragma unroll
+        for (int px = 0; px < SM; ++px) {
+            const float2 a = Ys[(px * SM + py) * SM + pz];
+            const float2 w = tw[(kidx(px) * I) & (SN - 1)];
+            re += a.x * w.x - a.y * w.y;
+            im += a.x * w.y + a.y * w.x;
+        }
+        T1[o] = make_float2(re, im);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int o = t + 256 * r;
+        const int I = o >> 7, J = (o >> 3) & 15, pz = o & 7;
+        float re = 0.f, im = 0.f;
+#pragma unroll
+        for (int py = 0; py < SM; ++py) {
+            const float2 a = T1[(I * SM + py) * SM + pz];
+            const float2 w = tw[(kidx(py) * J) & (SN - 1)];
+            re += a.x * w.x - a.y * w.y;
+            im += a.x * w.y + a.y * w.x;
+        }
+        T2[(I * SN + J) * 9 + pz] = make_float2(re, im);
+    }
+    __syncthreads();
+    float *dst = out + slice * SN3;
+#pragma unroll
+    for (int r = 0; r < SN3 / 256; ++r) {
+        const int x = t + 256 * r;
+        const int I = x >> 8, J = (x >> 4) & 15, K = x & 15;
+        float s = 0.f;
+#pragma unroll
+        for (int pz = 0; pz < SM; ++pz) {
+            const float2 a = T2[(I * SN + J) * 9 + pz];
+            const float2 w = tw[(kidx(pz) * K) & (SN - 1)];
+            s += a.x * w.x - a.y * w.y;
+        }
+        dst[x] = (accumulate ? dst[x] : 0.f) + scale * s;
+    }
+}
+
+// Channel mixing per mode.  MODE 0: Y(b,o,k) = sum_i X(b,i,k) R(i,o,k);
+// MODE 1: gX(b,i,k) = sum_o gY(b,o,k) conj R(i,o,k);
+// MODE 2: gR(i,o,k) += sum_b conj X(b,i,k) gY(b,o,k).
+// One thread per output; k is the fastest index, so every access is coalesced.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_sur_mix(const float2 *__restrict__ a, const float2 *__restrict__ bm,
+                                                 float2 *__restrict__ c, int64_t batch) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const int64_t total = MODE == 2 ? int64_t(SW) * SW * SM3 : batch * SW * SM3;
+    if (idx >= total) return;
+    const int k = int(idx & (SM3 - 1));
+    const int q = int((idx >> 9) & (SW - 1));
+    const int64_t hi = idx >> 14;
+    float2 acc = make_float2(0.f, 0.f);
+    if constexpr (MODE == 0) {        // a = X, bm = R, q = o, hi = b
+        for (int i = 0; i < SW; ++i) {
+            const float2 x = a[(hi * SW + i) * SM3 + k];
+            const float2 r = bm[(int64_t(i) * SW + q) * SM3 + k];
+            acc.x += x.x * r.x - x.y * r.y;
+            acc.y += x.x * r.y + x.y * r.x;
+        }
+        c[idx] = acc;
+    } else if constexpr (MODE == 1) { // a = gY, bm = R, q = i, hi = b
+        for (int o = 0; o < SW; ++o) {
+            const float2 g = a[(hi * SW + o) * SM3 + k];
+            const float2 r = bm[(int64_t(q) * SW + o) * SM3 + k];
+            acc.x += g.x * r.x + g.y * r.y;
+            acc.y += g.y * r.x - g.x * r.y;
+        }
+        c[idx] = acc;
+    } else {                          // a = X, bm = gY, q = o, hi = i
+        for (int64_t b = 0; b < batch; ++b) {
+            const float2 x = a[(b * SW + hi) * SM3 + k];
+            const float2 g = bm[(b * SW + q) * SM3 + k];
+            acc.x += x.x * g.x + x.y * g.y;
+            acc.y += x.x * g.y - x.y * g.x;
+        }
+        c[idx].x += acc.x;
+        c[idx].y += acc.y;
+    }
+}
+
+// v_0(b, c, x) = P_c g(b, x) + p0_c
+__global__ __launch_bounds__(256) void k_sur_lift(const float *__restrict__ g, const float *__restrict__ th,
+                                                  float *__restrict__ v, int64_t total) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= total) return;
+    const int64_t b = idx >> 12;
+    const int x = int(idx & (SN3 - 1));
+    const float gv = g[idx];
+#pragma unroll 8
+    for (int c = 0; c < SW; ++c) v[(b * SW + c) * SN3 + x] = th[c] * gv + th[SW + c];
+}
+
+// z = Wm v + b + s; vout = act ? gelu(z) : z (z stored when zout is set).
+__global__ __launch_bounds__(256) void k_sur_pw(const float *__restrict__ v, const float *__restrict__ s,
+                                                const float *__restrict__ lw, int act, float *__restrict__ zout,
+                                                float *__restrict__ vout, int64_t total) {
+    __shared__ float Ws[SW * SW + SW];
+    const int t = threadIdx.x;
+    for (int e = t; e < SW * SW + SW; e += 256) Ws[e] = lw[e];
+    __syncthreads();
+    const int64_t idx = int64_t(blockIdx.x) * 256 + t;
+    if (idx >= total) return;
+    const int64_t b = idx >> 12;
+    const int x = int(idx & (SN3 - 1));
+    float vi[SW];
+#pragma unroll
+    for (int i = 0; i < SW; ++i) vi[i] = v[(b * SW + i) * SN3 + x];
+#pragma unroll 4
+    for (int o = 0; o < SW; ++o) {
+        const int64_t at = (b * SW + o) * SN3 + x;
+        float z = Ws[SW * SW + o] + s[at];
+#pragma unroll
+        for (int i = 0; i < SW; ++i) z += Ws[o * SW + i] * vi[i];
+        if (zout) zout[at] = z;
+        vout[at] = act ? gelu(z) : z;
+    }
+}
+
+// out(b, x) = Qw . v_L(b, :, x) + q0
+__global__ __launch_bounds__(256) void k_sur_proj(const float *__restrict__ v, const float *__restrict__ qw,
+                                                  float *__restrict__ out, int64_t total) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= total) return;
+    const int64_t b = idx >> 12;
+    const int x = int(idx & (SN3 - 1));
+    float s = qw[SW];
+#pragma unroll 8
+    for (int c = 0; c < SW; ++c) s += qw[c] * v[(b * SW + c) * SN3 + x];
+    out[idx] = s;
+}
+
+// gv(b, c, x) = Qw_c gout(b, x)
+__global__ __launch_bounds__(256) void k_sur_proj_bwd(const float *__restrict__ gout, const float *__restrict__ qw,
+                                                      float *__restrict__ gv, int64_t total) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= total) return;
+    const int64_t b = idx >> 12;
+    const int x = int(idx & (SN3 - 1));
+    const float g = gout[idx];
+#pragma unroll 8
+    for (int c = 0; c < SW; ++c) gv[(b * SW + c) * SN3 + x] = qw[c] * g;
+}
+
+// gz = gv * gelu'(z) (or gv for the last layer), written over gv;
+// gvin = Wm^T gz.
+__global__ __launch_bounds__(256) void k_sur_pw_bwd(float *__restrict__ gv, const float *__restrict__ z,
+                                                    const float *__restrict__ lw, int act, float *__restrict__ gvin,
+                                                    int64_t total) {
+    __shared__ float Ws[SW * SW];
+    const int t = threadIdx.x;
+    for (int e = t; e < SW * SW; e += 256) Ws[e] = lw[e];
+    __syncthreads();
+    const int64_t idx = int64_t(blockIdx.x) * 256 + t;
+    if (idx >= total) return;
+    const int64_t b = idx >> 12;
+    const int x = int(idx & (SN3 - 1));
+    float gz[SW];
+#pragma unroll
+    for (int o = 0; o < SW; ++o) {
+        const int64_t at = (b * SW + o) * SN3 + x;
+        const float g = gv[at];
+        gz[o] = act ? g * gelu_grad(z[at]) : g;
+        gv[at] = gz[o];
+    }
+#pragma unroll 4
+    for (int i = 0; i < SW; ++i) {
+        float a = 0.f;
+#pragma unroll
+        for (int o = 0; o < SW; ++o) a += Ws[o * SW + i] * gz[o];
+        gvin[(b * SW + i) * SN3 + x] = a;
+    }
+}
+
+// G(o, i) += sum_{b, x} A(b, o, x) V(b, i, x);  gb(o) += sum_{b, x} A(b, o, x).
+// Grid (16^3 / 512, batch): each workgroup sums 512 points in LDS tiles of 64.
+template <int COUT, int CIN>
+__global__ __launch_bounds__(256) void k_sur_wgrad(const float *__restrict__ A, const float *__restrict__ V,
+                                                   float *__restrict__ G, float *__restrict__ gb) {
+    constexpr int NO = COUT * CIN;
+    constexpr int PER = (NO + 255) / 256;
+    __shared__ float As[COUT][65];
+    __shared__ float Vs[CIN][65];
+    const int t = threadIdx.x;
+    const int64_t b = blockIdx.y;
+    const int x0 = blockIdx.x * 512;
+    float acc[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) acc[q] = 0.f;
+    float accb = 0.f;
+    for (int p0 = x0; p0 < x0 + 512; p0 += 64) {
+        for (int e = t; e < COUT * 64; e += 256) As[e >> 6][e & 63] = A[(b * COUT + (e >> 6)) * SN3 + p0 + (e & 63)];
+        for (int e = t; e < CIN * 64; e += 256) Vs[e >> 6][e & 63] = V[(b * CIN + (e >> 6)) * SN3 + p0 + (e & 63)];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int j = t + 256 * q;
+            if (j < NO) {
+                const int o = j / CIN, i = j % CIN;
+                float s = 0.f;
+#pragma unroll 16
+                for (int p = 0; p < 64; ++p) s += As[o][p] * Vs[i][p];
+                acc[q] += s;
+            }
+        }
+        if (t < COUT) {
+#pragma unroll 16
+            for (int p = 0; p < 64; ++p) accb += As[t][p];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int j = t + 256 * q;
+        if (j < NO) atomicAdd(&G[j], acc[q]);
+    }
+    if (t < COUT) atomicAdd(&gb[t], accb);
+}
+
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// gout = 2 (out - target) / count; sums[0] += sum (out - target)^2,
+// sums[1] += sum target^2.
+__global__ __launch_bounds__(256) void k_sur_loss(const float *__restrict__ out, const float *__restrict__ tgt,
+                                                  float *__restrict__ gout, double *__restrict__ sums, int64_t count) {
+    __shared__ double red[256];
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    double d2 = 0, t2 = 0;
+    if (idx < count) {
+        const float d = out[idx] - tgt[idx];
+        gout[idx] = float(2.0 / double(count)) * d;
+        d2 = double(d) * d;
+        t2 = double(tgt[idx]) * tgt[idx];
+    }
+    const double s0 = block_sum(d2, red);
+    const double s1 = block_sum(t2, red);
+    if (threadIdx.x == 0) {
+        atomicAdd(&sums[0], s0);
+        atomicAdd(&sums[1], s1);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sur_adam(float *__restrict__ th, const float *__restrict__ g,
+                                                  float *__restrict__ m, float *__restrict__ v, int64_t n, float lr,
+                                                  float b1, float b2, float eps, float c1, float c2) {
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    th[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// dst(b, x) = src(dof, x) / (rho_b^power * scale), rho_b = rho_in[b] or
+// sum_x src(dof, x) cw (written to rho_out).  One workgroup per DOF.
+__global__ __launch_bounds__(256) void k_sur_gather(int n, double cw, double scale, int power,
+                                                    const double *__restrict__ src, int nupts, int64_t ldim,
+                                                    int64_t first, const double *__restrict__ rho_in,
+                                                    double *__restrict__ rho_out, float *__restrict__ dst) {
+    __shared__ double red[256];
+    const int64_t b = blockIdx.x;
+    const int64_t dof = first + b;
+    const int64_t e = dof / nupts;
+    const int u = int(dof % nupts);
+    const double *base = src + u * ldim + (e / 32) * int64_t(n) * 32 + e % 32;
+    double rho;
+    if (rho_in) {
+        rho = rho_in[b];
+    } else {
+        double s = 0;
+        for (int x = threadIdx.x; x < n; x += 256) s += base[int64_t(x) * 32];
+        rho = block_sum(s, red) * cw;
+        if (rho_out && threadIdx.x == 0) rho_out[b] = rho;
+    }
+    double den = scale;
+    for (int k = 0; k < power; ++k) den *= rho;
+    const double inv = den != 0.0 ? 1.0 / den : 0.0;
+    for (int x = threadIdx.x; x < n; x += 256) dst[b * n + x] = float(base[int64_t(x) * 32] * inv);
+}
+
+struct SurGram {
+    double ginv[25];
+    double gdiag[5];
+};
+
+// Conservation check and scatter.  For DOF b: moments m_j = cw sum_x q phi_j
+// of the collision invariants phi = (1, cx, cy, cz, |c|^2); defect = max_j
+// |m_j| / sqrt(G_jj cw sum q^2); DOFs with defect > tol are flagged and not
+// written; the rest get Q = rho_b^2 scale (q - sum_j lambda_j phi_j) with
+// lambda = G^{-1} m (the least-squares conservation projection).
+__global__ __launch_bounds__(256) void k_sur_finish(int N, double c0, double dc, double cw, double scale, double tol,
+                                                    const float *__restrict__ q, const double *__restrict__ rho,
+                                                    SurGram gm, double *__restrict__ dst, int nupts, int64_t ldim,
+                                                    int64_t first, double *__restrict__ defect,
+                                                    int32_t *__restrict__ flag) {
+    __shared__ double red[256];
+    const int n = N * N * N;
+    const int64_t b = blockIdx.x;
+    const float *qb = q + b * n;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int x = threadIdx.x; x < n; x += 256) {
+        const double cx = c0 + dc * (x / (N * N)), cy = c0 + dc * ((x / N) % N), cz = c0 + dc * (x % N);
+        const double v = qb[x];
+        acc[0] += v;
+        acc[1] += v * cx;
+        acc[2] += v * cy;
+        acc[3] += v * cz;
+        acc[4] += v * (cx * cx + cy * cy + cz * cz);
+        acc[5] += v * v;
+    }
+    double mom[5];
+    for (int j = 0; j < 5; ++j) mom[j] = block_sum(acc[j], red) * cw;
+    const double nq = sqrt(block_sum(acc[5], red) * cw);
+    double d = 0;
+    for (int j = 0; j < 5; ++j) {
+        const double cj = nq > 0 ? fabs(mom[j]) / (sqrt(gm.gdiag[j]) * nq) : 0.0;
+        d = cj > d ? cj : d;
+    }
+    const bool bad = !(d <= tol);
+    if (threadIdx.x == 0) {
+        defect[b] = d;
+        flag[b] = bad ? 1 : 0;
+    }
+    if (bad) return;
+    double lam[5];
+    for (int j = 0; j < 5; ++j) {
+        double s = 0;
+        for (int k = 0; k < 5; ++k) s += gm.ginv[j * 5 + k] * mom[k];
+        lam[j] = s;
+    }
+    const double r = rho[b];
+    const double f = r * r * scale;
+    const int64_t dof = first + b;
+    const int64_t e = dof / nupts;
+    const int u = int(dof % nupts);
+    double *out = dst + u * ldim + (e / 32) * int64_t(n) * 32 + e % 32;
+    for (int x = threadIdx.x; x < n; x += 256) {
+        const double cx = c0 + dc * (x / (N * N)), cy = c0 + dc * ((x / N) % N), cz = c0 + dc * (x % N);
+        const double p = lam[0] + lam[1] * cx + lam[2] * cy + lam[3] * cz + lam[4] * (cx * cx + cy * cy + cz * cz);
+        out[int64_t(x) * 32] = f * (double(qb[x]) - p);
+    }
+}
+
+inline unsigned nblk(int64_t n) { return unsigned((n + 255) / 256); }
+
+int check_shape(const fq_sur_shape *s, const char *who) {
+    if (!s) return fq_internal_fail(FQ_ERR_INVALID, std::string(who) + ": null shape");
+    if (s->abi_version != FQ_SUR_ABI_VERSION)
+        return fq_internal_fail(FQ_ERR_INVALID, std::string(who) + ": ABI version mismatch");
+    if (s->Nv != SN || s->width != SW || s->modes != SM || s->layers < 1 || s->layers > FQ_SUR_MAX_LAYERS)
+        return fq_internal_fail(FQ_ERR_UNSUPPORTED, std::string(who) +
+                                ": supported shape is Nv 16, width 32, modes 8, 1..8 layers");
+    return FQ_SUCCESS;
+}
+
+int hip_status(const char *who) {
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? FQ_SUCCESS : fq_internal_fail(FQ_ERR_HIP, std::string(who) + ": " + hipGetErrorString(e));
+}
+
+struct ActPtrs {
+    float *v[FQ_SUR_MAX_LAYERS + 1];
+    float *z[FQ_SUR_MAX_LAYERS];
+    float2 *X[FQ_SUR_MAX_LAYERS];
+    float *s;
+    float2 *Y;
+};
+
+// Activation layout (floats): training keeps v_0..v_L, z_0..z_{L-1} and
+// X_0..X_{L-1}; inference keeps two v buffers and one X.  Both end with the
+// s and Y scratch.
+int64_t act_floats(int L, int64_t B, bool training) {
+    const int64_t vol = B * SW * SN3, md = 2 * B * SW * SM3;
+    return training ? (L + 1) * vol + L * vol + L * md + vol + md : 2 * vol + md + vol + md;
+}
+
+ActPtrs act_ptrs(float *acts, int L, int64_t B, bool training) {
+    const int64_t vol = B * SW * SN3, md = 2 * B * SW * SM3;
+    ActPtrs p{};
+    float *q = acts;
+    if (training) {
+        for (int l = 0; l <= L; ++l) p.v[l] = q + l * vol;
+        q += (L + 1) * vol;
+        for (int l = 0; l < L; ++l) p.z[l] = q + l * vol;
+        q += L * vol;
+        for (int l = 0; l < L; ++l) p.X[l] = reinterpret_cast<float2 *>(q + l * md);
+        q += L * md;
+    } else {
+        for (int l = 0; l <= L; ++l) p.v[l] = q + (l & 1) * vol;
+        q += 2 * vol;
+        for (int l = 0; l < L; ++l) p.X[l] = reinterpret_cast<float2 *>(q);
+        q += md;
+    }
+    p.s = q;
+    p.Y = reinterpret_cast<float2 *>(q + vol);
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fq_sur_sizes(const fq_sur_shape *shape, int64_t batch, int32_t training, int64_t *nparams, int64_t *act_size,
+                 int64_t *work_size) {
+    if (int rc = check_shape(shape, "fq_sur_sizes")) return rc;
+    if (batch < 0) return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_sizes: negative batch");
+    const int L = shape->layers;
+    if (nparams) *nparams = 2 * SW + L * SLAYER + SW + 1;
+    if (act_size) *act_size = act_floats(L, batch, training != 0);
+    if (work_size) *work_size = 2 * batch * SW * SN3 + 4 * batch * SW * SM3;
+    return FQ_SUCCESS;
+}
+
+int fq_sur_forward(const fq_sur_shape *shape, const float *theta, const float *g, int64_t batch, int32_t training,
+                   float *acts, float *out, void *stream) {
+    if (int rc = check_shape(shape, "fq_sur_forward")) return rc;
+    if (batch < 0 || batch > FQ_SUR_MAX_BATCH || !theta || !g || !acts || !out)
+        return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_forward: invalid arguments");
+    if (batch == 0) return FQ_SUCCESS;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int L = shape->layers;
+    const bool tr = training != 0;
+    const ActPtrs p = act_ptrs(acts, L, batch, tr);
+    const int64_t pts = batch * SN3;
+    const unsigned slices = unsigned(batch * SW);
+    hipLaunchKernelGGL(k_sur_lift, dim3(nblk(pts)), dim3(256), 0, st, g, theta, p.v[0], pts);
+    for (int l = 0; l < L; ++l) {
+        const float *lay = theta + 2 * SW + l * SLAYER;
+        const float2 *R = reinterpret_cast<const float2 *>(lay);
+        hipLaunchKernelGGL(k_sur_dft, dim3(slices), dim3(256), 0, st, p.v[l], 1.0f / SN3, p.X[l]);
+        hipLaunchKernelGGL(k_sur_mix<0>, dim3(nblk(batch * SW * SM3)), dim3(256), 0, st, p.X[l], R, p.Y, batch);
+        hipLaunchKernelGGL(k_sur_idft, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, p.s);
+        hipLaunchKernelGGL(k_sur_pw, dim3(nblk(pts)), dim3(256), 0, st, p.v[l], p.s, lay + SR, int(l < L - 1),
+                           tr ? p.z[l] : nullptr, p.v[l + 1], pts);
+    }
+    hipLaunchKernelGGL(k_sur_proj, dim3(nblk(pts)), dim3(256), 0, st, p.v[L], theta + 2 * SW + L * SLAYER, out, pts);
+    return hip_status("fq_sur_forward");
+}
+
+int fq_sur_backward(const fq_sur_shape *shape, const float *theta, const float *g, int64_t batch, float *acts,
+                    const float *gout, float *grad, float *work, void *stream) {
+    if (int rc = check_shape(shape, "fq_sur_backward")) return rc;
+    if (batch < 0 || batch > FQ_SUR_MAX_BATCH || !theta || !g || !acts || !gout || !grad || !work)
+        return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_backward: invalid arguments");
+    if (batch == 0) return FQ_SUCCESS;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int L = shape->layers;
+    const ActPtrs p = act_ptrs(acts, L, batch, true);
+    const int64_t pts = batch * SN3, vol = batch * SW * SN3;
+    const unsigned slices = unsigned(batch * SW);
+    float *gv = work, *gvin = work + vol;
+    float2 *gY = reinterpret_cast<float2 *>(work + 2 * vol);
+    float2 *gX = gY + batch * SW * SM3;
+    const dim3 wg(SN3 / 512, unsigned(batch));
+    const int64_t qoff = 2 * SW + L * SLAYER;
+    hipLaunchKernelGGL((k_sur_wgrad<1, SW>), wg, dim3(256), 0, st, gout, p.v[L], grad + qoff, grad + qoff + SW);
+    hipLaunchKernelGGL(k_sur_proj_bwd, dim3(nblk(pts)), dim3(256), 0, st, gout, theta + qoff, gv, pts);
+    for (int l = L - 1; l >= 0; --l) {
+        const int64_t lo = 2 * SW + l * SLAYER;
+        const float2 *R = reinterpret_cast<const float2 *>(theta + lo);
+        float2 *gR = reinterpret_cast<float2 *>(grad + lo);
+        hipLaunchKernelGGL(k_sur_pw_bwd, dim3(nblk(pts)), dim3(256), 0, st, gv, p.z[l], theta + lo + SR,
+                           int(l < L - 1), gvin, pts);
+        hipLaunchKernelGGL((k_sur_wgrad<SW, SW>), wg, dim3(256), 0, st, gv, p.v[l], grad + lo + SR,
+                           grad + lo + SR + SW * SW);
+        hipLaunchKernelGGL(k_sur_dft, dim3(slices), dim3(256), 0, st, gv, 1.0f, gY);
+        hipLaunchKernelGGL(k_sur_mix<2>, dim3(nblk(int64_t(SW) * SW * SM3)), dim3(256), 0, st, p.X[l], gY, gR, batch);
+        hipLaunchKernelGGL(k_sur_mix<1>, dim3(nblk(batch * SW * SM3)), dim3(256), 0, st, gY, R, gX, batch);
+        hipLaunchKernelGGL(k_sur_idft, dim3(slices), dim3(256), 0, st, gX, 1.0f / SN3, 1, gvin);
+        float *tmp = gv;
+        gv = gvin;
+        gvin = tmp;
+    }
+    hipLaunchKernelGGL((k_sur_wgrad<SW, 1>), wg, dim3(256), 0, st, gv, g, grad, grad + SW);
+    return hip_status("fq_sur_backward");
+}
+
+int fq_sur_loss(int64_t count, const float *out, const float *target, float *gout, double *sums, void *stream) {
+    if (count < 0 || !out || !target || !gout || !sums)
+        return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_loss: invalid arguments");
+    if (count == 0) return FQ_SUCCESS;
+    hipLaunchKernelGGL(k_sur_loss, dim3(nblk(count)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), out,
+                       target, gout, sums, count);
+    return hip_status("fq_sur_loss");
+}
+
+int fq_sur_adam(int64_t nparams, float *theta, const float *grad, float *m, float *v, float lr, float beta1,
+                float beta2, float eps, int32_t step, void *stream) {
+    if (nparams < 0 || !theta || !grad || !m || !v || step < 1 || !(lr >= 0) || !(beta1 >= 0 && beta1 < 1) ||
+        !(beta2 >= 0 && beta2 < 1) || !(eps > 0))
+        return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_adam: invalid arguments");
+    if (nparams == 0) return FQ_SUCCESS;
+    const float c1 = float(1.0 / (1.0 - pow(double(beta1), step)));
+    const float c2 = float(1.0 / (1.0 - pow(double(beta2), step)));
+    hipLaunchKernelGGL(k_sur_adam, dim3(nblk(nparams)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), theta,
+                       grad, m, v, nparams, lr, beta1, beta2, eps, c1, c2);
+    return hip_status("fq_sur_adam");
+}
+
+int fq_sur_gather(int32_t Nv, double cw, double scale, int32_t power, const double *src, int32_t nupts, int64_t ldim,
+                  int32_t neles, int64_t first_dof, int64_t ndof, const double *rho_in, double *rho_out, float *dst,
+                  void *stream) {
+    if (Nv <= 0 || !(cw > 0) || !(scale != 0) || power < 0 || power > 4 || !src || !dst || nupts <= 0 ||
+        neles <= 0 || first_dof < 0 || ndof < 0 || first_dof + ndof > int64_t(nupts) * neles)
+        return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_gather: invalid arguments");
+    const int64_t nparr = (int64_t(neles) + 31) / 32 * 32;
+    if (ldim < int64_t(Nv) * Nv * Nv * nparr) return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_gather: ldim too small");
+    if (ndof == 0) return FQ_SUCCESS;
+    hipLaunchKernelGGL(k_sur_gather, dim3(unsigned(ndof)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       Nv * Nv * Nv, cw, scale, power, src, nupts, ldim, first_dof, rho_in, rho_out, dst);
+    return hip_status("fq_sur_gather");
+}
+
+int fq_sur_finish(int32_t Nv, double L, double cw, double scale, double tol, const float *q, const double *rho,
+                  const double *gram_inv, const double *gram_diag, double *dst, int32_t nupts, int64_t ldim,
+                  int32_t neles, int64_t first_dof, int64_t ndof, double *defect, int32_t *flag, void *stream) {
+    if (Nv <= 0 || !(L > 0) || !(cw > 0) || !q || !rho || !gram_inv || !gram_diag || !dst || !defect || !flag ||
+        nupts <= 0 || neles <= 0 || first_dof < 0 || ndof < 0 || first_dof + ndof > int64_t(nupts) * neles)
+        return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_finish: invalid arguments");
+    const int64_t nparr = (int64_t(neles) + 31) / 32 * 32;
+    if (ldim < int64_t(Nv) * Nv * Nv * nparr) return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_finish: ldim too small");
+    if (ndof == 0) return FQ_SUCCESS;
+    SurGram gm;
+    for (int j = 0; j < 25; ++j) gm.ginv[j] = gram_inv[j];
+    for (int j = 0; j < 5; ++j) gm.gdiag[j] = gram_diag[j];
+    hipLaunchKernelGGL(k_sur_finish, dim3(unsigned(ndof)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), Nv,
+                       -L + L / Nv, 2 * L / Nv, cw, scale, tol, q, rho, gm, dst, nupts, ldim, first_dof, defect, flag);
+    return hip_status("fq_sur_finish");
+}
+
+}  // extern "C"

@@ -1,0 +1,277 @@
+"""Learned stand-in for the collision operator: a Fourier-layer neural
+operator evaluated, trained and checked by the HIP kernels declared in
+include/frfs_sur.h.
+
+The reference has no learned operator.  The stand-in takes the plugin place
+of ``vhs-gll`` (frfs/solvers/dgfs/scattering.py:227-286, type key
+``vhs-gll-fno``) on the 16^3 grid and keeps the spectral operator both as
+its teacher (training pairs are (f, Q(f)) from ``fs_batch``) and as its
+fallback: ``fs_batch`` runs the network on every DOF, measures each output's
+conservation defect (the largest cosine between Q and a collision invariant
+1, c, |c|^2), writes the conservation-projected network output where the
+defect is within ``tol`` and evaluates the spectral operator on the rest.
+
+Normalisation: Q is quadratic in f, so with the density rho of f the network
+maps g = f / (rho f_ref) to Q / (rho^2 q_ref) (f_ref = pi^{-3/2}, the peak of
+the unit Maxwellian; q_ref is the RMS of Q / rho^2 over the training set).
+
+Extra configuration keys: ``[scattering-model] surrogate-weights`` (an .npz
+written by :meth:`FourierSurrogate.save`), ``surrogate-tol`` (default 1e-4)
+and ``surrogate-layers`` (default 4).  Without weights the network is the
+seeded initialisation and every DOF falls back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+
+import numpy as np
+
+from . import _lib
+from .scattering import DGFSVHSGLLScatteringModel, _stream
+
+F_REF = math.pi ** -1.5
+NV, WIDTH, MODES = 16, 32, 8
+
+
+def gram(cv, cw):
+    """Gram matrix of the collision invariants (1, cx, cy, cz, |c|^2) in the
+    cw-weighted inner product: (inverse, diagonal)."""
+    phi = np.stack([np.ones(cv.shape[1]), cv[0], cv[1], cv[2], (cv ** 2).sum(axis=0)])
+    G = phi @ phi.T * cw
+    return np.ascontiguousarray(np.linalg.inv(G)), np.ascontiguousarray(np.diag(G).copy())
+
+
+def init_params(layers, seed, width=WIDTH, modes=MODES):
+    """Seeded initial parameter vector (float64, theta layout of frfs_sur.h):
+    P ~ U(-1, 1), p0 ~ U(-0.1, 0.1), R ~ U(-1, 1)/W^2 per real part,
+    Wm, b, Qw ~ U(-1/sqrt(W), 1/sqrt(W)), q0 = 0."""
+    rng = np.random.default_rng(seed)
+    m3, W = modes ** 3, width
+    parts = [rng.uniform(-1, 1, W), rng.uniform(-0.1, 0.1, W)]
+    lim = 1.0 / math.sqrt(W)
+    for _ in range(layers):
+        parts.append(rng.uniform(-1, 1, 2 * W * W * m3) / (W * W))
+        parts.append(rng.uniform(-lim, lim, W * W))
+        parts.append(rng.uniform(-lim, lim, W))
+    parts.append(rng.uniform(-lim, lim, W))
+    parts.append(np.zeros(1))
+    return np.concatenate(parts)
+
+
+class FourierSurrogate:
+    """The network, its parameters and optimiser state on the current device."""
+
+    def __init__(self, backend, layers=4, seed=0, theta=None, q_ref=1.0):
+        import torch
+        self.torch = torch
+        self.backend = backend
+        self.dev = backend.device
+        self.shape = _lib.fq_sur_shape(_lib.SUR_ABI_VERSION, NV, WIDTH, MODES, layers)
+        self.layers = layers
+        self.nparams = self._sizes(0, False)[0]
+        th = init_params(layers, seed) if theta is None else np.asarray(theta, dtype=np.float64)
+        if th.shape != (self.nparams,):
+            raise ValueError(f'theta has {th.size} entries, the network has {self.nparams}')
+        self.theta = torch.tensor(th, dtype=torch.float32, device=self.dev)
+        self.q_ref = float(q_ref)
+        self._adam = None
+        self._bufs = {}
+
+    # -- sizes and buffers -------------------------------------------------
+    def _sizes(self, batch, training):
+        npar, act, work = C.c_int64(), C.c_int64(), C.c_int64()
+        _lib.check(_lib.load().fq_sur_sizes(C.byref(self.shape), batch, int(training), C.byref(npar),
+                                            C.byref(act), C.byref(work)))
+        return npar.value, act.value, work.value
+
+    def _buf(self, name, n, dtype=None):
+        torch = self.torch
+        dtype = dtype or torch.float32
+        b = self._bufs.get(name)
+        if b is None or b.numel() < n or b.dtype != dtype:
+            b = torch.empty(max(n, 1), dtype=dtype, device=self.dev)
+            self._bufs[name] = b
+        return b
+
+    def _stream(self):
+        return _stream(self.backend)
+
+    # -- network -----------------------------------------------------------
+    def forward(self, g, out, training=False):
+        """out[b] = network(g[b]) for (batch, 16^3) float32 device tensors."""
+        batch = g.shape[0]
+        _, act, _ = self._sizes(batch, training)
+        acts = self._buf('acts_train' if training else 'acts', act)
+        _lib.check(_lib.load().fq_sur_forward(C.byref(self.shape), self.theta.data_ptr(), g.data_ptr(), batch,
+                                              int(training), acts.data_ptr(), out.data_ptr(), self._stream()))
+        return acts
+
+    def backward(self, g, acts, gout, grad):
+        batch = g.shape[0]
+        _, _, work = self._sizes(batch, True)
+        w = self._buf('work', work)
+        _lib.check(_lib.load().fq_sur_backward(C.byref(self.shape), self.theta.data_ptr(), g.data_ptr(), batch,
+                                               acts.data_ptr(), gout.data_ptr(), grad.data_ptr(), w.data_ptr(),
+                                               self._stream()))
+
+    def gradient(self, g, target):
+        """(loss sums [sum d^2, sum t^2], grad) of the mean-square loss."""
+        torch = self.torch
+        out = torch.empty_like(g)
+        acts = self.forward(g, out, training=True)
+        gout = torch.empty_like(g)
+        sums = torch.zeros(2, dtype=torch.float64, device=self.dev)
+        grad = torch.zeros(self.nparams, dtype=torch.float32, device=self.dev)
+        _lib.check(_lib.load().fq_sur_loss(g.numel(), out.data_ptr(), target.data_ptr(), gout.data_ptr(),
+                                           sums.data_ptr(), self._stream()))
+        self.backward(g, acts, gout, grad)
+        return sums, grad
+
+    def train_step(self, g, target, lr, betas=(0.9, 0.999), eps=1e-8):
+        """One Adam step on a batch; returns the device loss sums."""
+        torch = self.torch
+        if self._adam is None:
+            self._adam = [torch.zeros_like(self.theta), torch.zeros_like(self.theta), 0]
+        sums, grad = self.gradient(g, target)
+        self._adam[2] += 1
+        m, v, step = self._adam
+        _lib.check(_lib.load().fq_sur_adam(self.nparams, self.theta.data_ptr(), grad.data_ptr(), m.data_ptr(),
+                                           v.data_ptr(), lr, betas[0], betas[1], eps, step, self._stream()))
+        return sums
+
+    def fit(self, g_all, t_all, *, epochs=1, batch=64, lr=1e-3, lr_final=None, seed=0, time_limit=None,
+            log=None):
+        """Adam over shuffled mini-batches of the (g, target) rows; cosine
+        decay from lr to lr_final.  Returns the per-epoch mean relative
+        squared error."""
+        torch = self.torch
+        n = g_all.shape[0]
+        nb = max(n // batch, 1)
+        total = epochs * nb
+        lr_final = lr * 0.05 if lr_final is None else lr_final
+        gen = torch.Generator(device='cpu').manual_seed(seed)
+        hist, step, t0 = [], 0, time.time()
+        for ep in range(epochs):
+            perm = torch.randperm(n, generator=gen).to(self.dev)
+            acc = torch.zeros(2, dtype=torch.float64, device=self.dev)
+            for j in range(nb):
+                idx = perm[j * batch:(j + 1) * batch]
+                a = lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(math.pi * step / total))
+                acc += self.train_step(g_all[idx].contiguous(), t_all[idx].contiguous(), a)
+                step += 1
+            r = float(acc[0] / acc[1])
+            hist.append(r)
+            if log:
+                log(f'epoch {ep}: relative squared error {r:.4e} ({time.time() - t0:.1f} s)')
+            if time_limit is not None and time.time() - t0 > time_limit:
+                break
+        return hist
+
+    def predict(self, g, chunk=8192):
+        """Network output for (batch, 16^3) float32 rows, in chunks."""
+        out = self.torch.empty_like(g)
+        for b0 in range(0, g.shape[0], chunk):
+            self.forward(g[b0:b0 + chunk], out[b0:b0 + chunk])
+        return out
+
+    # -- persistence -------------------------------------------------------
+    def save(self, path):
+        np.savez(path, theta=self.theta.cpu().numpy(), q_ref=self.q_ref, layers=self.layers)
+
+    @classmethod
+    def load(cls, backend, path):
+        with np.load(path, allow_pickle=False) as z:
+            return cls(backend, layers=int(z['layers']), theta=z['theta'].astype(np.float64),
+                       q_ref=float(z['q_ref']))
+
+
+def training_pairs(model, f, chunk=8192):
+    """(g, target, rho) float32/float64 device tensors for the rows of f
+    (ndof, 16^3): g = f / (rho f_ref), target = Q(f) / (rho^2 q) with q = 1
+    (the caller rescales by q_ref)."""
+    import torch
+    from .inputs import to_aosoa
+    be, vm = model.backend, model.vm
+    n = vm.vsize()
+    nd = f.shape[0]
+    g = torch.empty((nd, n), dtype=torch.float32, device=be.device)
+    t = torch.empty_like(g)
+    rho = torch.empty(nd, dtype=torch.float64, device=be.device)
+    lib = _lib.load()
+    for d0 in range(0, nd, chunk):
+        m = min(chunk, nd - d0)
+        fin = be.matrix((1, n, m))
+        q = be.matrix((1, n, m))
+        to_aosoa(f[d0:d0 + m], fin)
+        DGFSVHSGLLScatteringModel.fs_batch(model, fin, q)
+        st = _stream(be)
+        _lib.check(lib.fq_sur_gather(NV, vm.cw(), F_REF, 1, fin._as_parameter_, 1, fin.leaddim, m, 0, m, None,
+                                     rho[d0:].data_ptr(), g[d0:].data_ptr(), st))
+        _lib.check(lib.fq_sur_gather(NV, vm.cw(), 1.0, 2, q._as_parameter_, 1, q.leaddim, m, 0, m,
+                                     rho[d0:].data_ptr(), None, t[d0:].data_ptr(), st))
+    return g, t, rho
+
+
+class DGFSVHSGLLFNOScatteringModel(DGFSVHSGLLScatteringModel):
+    """vhs-gll with the learned stand-in in front of the spectral operator."""
+    scattering_model = 'vhs-gll-fno'
+
+    def perform_precomputation(self):
+        super().perform_precomputation()
+        if self.vm.Nv() != NV:
+            raise ValueError(f'the learned stand-in is built for the {NV}^3 grid')
+        path = self.cfg.get('scattering-model', 'surrogate-weights', '')
+        layers = self.cfg.getint('scattering-model', 'surrogate-layers', 4)
+        self.tol = self.cfg.getfloat('scattering-model', 'surrogate-tol', 1e-4)
+        self.net = FourierSurrogate.load(self.backend, path) if path else FourierSurrogate(self.backend, layers)
+        gi, gd = gram(self.vm.cv(), self.vm.cw())
+        torch = self.net.torch
+        self._ginv = torch.tensor(gi.reshape(-1), dtype=torch.float64, device=self.backend.device)
+        self._gdiag = torch.tensor(gd, dtype=torch.float64, device=self.backend.device)
+        self.last_stats = None
+
+    def fs(self, d_arr_in, d_arr_out, elem, upt):
+        nrow = d_arr_in.traits[0]
+        self.fs_batch(d_arr_in, d_arr_out, elem * nrow + upt, 1)
+
+    def fs_spectral(self, d_arr_in, d_arr_out, first_dof=0, ndof=None):
+        """The spectral operator alone (the teacher and the fallback)."""
+        DGFSVHSGLLScatteringModel.fs_batch(self, d_arr_in, d_arr_out, first_dof, ndof)
+
+    def fs_batch(self, d_arr_in, d_arr_out, first_dof=0, ndof=None, chunk=8192):
+        torch = self.net.torch
+        nrow, ldim, _ = d_arr_in.traits
+        neles = d_arr_in.ioshape[2]
+        ndof = nrow * neles - first_dof if ndof is None else ndof
+        lib = _lib.load()
+        st = _stream(self.backend)
+        n = self.vm.vsize()
+        cw, L = self.vm.cw(), self.vm.L()
+        flags_all = []
+        defects = []
+        for d0 in range(first_dof, first_dof + ndof, chunk):
+            m = min(chunk, first_dof + ndof - d0)
+            g = self.net._buf('g', m * n)[:m * n].view(m, n)
+            q = self.net._buf('q', m * n)[:m * n].view(m, n)
+            rho = self.net._buf('rho', m, torch.float64)
+            dfc = self.net._buf('defect', m, torch.float64)
+            flag = self.net._buf('flag', m, torch.int32)
+            _lib.check(lib.fq_sur_gather(NV, cw, F_REF, 1, d_arr_in._as_parameter_, nrow, ldim, neles, d0, m,
+                                         None, rho.data_ptr(), g.data_ptr(), st))
+            self.net.forward(g, q)
+            _lib.check(lib.fq_sur_finish(NV, L, cw, self.net.q_ref, self.tol, q.data_ptr(), rho.data_ptr(),
+                                         self._ginv.data_ptr(), self._gdiag.data_ptr(), d_arr_out._as_parameter_,
+                                         nrow, ldim, neles, d0, m, dfc.data_ptr(), flag.data_ptr(), st))
+            flags_all.append(flag[:m].cpu().numpy().copy())
+            defects.append(dfc[:m].cpu().numpy().copy())
+        flags = np.concatenate(flags_all) if flags_all else np.zeros(0, np.int32)
+        # spectral operator on each run of flagged DOFs
+        bad = np.flatnonzero(flags)
+        runs = np.split(bad, np.flatnonzero(np.diff(bad) != 1) + 1) if bad.size else []
+        for r in runs:
+            self.fs_spectral(d_arr_in, d_arr_out, first_dof + int(r[0]), int(r.size))
+        self.last_stats = {'ndof': int(ndof), 'fallback': int(bad.size),
+                           'defect': np.concatenate(defects) if defects else np.zeros(0)}
+        return self.last_stats

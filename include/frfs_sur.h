@@ -1,0 +1,95 @@
+/*
+ * frfs_sur.h -- C ABI of the learned stand-in for Q(f,f): a Fourier-layer
+ * neural operator with its own forward, backward, loss and Adam kernels,
+ * plus the AoSoA gather and the conservation check / scatter around it.
+ *
+ * The reference has no learned operator; the stand-in takes the place of
+ * the same plugin call as the spectral operator,
+ *   DGFSVHSGLLScatteringModel.fs (frfs/solvers/dgfs/scattering.py:227-286)
+ * batched like fq_eval, and reads and writes the same AoSoA matrices
+ * (frfs/backends/base/types.py:26-47; dof = elem*nupts + upt).  DOFs whose
+ * conservation defect exceeds the tolerance are flagged by fq_sur_finish and
+ * left for the spectral operator (fq_eval) to fill.
+ *
+ * Network (16^3 grid, width 32, 8^3 retained modes, 1..8 layers):
+ *   v_0 = P g + p0;  z_l = Wm_l v_l + b_l + Re IDFT(R_l . DFT(v_l)/n);
+ *   v_{l+1} = gelu(z_l) (last layer: z_l);  out = Qw . v_L + q0,
+ * with g = f / (rho f_ref) and out ~ Q / (rho^2 q_ref) (Q is quadratic in f).
+ * theta layout (floats): P (W), p0 (W), per layer [R (W*W*m^3 complex,
+ * (i, o, k) order, interleaved), Wm (W*W, (o, i)), b (W)], Qw (W), q0.
+ * Activation arrays are [batch][channel][x] FP32, x = (I*16 + J)*16 + K.
+ *
+ * All buffers are caller-owned device memory; nothing here allocates.
+ * Errors: FQ_SUCCESS or a negative FQ_ERR_* code with fq_last_error().
+ */
+#ifndef FRFS_SUR_H
+#define FRFS_SUR_H
+
+#include <stdint.h>
+
+#include "frfs_fq.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FQ_SUR_ABI_VERSION 1
+#define FQ_SUR_MAX_LAYERS 8
+#define FQ_SUR_MAX_BATCH 65536
+
+typedef struct fq_sur_shape {
+    int32_t abi_version;   /* FQ_SUR_ABI_VERSION */
+    int32_t Nv;            /* grid points per axis: 16 */
+    int32_t width;         /* channels: 32 */
+    int32_t modes;         /* retained modes per axis: 8 */
+    int32_t layers;        /* Fourier layers: 1..FQ_SUR_MAX_LAYERS */
+} fq_sur_shape;
+
+/* Sizes in floats: the parameter vector, the activations of a forward call
+ * (training != 0 keeps what fq_sur_backward needs) and the backward work
+ * array, for a batch of `batch` DOFs.  Any output pointer may be NULL. */
+int fq_sur_sizes(const fq_sur_shape *shape, int64_t batch, int32_t training, int64_t *nparams,
+                 int64_t *act_size, int64_t *work_size);
+
+/* out[batch][16^3] = network(g[batch][16^3]).  acts: fq_sur_sizes floats. */
+int fq_sur_forward(const fq_sur_shape *shape, const float *theta, const float *g, int64_t batch,
+                   int32_t training, float *acts, float *out, void *stream);
+
+/* grad += d(sum gout*out)/d theta, from the activations of a training
+ * forward call on the same theta and g (grad is caller-zeroed; acts is
+ * overwritten only in its scratch part). */
+int fq_sur_backward(const fq_sur_shape *shape, const float *theta, const float *g, int64_t batch,
+                    float *acts, const float *gout, float *grad, float *work, void *stream);
+
+/* gout = 2 (out - target) / count;  sums[0] += sum (out - target)^2,
+ * sums[1] += sum target^2 (sums caller-zeroed, FP64). */
+int fq_sur_loss(int64_t count, const float *out, const float *target, float *gout, double *sums,
+                void *stream);
+
+/* One Adam step (bias-corrected, step >= 1) over the whole parameter vector. */
+int fq_sur_adam(int64_t nparams, float *theta, const float *grad, float *m, float *v, float lr,
+                float beta1, float beta2, float eps, int32_t step, void *stream);
+
+/* dst[b][x] = src(first_dof + b, x) / (rho_b^power * scale) from an AoSoA
+ * matrix; rho_b = rho_in[b], or (rho_in NULL) the density cw * sum_x src,
+ * written to rho_out when that is not NULL. */
+int fq_sur_gather(int32_t Nv, double cw, double scale, int32_t power, const double *src,
+                  int32_t nupts, int64_t ldim, int32_t neles, int64_t first_dof, int64_t ndof,
+                  const double *rho_in, double *rho_out, float *dst, void *stream);
+
+/* Conservation check and scatter of a normalised network output q[ndof][n]:
+ * defect_b = max_j |<q, phi_j>| / (|phi_j| |q|) over the collision invariants
+ * phi = (1, cx, cy, cz, |c|^2) in the cw inner product (gram_diag = |phi_j|^2,
+ * gram_inv the inverse Gram matrix, row-major 5x5).  flag_b = defect_b > tol;
+ * unflagged DOFs get Q = rho_b^2 scale (q - projection of q onto the
+ * invariants) written into dst; flagged DOFs are left untouched. */
+int fq_sur_finish(int32_t Nv, double L, double cw, double scale, double tol, const float *q,
+                  const double *rho, const double *gram_inv, const double *gram_diag, double *dst,
+                  int32_t nupts, int64_t ldim, int32_t neles, int64_t first_dof, int64_t ndof,
+                  double *defect, int32_t *flag, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FRFS_SUR_H */

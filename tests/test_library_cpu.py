@@ -23,7 +23,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _header_functions():
-    txt = open(os.path.join(ROOT, 'include', 'frfs_fq.h')).read()
+    txt = ''.join(open(os.path.join(ROOT, 'include', h)).read() for h in ('frfs_fq.h', 'frfs_sur.h'))
     txt = re.sub(r'/\*.*?\*/', '', txt, flags=re.S)
     return sorted(set(re.findall(r'\b(fq_\w+)\s*\(', txt)))
 
@@ -31,7 +31,7 @@ def _header_functions():
 def test_library_exports_every_header_symbol():
     lib = _lib.load()
     names = _header_functions()
-    assert len(names) >= 15
+    assert len(names) >= 22
     for nm in names:
         assert hasattr(lib, nm), nm
         assert nm in _lib.SIGNATURES, nm
