@@ -33,9 +33,10 @@ K_PROD_TRAFFIC_PER_DOF = {'mixed': 34.47e6, 'fp64': 84.67e6}
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--workload', default='vss32', choices=['vss32', 'vhs16', 'vss32x2', 'rhs24'],
+    ap.add_argument('--workload', default='vss32', choices=['vss32', 'vhs16', 'vss32x2', 'rhs24', 'fno16'],
                     help='vss32: the Q(f,f) headline; vhs16: plain FP64 16^3, 64 DOFs; '
-                         'vss32x2: two species, four pairs; rhs24: full SSP-RK2 step (FR advection + Q)')
+                         'vss32x2: two species, four pairs; rhs24: full SSP-RK2 step (FR advection + Q); '
+                         'fno16: the learned stand-in on the 16^3 grid, 8192 DOFs')
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--ndof', type=int, default=4096)
@@ -52,6 +53,13 @@ def parse():
     ap.add_argument('--modes', default='fp64,mixed,mixedproj')
     ap.add_argument('--sm-opt', action='append', default=[], metavar='KEY=VALUE',
                     help='extra [scattering-model] key, e.g. pair-pack=0 (repeatable)')
+    ap.add_argument('--fno-weights', default='', help='fno16: trained weights (.npz); default: train first')
+    ap.add_argument('--fno-save', default='', help='fno16: write the trained weights here')
+    ap.add_argument('--fno-pairs', type=int, default=200000)
+    ap.add_argument('--fno-epochs', type=int, default=20)
+    ap.add_argument('--fno-train-seconds', type=float, default=150.0)
+    ap.add_argument('--fno-tol', type=float, default=1e-4)
+    ap.add_argument('--fno-dofs', type=int, default=8192)
     ap.add_argument('--calib-copy-mib', type=int, default=0,
                     help='run one device-to-device copy of this size first (counter calibration)')
     return ap.parse_args()
@@ -285,6 +293,142 @@ def bench_vhs16(a):
                          'sample': f'all {ndof} DOFs, NumPy complex128 oracle, one process'}}))
 
 
+# Network forward per DOF (16^3 grid, width 32, 8^3 modes, 4 layers): per layer
+# the truncated forward transform (0.33 MFLOP per channel), the per-mode
+# channel mixing (512 x 32 x 32 complex multiply-adds, 4.19 MFLOP), the
+# inverse transform (0.33 MFLOP per channel) and the pointwise 32 x 32 layer
+# (8.39 MFLOP); lift and projection 0.26 MFLOP each.
+FNO_FLOP_PER_DOF = 4 * (32 * 327680 + 4194304 + 32 * 327680 + 8388608) + 2 * 262144
+
+
+def bench_fno16(a):
+    """The learned stand-in (vhs-gll-fno) on the 16^3 grid: 4 Fourier layers
+    of width 32 with 8^3 modes, trained on the GPU (unless --fno-weights) on
+    --fno-pairs pairs of the FP64 vhs-gll operator (omega = 0.5, M = 6,
+    Nrho = 8), then timed on 8192 held-out DOFs of seeded sheared mixtures:
+    the whole plugin call (gather, network, conservation check and
+    projection, spectral fallback on the flagged DOFs) against the spectral
+    operator on the same DOFs in FP64 and mixed precision."""
+    import torch
+    from frfs_amd import train_surrogate as ts
+    from frfs_amd.backend import HIPBackend
+    from frfs_amd.inputs import from_aosoa, shear_mixture_device, shear_mixture_params, to_aosoa
+    from frfs_amd.scattering import DGFSVHSGLLScatteringModel
+    from frfs_amd.surrogate import DGFSVHSGLLFNOScatteringModel
+    from frfs_amd.velocitymesh import DGFSVelocityMesh
+    be = HIPBackend()
+    ndof = a.fno_dofs
+    if a.fno_weights:
+        cfg = ts.config(tol=a.fno_tol, weights=a.fno_weights)
+        vm = DGFSVelocityMesh(be, cfg)
+        sm = DGFSVHSGLLFNOScatteringModel(be, cfg, vm)
+        rep = {'weights': os.path.basename(a.fno_weights)}
+    else:
+        sm, rep = ts.train(be, pairs=a.fno_pairs, epochs=a.fno_epochs, time_limit=a.fno_train_seconds,
+                           seed=a.seed, log=lambda m: print(m, file=sys.stderr, flush=True))
+        sm.tol = a.fno_tol
+        vm = sm.vm
+        if a.fno_save:
+            sm.net.save(a.fno_save)
+    n = vm.vsize()
+    f = shear_mixture_device(vm.cv(), shear_mixture_params(ndof, a.seed + 1), device=be.device)
+    fin = be.matrix((1, n, ndof))
+    to_aosoa(f, fin)
+    del f
+    qref, qs = be.matrix(fin.ioshape), be.matrix(fin.ioshape)
+
+    def timed(fn):
+        for _ in range(a.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / a.steps
+
+    dt_fp64 = timed(lambda: sm.fs_spectral(fin, qref))
+    cfgm = ts.config()
+    cfgm.set('scattering-model', 'type', 'vhs-gll')
+    cfgm.set('scattering-model', 'precision', 'mixed')
+    smm = DGFSVHSGLLScatteringModel(be, cfgm, DGFSVelocityMesh(be, cfgm))
+    qm = be.matrix(fin.ioshape)
+    dt_mixed = timed(lambda: smm.fs_batch(fin, qm))
+    stats = {}
+
+    def run():
+        stats.update(sm.fs_batch(fin, qs))
+    dt = timed(run)
+    # the network forward alone, HIP events on the stream it runs on
+    g = torch.empty((ndof, n), dtype=torch.float32, device=be.device)
+    out = torch.empty_like(g)
+    from frfs_amd import _lib
+    from frfs_amd.surrogate import F_REF, NV
+    rho = torch.empty(ndof, dtype=torch.float64, device=be.device)
+    _lib.check(_lib.load().fq_sur_gather(NV, vm.cw(), F_REF, 1, fin._as_parameter_, 1, fin.leaddim, ndof, 0, ndof,
+                                         None, rho.data_ptr(), g.data_ptr(), be.stream_handle()))
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    sm.net.forward(g, out)
+    e0.record(st)
+    for _ in range(a.steps):
+        sm.net.forward(g, out)
+    e1.record(st)
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / a.steps
+    qa, qb = from_aosoa(qs), from_aosoa(qref)
+    r = ((qa - qb).pow(2).sum(1) / qb.pow(2).sum(1)).sqrt().cpu().numpy()
+    fl = stats.get('defect', np.zeros(ndof)) > sm.tol
+    net_r = r[~fl] if (~fl).any() else np.zeros(1)
+    rm = from_aosoa(qm)
+    rmix = float(((rm - qb).pow(2).sum(1) / qb.pow(2).sum(1)).sqrt().median())
+    # untimed: the network (conservation-projected) on every DOF, no fallback
+    tol = sm.tol
+    sm.tol = 2.0
+    qn = be.matrix(fin.ioshape)
+    sm.fs_batch(fin, qn)
+    sm.tol = tol
+    rn = ((from_aosoa(qn) - qb).pow(2).sum(1) / qb.pow(2).sum(1)).sqrt().cpu().numpy()
+    dfc = stats['defect']
+    cpu = None
+    if not a.no_cpu_baseline:
+        from oracle import surrogate as osur
+        th = sm.net.theta.double().cpu().numpy()
+        gs = g[:4].double().cpu().numpy()
+        t0 = time.perf_counter()
+        osur.forward(th, gs, 16, 32, 8, sm.net.layers)
+        cs = time.perf_counter() - t0
+        cpu = {'value': 4 / cs, 'unit': 'DOF-evals/s', 'cores': 1, 'kind': 'port',
+               'sample': '4 DOFs of the same workload, network forward only, NumPy float64 oracle, one process'}
+    tflops = FNO_FLOP_PER_DOF * ndof / (fwd_ms * 1e-3) / 1e12
+    print(json.dumps({
+        'metric': 'learned stand-in Q(f,f) DOF-evals/s (with conservation check and spectral fallback), '
+                  'vhs-gll 16^3, 8192 DOFs', 'value': ndof / dt,
+        'unit': 'DOF-evals/s', 'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': dt * 1e3,
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': 'Fourier-layer stand-in (4 layers, width 32, 8^3 modes) for vhs-gll omega=0.5, '
+                               '16^3 grid on [-5,5]^3, M=6, Nrho=8; 8192 DOFs of seeded mixtures of up to 3 '
+                               'sheared Maxwellians; fallback tolerance ' + repr(sm.tol)},
+        'spectral_fp64_dof_evals_per_s': ndof / dt_fp64, 'spectral_mixed_dof_evals_per_s': ndof / dt_mixed,
+        'speedup_vs_fp64': dt_fp64 / dt, 'network_forward_ms': fwd_ms,
+        'network_only_speedup_vs_fp64': dt_fp64 / (fwd_ms * 1e-3),
+        'fallback_rate': float(fl.mean()),
+        'fallback_rate_at_tol': {t: float((dfc > t).mean()) for t in (1e-3, 3e-3, 1e-2, 3e-2)},
+        'defect': {'median': float(np.median(dfc)), 'p90': float(np.quantile(dfc, 0.9))},
+        'network_projected_rel_l2_vs_fp64_all_dofs': {'median': float(np.median(rn)),
+                                                      'p90': float(np.quantile(rn, 0.9)),
+                                                      'max': float(rn.max())},
+        'rel_l2_vs_fp64': {'network_dofs_median': float(np.median(net_r)),
+                           'network_dofs_p90': float(np.quantile(net_r, 0.9)),
+                           'network_dofs_max': float(net_r.max()), 'all_dofs_median': float(np.median(r))},
+        'spectral_mixed_rel_l2_median': rmix,
+        'training': rep,
+        'roofline': {'bound': 'fp32-valu', 'achieved': tflops, 'peak': PEAK['fp32'], 'unit': 'TFLOP/s',
+                     'frac': tflops / PEAK['fp32'], 'traffic': None,
+                     'kernel': 'network forward (all its kernels), ' + str(FNO_FLOP_PER_DOF) + ' flop per DOF'},
+        'cpu_baseline': cpu}))
+
+
 def bench_rhs(a):
     """Full explicit step: 32x32 periodic quads, p = 2 (9216 DOFs), 24^3 velocity
     grid on [-5,5]^3, vhs-gll M = 6 (the 12-point design as 6 antipodal pairs),
@@ -360,6 +504,9 @@ def main():
         return
     if a.workload == 'vss32x2':
         bench_multi(a)
+        return
+    if a.workload == 'fno16':
+        bench_fno16(a)
         return
     from frfs_amd.inputs import mixture_params
     fpar = mixture_params(a.ndof, a.seed)

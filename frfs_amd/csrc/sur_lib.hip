@@ -4,7 +4,7 @@
 // The network (16^3 grid, width 32, 8^3 retained modes, L layers):
 //   v_0      = P g + p0                                 (lift, per point)
 //   X_l(k)   = (1/n) sum_x v_l(x) e^{-2 pi i k.x/16}   (k_sur_dft, per channel)
-//   Y_l(o,k) = sum_i X_l(i,k) R_l(i,o,k)               (k_sur_mix, per mode)
+//   Y_l(o,k) = sum_i X_l(i,k) R_l(i,o,k)               (k_sur_mix_bk, per mode)
 //   s_l(x)   = Re sum_k Y_l(k) e^{+2 pi i k.x/16}       (k_sur_idft, per channel)
 //   z_l      = Wm_l v_l + b_l + s_l,  v_{l+1} = gelu(z_l) (last layer: z_l)
 //   out      = Qw . v_L + q0                            (projection)
@@ -176,46 +176,58 @@ __global__ __launch_bounds__(256) void k_sur_idft(const float2 *__restrict__ Y, 
     }
 }
 
-// Channel mixing per mode.  MODE 0: Y(b,o,k) = sum_i X(b,i,k) R(i,o,k);
-// MODE 1: gX(b,i,k) = sum_o gY(b,o,k) conj R(i,o,k);
-// MODE 2: gR(i,o,k) += sum_b conj X(b,i,k) gY(b,o,k).
-// One thread per output; k is the fastest index, so every access is coalesced.
-template <int MODE>
-__global__ __launch_bounds__(256) void k_sur_mix(const float2 *__restrict__ a, const float2 *__restrict__ bm,
-                                                 float2 *__restrict__ c, int64_t batch) {
+// Weight gradient of the mixing: gR(i,o,k) += sum_b conj X(b,i,k) gY(b,o,k).
+// One thread per (i, o, k), k fastest, so every access is coalesced.
+__global__ __launch_bounds__(256) void k_sur_mix_grad(const float2 *__restrict__ X, const float2 *__restrict__ gY,
+                                                      float2 *__restrict__ gR, int64_t batch) {
     const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    const int64_t total = MODE == 2 ? int64_t(SW) * SW * SM3 : batch * SW * SM3;
-    if (idx >= total) return;
+    if (idx >= int64_t(SW) * SW * SM3) return;
     const int k = int(idx & (SM3 - 1));
-    const int q = int((idx >> 9) & (SW - 1));
-    const int64_t hi = idx >> 14;
+    const int o = int((idx >> 9) & (SW - 1));
+    const int i = int(idx >> 14);
     float2 acc = make_float2(0.f, 0.f);
-    if constexpr (MODE == 0) {        // a = X, bm = R, q = o, hi = b
-        for (int i = 0; i < SW; ++i) {
-            const float2 x = a[(hi * SW + i) * SM3 + k];
-            const float2 r = bm[(int64_t(i) * SW + q) * SM3 + k];
-            acc.x += x.x * r.x - x.y * r.y;
-            acc.y += x.x * r.y + x.y * r.x;
-        }
-        c[idx] = acc;
-    } else if constexpr (MODE == 1) { // a = gY, bm = R, q = i, hi = b
-        for (int o = 0; o < SW; ++o) {
-            const float2 g = a[(hi * SW + o) * SM3 + k];
-            const float2 r = bm[(int64_t(q) * SW + o) * SM3 + k];
-            acc.x += g.x * r.x + g.y * r.y;
-            acc.y += g.y * r.x - g.x * r.y;
-        }
-        c[idx] = acc;
-    } else {                          // a = X, bm = gY, q = o, hi = i
-        for (int64_t b = 0; b < batch; ++b) {
-            const float2 x = a[(b * SW + hi) * SM3 + k];
-            const float2 g = bm[(b * SW + q) * SM3 + k];
-            acc.x += x.x * g.x + x.y * g.y;
-            acc.y += x.x * g.y - x.y * g.x;
-        }
-        c[idx].x += acc.x;
-        c[idx].y += acc.y;
+    for (int64_t b = 0; b < batch; ++b) {
+        const float2 x = X[(b * SW + i) * SM3 + k];
+        const float2 g = gY[(b * SW + o) * SM3 + k];
+        acc.x += x.x * g.x + x.y * g.y;
+        acc.y += x.x * g.y - x.y * g.x;
     }
+    gR[idx].x += acc.x;
+    gR[idx].y += acc.y;
+}
+
+// The forward and input-gradient mixing with one thread per (DOF, mode):
+// MODE 0: Y(b,o,k) = sum_i X(b,i,k) R(i,o,k);
+// MODE 1: gX(b,i,k) = sum_o gY(b,o,k) conj R(i,o,k).
+// A workgroup is 64 consecutive modes (one per lane) x 4 DOFs (one per wave):
+// each input value is read once, every output written once, and the four
+// waves read the same R lines, so R comes from L2 once per workgroup.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_sur_mix_bk(const float2 *__restrict__ a, const float2 *__restrict__ R,
+                                                    float2 *__restrict__ c, int64_t batch) {
+    const int k = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int64_t b = int64_t(blockIdx.y) * 4 + (threadIdx.x >> 6);
+    if (b >= batch) return;
+    float2 acc[SW];
+#pragma unroll
+    for (int j = 0; j < SW; ++j) acc[j] = make_float2(0.f, 0.f);
+    for (int j = 0; j < SW; ++j) {
+        const float2 x = a[(b * SW + j) * SM3 + k];
+#pragma unroll
+        for (int q = 0; q < SW; ++q) {
+            if constexpr (MODE == 0) {      // j = i, q = o
+                const float2 r = R[(int64_t(j) * SW + q) * SM3 + k];
+                acc[q].x += x.x * r.x - x.y * r.y;
+                acc[q].y += x.x * r.y + x.y * r.x;
+            } else {                        // j = o, q = i
+                const float2 r = R[(int64_t(q) * SW + j) * SM3 + k];
+                acc[q].x += x.x * r.x + x.y * r.y;
+                acc[q].y += x.y * r.x - x.x * r.y;
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < SW; ++j) c[(b * SW + j) * SM3 + k] = acc[j];
 }
 
 // v_0(b, c, x) = P_c g(b, x) + p0_c
@@ -579,7 +591,8 @@ int fq_sur_forward(const fq_sur_shape *shape, const float *theta, const float *g
         const float *lay = theta + 2 * SW + l * SLAYER;
         const float2 *R = reinterpret_cast<const float2 *>(lay);
         hipLaunchKernelGGL(k_sur_dft, dim3(slices), dim3(256), 0, st, p.v[l], 1.0f / SN3, p.X[l]);
-        hipLaunchKernelGGL(k_sur_mix<0>, dim3(nblk(batch * SW * SM3)), dim3(256), 0, st, p.X[l], R, p.Y, batch);
+        hipLaunchKernelGGL(k_sur_mix_bk<0>, dim3(SM3 / 64, unsigned((batch + 3) / 4)), dim3(256), 0, st, p.X[l], R,
+                           p.Y, batch);
         hipLaunchKernelGGL(k_sur_idft, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, p.s);
         hipLaunchKernelGGL(k_sur_pw, dim3(nblk(pts)), dim3(256), 0, st, p.v[l], p.s, lay + SR, int(l < L - 1),
                            tr ? p.z[l] : nullptr, p.v[l + 1], pts);
@@ -615,8 +628,10 @@ int fq_sur_backward(const fq_sur_shape *shape, const float *theta, const float *
         hipLaunchKernelGGL((k_sur_wgrad<SW, SW>), wg, dim3(256), 0, st, gv, p.v[l], grad + lo + SR,
                            grad + lo + SR + SW * SW);
         hipLaunchKernelGGL(k_sur_dft, dim3(slices), dim3(256), 0, st, gv, 1.0f, gY);
-        hipLaunchKernelGGL(k_sur_mix<2>, dim3(nblk(int64_t(SW) * SW * SM3)), dim3(256), 0, st, p.X[l], gY, gR, batch);
-        hipLaunchKernelGGL(k_sur_mix<1>, dim3(nblk(batch * SW * SM3)), dim3(256), 0, st, gY, R, gX, batch);
+        hipLaunchKernelGGL(k_sur_mix_grad, dim3(nblk(int64_t(SW) * SW * SM3)), dim3(256), 0, st, p.X[l], gY, gR,
+                           batch);
+        hipLaunchKernelGGL(k_sur_mix_bk<1>, dim3(SM3 / 64, unsigned((batch + 3) / 4)), dim3(256), 0, st, gY, R, gX,
+                           batch);
         hipLaunchKernelGGL(k_sur_idft, dim3(slices), dim3(256), 0, st, gX, 1.0f / SN3, 1, gvin);
         float *tmp = gv;
         gv = gvin;

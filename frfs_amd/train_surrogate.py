@@ -1,0 +1,111 @@
+"""Train the learned stand-in on the GPU with the library's own kernels.
+
+    python -m frfs_amd.train_surrogate --pairs 200000 --epochs 20 --out fno16.npz
+
+Training pairs come from the spectral operator itself: seeded mixtures of up
+to three sheared Maxwellians (``inputs.shear_mixture_params``) on the 16^3
+grid, Q(f) from ``vhs-gll`` in FP64 (omega = 0.5, M = 6, Nrho = 8,
+[-5, 5]^3: the operator of frfs/solvers/dgfs/scattering.py:227-286 at the
+reference's hard-sphere settings).  Forward, backward, loss and Adam run in
+the HIP kernels of include/frfs_sur.h; PyTorch only holds the device arrays.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+from .inifile import Inifile
+
+ARGON = dict(T0=273.0, H0=1e-3, rho0=0.00011140244750324529, molarMass0=39.96e-3)
+
+
+def config(tol=1e-4, weights='', layers=4):
+    """vhs-gll-fno on the 16^3 grid (hard spheres, M = 6, Nrho = 8)."""
+    return Inifile.from_dict({
+        'non-dim': ARGON,
+        'constants': {'Nv': 16, 'NvBatchSize': 16 ** 3, 'Nrho': 8},
+        'velocity-mesh': {'dev': 5.0, 'cmax': 0, 'Tmax': 273},
+        'spherical-design-rule': {'ssrule': 'womersley', 'M': 6},
+        'scattering-model': {'type': 'vhs-gll-fno', 'omega': 0.5, 'dRef': 4.17e-10, 'Tref': 273,
+                             'precision': 'double', 'surrogate-tol': tol, 'surrogate-weights': weights,
+                             'surrogate-layers': layers}})
+
+
+def make_pairs(sm, ndof, seed, chunk=8192):
+    """(g, Q / rho^2, rho) device tensors for ndof seeded sheared mixtures."""
+    import torch
+    from .inputs import shear_mixture_device, shear_mixture_params
+    from .surrogate import training_pairs
+    n = sm.vm.vsize()
+    dev = sm.backend.device
+    g = torch.empty((ndof, n), dtype=torch.float32, device=dev)
+    t = torch.empty_like(g)
+    rho = torch.empty(ndof, dtype=torch.float64, device=dev)
+    prm = shear_mixture_params(ndof, seed)
+    for d0 in range(0, ndof, chunk):
+        f = shear_mixture_device(sm.vm.cv(), prm[d0:d0 + chunk], device=dev)
+        g[d0:d0 + chunk], t[d0:d0 + chunk], rho[d0:d0 + chunk] = training_pairs(sm, f, chunk)
+    return g, t, rho
+
+
+def rel_l2_rows(p, t):
+    return ((p - t).pow(2).sum(1) / t.pow(2).sum(1)).sqrt()
+
+
+def train(backend, *, pairs=200000, heldout=2048, epochs=20, batch=64, lr=2e-3, seed=0, time_limit=None,
+          layers=4, log=print):
+    """Returns (model, report).  The model's network is trained in place."""
+    import torch
+    from .surrogate import DGFSVHSGLLFNOScatteringModel
+    from .velocitymesh import DGFSVelocityMesh
+    cfg = config(layers=layers)
+    vm = DGFSVelocityMesh(backend, cfg)
+    sm = DGFSVHSGLLFNOScatteringModel(backend, cfg, vm)
+    t0 = time.time()
+    g, t, _ = make_pairs(sm, pairs + heldout, seed)
+    torch.cuda.synchronize()
+    t_gen = time.time() - t0
+    q_ref = float(t[:pairs].pow(2).mean().sqrt())
+    t /= q_ref
+    sm.net.q_ref = q_ref
+    before = float(rel_l2_rows(sm.net.predict(g[pairs:]), t[pairs:]).median())
+    t0 = time.time()
+    hist = sm.net.fit(g[:pairs], t[:pairs], epochs=epochs, batch=batch, lr=lr, seed=seed,
+                      time_limit=time_limit, log=log)
+    torch.cuda.synchronize()
+    t_fit = time.time() - t0
+    r = rel_l2_rows(sm.net.predict(g[pairs:]), t[pairs:])
+    rep = {'pairs': pairs, 'heldout': heldout, 'epochs_run': len(hist), 'batch': batch, 'lr': lr,
+           'layers': layers, 'generate_s': round(t_gen, 2), 'train_s': round(t_fit, 2),
+           'train_rel_sq_err': hist[-1] if hist else None,
+           'heldout_rel_l2_untrained_median': before,
+           'heldout_rel_l2_median': float(r.median()), 'heldout_rel_l2_p90': float(r.quantile(0.9)),
+           'heldout_rel_l2_max': float(r.max())}
+    del g, t
+    return sm, rep
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--pairs', type=int, default=200000)
+    ap.add_argument('--heldout', type=int, default=2048)
+    ap.add_argument('--epochs', type=int, default=20)
+    ap.add_argument('--batch', type=int, default=64)
+    ap.add_argument('--lr', type=float, default=2e-3)
+    ap.add_argument('--layers', type=int, default=4)
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--time-limit', type=float, default=None, help='seconds; stops after the running epoch')
+    ap.add_argument('--out', required=True)
+    a = ap.parse_args()
+    from .backend import HIPBackend
+    be = HIPBackend()
+    sm, rep = train(be, pairs=a.pairs, heldout=a.heldout, epochs=a.epochs, batch=a.batch, lr=a.lr,
+                    seed=a.seed, time_limit=a.time_limit, layers=a.layers,
+                    log=lambda s: print(s, flush=True))
+    sm.net.save(a.out)
+    print(json.dumps(rep), flush=True)
+
+
+if __name__ == '__main__':
+    main()
