@@ -1513,9 +1513,15 @@ __device__ __forceinline__ void phis(int64_t v, int N, double L, double *ph) {
     ph[0] = 1; ph[1] = cx; ph[2] = cy; ph[3] = cz; ph[4] = cx * cx + cy * cy + cz * cz;
 }
 
+// One workgroup per DOF.  Workgroups are dealt round-robin to the eight XCDs,
+// so workgroup w takes slot (w % 8) * per + w / 8: each XCD works through a
+// contiguous DOF range, and the DOFs whose values share an AoSoA cache line
+// (neighbouring elements) meet in the same L2.
 __global__ __launch_bounds__(256) void k_project(AosoaPtrs ptr, Dof dof, int ns, int N, double L,
-                                                 const double *ginv, const double *mass) {
-    const int slot = blockIdx.x;
+                                                 const double *ginv, const double *mass, int D) {
+    const int per = (D + 7) / 8;
+    const int slot = int(blockIdx.x & 7) * per + int(blockIdx.x >> 3);
+    if (slot >= D) return;
     const int64_t b = dof.base(slot);
     const int64_t n = int64_t(N) * N * N;
     const int K = ns + 4;
@@ -1994,8 +2000,8 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     post(5);
     if (h->flags & FQ_FLAG_PROJECT) {
         pre();
-        hipLaunchKernelGGL(k_project, dim3(unsigned(D)), dim3(256), 0, st, ptr, dof, ntgt, N, P.L, h->ginv,
-                           h->pmass);
+        hipLaunchKernelGGL(k_project, dim3(unsigned(8 * ((D + 7) / 8))), dim3(256), 0, st, ptr, dof, ntgt, N, P.L,
+                           h->ginv, h->pmass, int(D));
         post(6);
     }
     hipError_t e = hipGetLastError();
