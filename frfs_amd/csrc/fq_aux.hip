@@ -65,9 +65,15 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
 }
 
 __global__ __launch_bounds__(256) void k_moments(int N, double L, double cw, double mr, const double *__restrict__ f,
-                                                 int nupts, int64_t ldim, int64_t first, double *__restrict__ out) {
+                                                 int nupts, int64_t ldim, int64_t first, double *__restrict__ out,
+                                                 int64_t ndof) {
     __shared__ double red[256];
-    const int64_t dof = first + blockIdx.x;
+    // workgroups are dealt round-robin to the eight XCDs: each XCD takes a
+    // contiguous DOF range, so neighbouring elements (one AoSoA line) share an L2
+    const int64_t per = (ndof + 7) / 8;
+    const int64_t slot = int64_t(blockIdx.x & 7) * per + int64_t(blockIdx.x >> 3);
+    if (slot >= ndof) return;
+    const int64_t dof = first + slot;
     const int64_t elem = dof / nupts, upt = dof % nupts;
     const int64_t nvol = int64_t(N) * N * N;
     const int64_t base = upt * ldim + (elem / 32) * nvol * 32 + elem % 32;
@@ -98,7 +104,7 @@ __global__ __launch_bounds__(256) void k_moments(int N, double L, double cw, dou
     double r[10];
     for (int k = 0; k < 10; ++k) r[k] = block_sum(m[k], red);
     if (threadIdx.x == 0) {
-        double *o = out + int64_t(blockIdx.x) * FQ_NMOMENTS;
+        double *o = out + slot * FQ_NMOMENTS;
         o[0] = rho; o[1] = ux; o[2] = uy; o[3] = uz;
         o[4] = r[0] * (2.0 / 3.0 * mcw * mr) / rho;
         o[5] = mr * r[1] * mcw; o[6] = mr * r[2] * mcw; o[7] = mr * r[3] * mcw;
@@ -233,8 +239,9 @@ int fq_moments(int32_t Nv, double L, double cw, double mass_ratio, const double 
     const int64_t nparr = (int64_t(neles) + 31) / 32 * 32;
     if (ldim < int64_t(Nv) * Nv * Nv * nparr) return fq_internal_fail(FQ_ERR_INVALID, "fq_moments: ldim too small");
     if (ndof == 0) return FQ_SUCCESS;
-    hipLaunchKernelGGL(k_moments, dim3(unsigned(ndof)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), Nv, L,
-                       cw, mass_ratio, f, nupts, ldim, first_dof, out);
+    hipLaunchKernelGGL(k_moments, dim3(unsigned(8 * ((ndof + 7) / 8))), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), Nv, L, cw, mass_ratio, f, nupts, ldim, first_dof, out,
+                       ndof);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? FQ_SUCCESS : fq_internal_fail(FQ_ERR_HIP, std::string("fq_moments: ") + hipGetErrorString(e));
 }

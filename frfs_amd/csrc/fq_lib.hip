@@ -67,6 +67,14 @@ template <int N> struct Geo {
     static constexpr int LDSROW = N + 1;      // padded LDS row (complex elements)
 };
 
+// Workgroups are dealt round-robin to the eight XCDs.  A kernel launched with
+// a grid that is a multiple of 8 runs block w as block (w % 8) * (grid / 8) +
+// w / 8, so each XCD takes a contiguous block range and neighbouring DOFs,
+// whose AoSoA values share cache lines, meet in one L2.
+__device__ __forceinline__ int64_t xcd_block() {
+    return int64_t(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+}
+
 constexpr int kPencilThreads = 256;           // pencil kernels (no LDS)
 constexpr int kPlaneThreads = 128;            // plane kernels (LDS transpose per unit)
 
@@ -193,7 +201,7 @@ __global__ __launch_bounds__(256) void k_fwd_z(AosoaPtrs ptr, Dof dof, cpx<doubl
     using G = Geo<N>;
     constexpr int E = G::E;
     const int lt = threadIdx.x % G::LPP;
-    const int64_t unit = int64_t(blockIdx.x) * (blockDim.x / G::LPP) + threadIdx.x / G::LPP;
+    const int64_t unit = xcd_block() * (blockDim.x / G::LPP) + threadIdx.x / G::LPP;
     if (unit >= nunits) return;   // whole unit exits together (no barriers here)
     const int I = int(unit % N);
     const int64_t ds = unit / N;
@@ -358,7 +366,7 @@ __global__ __launch_bounds__(kPlaneThreads) void k_plane(const cpx<T> *Z, cpx<T>
     __shared__ cpx<T> lds[UPB][N * G::LDSROW];
     const int u = threadIdx.x / G::LPP;
     const int lt = threadIdx.x % G::LPP;
-    const int64_t unit = int64_t(blockIdx.x) * UPB + u;
+    const int64_t unit = xcd_block() * UPB + u;
     const bool valid = unit < nunits;
     const int vz = int(unit % N);
     const int64_t dg = unit / N;
@@ -1891,11 +1899,12 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     };
     auto blocks = [&](int64_t units) { return unsigned((units + upb - 1) / upb); };
     auto pblocks = [&](int64_t units) { return unsigned((units + upp - 1) / upp); };
+    auto xcd8 = [](unsigned nb) { return (nb + 7) / 8 * 8; };    // grids of xcd_block() kernels
     const dim3 bt(kPencilThreads), pt(kPlaneThreads);
     int64_t u;
     u = int64_t(D) * nsrc * N;
     pre();
-    hipLaunchKernelGGL(k_fwd_z<N>, dim3(blocks(u)), bt, 0, st, ptr, dof, T0, nsrc, u);
+    hipLaunchKernelGGL(k_fwd_z<N>, dim3(xcd8(blocks(u))), bt, 0, st, ptr, dof, T0, nsrc, u);
     post(0);
     // the fused FP32 path reads an FP32 copy of F, written by k_fwd_xy itself
     const bool want32 = h->fused && std::is_same<T, float>::value;
@@ -1996,7 +2005,7 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
                            h->t_cn, h->l_src, h->t_l0, h->t_ln, int(P.contribs.size()), Y, ngroups, nsrc, ntgt, D, u);
     post(4);
     pre();
-    hipLaunchKernelGGL(k_out<N>, dim3(pblocks(u)), pt, 0, st, Y, ptr, dof, ntgt, u);
+    hipLaunchKernelGGL(k_out<N>, dim3(xcd8(pblocks(u))), pt, 0, st, Y, ptr, dof, ntgt, u);
     post(5);
     if (h->flags & FQ_FLAG_PROJECT) {
         pre();

@@ -4,7 +4,7 @@
 // The network (16^3 grid, width 32, 8^3 retained modes, L layers):
 //   v_0      = P g + p0                                 (lift, per point)
 //   X_l(k)   = (1/n) sum_x v_l(x) e^{-2 pi i k.x/16}   (k_sur_dft, per channel)
-//   Y_l(o,k) = sum_i X_l(i,k) R_l(i,o,k)               (k_sur_mix_bk, per mode)
+//   Y_l(o,k) = sum_i X_l(i,k) R_l(i,o,k)               (k_sur_mix_lds, per mode)
 //   s_l(x)   = Re sum_k Y_l(k) e^{+2 pi i k.x/16}       (k_sur_idft, per channel)
 //   z_l      = Wm_l v_l + b_l + s_l,  v_{l+1} = gelu(z_l) (last layer: z_l)
 //   out      = Qw . v_L + q0                            (projection)
@@ -196,18 +196,25 @@ __global__ __launch_bounds__(256) void k_sur_mix_grad(const float2 *__restrict__
     gR[idx].y += acc.y;
 }
 
-// The forward and input-gradient mixing with one thread per (DOF, mode):
+// The forward and input-gradient mixing:
 // MODE 0: Y(b,o,k) = sum_i X(b,i,k) R(i,o,k);
 // MODE 1: gX(b,i,k) = sum_o gY(b,o,k) conj R(i,o,k).
-// A workgroup is 64 consecutive modes (one per lane) x 4 DOFs (one per wave):
-// each input value is read once, every output written once, and the four
-// waves read the same R lines, so R comes from L2 once per workgroup.
+// A workgroup takes 8 consecutive modes and 32 DOFs and stages the 8 modes'
+// 32 x 32 weights in LDS (64 KB) once; thread (mode m = t % 8, DOF d = t / 8)
+// keeps its 32 outputs in registers.  Within a wave the eight DOFs of a mode
+// read the same LDS word (a broadcast), so the weight reads are conflict-free.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_sur_mix_bk(const float2 *__restrict__ a, const float2 *__restrict__ R,
-                                                    float2 *__restrict__ c, int64_t batch) {
-    const int k = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int64_t b = int64_t(blockIdx.y) * 4 + (threadIdx.x >> 6);
+__global__ __launch_bounds__(256) void k_sur_mix_lds(const float2 *__restrict__ a, const float2 *__restrict__ R,
+                                                     float2 *__restrict__ c, int64_t batch) {
+    __shared__ float2 Rl[SW * SW * 8];               // [(i*32 + o)*8 + m]
+    const int t = threadIdx.x;
+    const int k0 = blockIdx.x * 8;
+    for (int e = t; e < SW * SW * 8; e += 256) Rl[e] = R[int64_t(e >> 3) * SM3 + k0 + (e & 7)];
+    __syncthreads();
+    const int m = t & 7;
+    const int64_t b = int64_t(blockIdx.y) * 32 + (t >> 3);
     if (b >= batch) return;
+    const int k = k0 + m;
     float2 acc[SW];
 #pragma unroll
     for (int j = 0; j < SW; ++j) acc[j] = make_float2(0.f, 0.f);
@@ -216,11 +223,11 @@ __global__ __launch_bounds__(256) void k_sur_mix_bk(const float2 *__restrict__ a
 #pragma unroll
         for (int q = 0; q < SW; ++q) {
             if constexpr (MODE == 0) {      // j = i, q = o
-                const float2 r = R[(int64_t(j) * SW + q) * SM3 + k];
+                const float2 r = Rl[(j * SW + q) * 8 + m];
                 acc[q].x += x.x * r.x - x.y * r.y;
                 acc[q].y += x.x * r.y + x.y * r.x;
             } else {                        // j = o, q = i
-                const float2 r = R[(int64_t(q) * SW + j) * SM3 + k];
+                const float2 r = Rl[(q * SW + j) * 8 + m];
                 acc[q].x += x.x * r.x + x.y * r.y;
                 acc[q].y += x.y * r.x - x.x * r.y;
             }
@@ -324,7 +331,9 @@ __global__ __launch_bounds__(256) void k_sur_pw_bwd(float *__restrict__ gv, cons
 }
 
 // G(o, i) += sum_{b, x} A(b, o, x) V(b, i, x);  gb(o) += sum_{b, x} A(b, o, x).
-// Grid (16^3 / 512, batch): each workgroup sums 512 points in LDS tiles of 64.
+// Grid 8 * batch: workgroup w sums the 512 points from (w % 8) * 512 of DOF
+// w / 8 in LDS tiles of 64.
+static_assert(SN3 / 512 == 8, "k_sur_wgrad splits a DOF into eight 512-point tiles");
 template <int COUT, int CIN>
 __global__ __launch_bounds__(256) void k_sur_wgrad(const float *__restrict__ A, const float *__restrict__ V,
                                                    float *__restrict__ G, float *__restrict__ gb) {
@@ -333,8 +342,8 @@ __global__ __launch_bounds__(256) void k_sur_wgrad(const float *__restrict__ A, 
     __shared__ float As[COUT][65];
     __shared__ float Vs[CIN][65];
     const int t = threadIdx.x;
-    const int64_t b = blockIdx.y;
-    const int x0 = blockIdx.x * 512;
+    const int64_t b = blockIdx.x >> 3;
+    const int x0 = int(blockIdx.x & 7) * 512;
     float acc[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) acc[q] = 0.f;
@@ -367,6 +376,17 @@ __global__ __launch_bounds__(256) void k_sur_wgrad(const float *__restrict__ A, 
     }
     if (t < COUT) atomicAdd(&gb[t], accb);
 }
+
+// Per-DOF workgroups are dealt round-robin to the eight XCDs; workgroup w
+// takes DOF (w % 8) * per + w / 8 so that each XCD works through a contiguous
+// DOF range and neighbouring elements, which share AoSoA cache lines, meet in
+// one L2.  Launch 8 * per workgroups; the result is count or more when idle.
+__device__ __forceinline__ int64_t xcd_slot(int64_t count) {
+    const int64_t per = (count + 7) / 8;
+    return int64_t(blockIdx.x & 7) * per + int64_t(blockIdx.x >> 3);
+}
+
+inline unsigned xcd_grid(int64_t count) { return unsigned(8 * ((count + 7) / 8)); }
 
 __device__ __forceinline__ double block_sum(double v, double *red) {
     red[threadIdx.x] = v;
@@ -419,9 +439,11 @@ __global__ __launch_bounds__(256) void k_sur_adam(float *__restrict__ th, const 
 __global__ __launch_bounds__(256) void k_sur_gather(int n, double cw, double scale, int power,
                                                     const double *__restrict__ src, int nupts, int64_t ldim,
                                                     int64_t first, const double *__restrict__ rho_in,
-                                                    double *__restrict__ rho_out, float *__restrict__ dst) {
+                                                    double *__restrict__ rho_out, float *__restrict__ dst,
+                                                    int64_t ndof) {
     __shared__ double red[256];
-    const int64_t b = blockIdx.x;
+    const int64_t b = xcd_slot(ndof);
+    if (b >= ndof) return;
     const int64_t dof = first + b;
     const int64_t e = dof / nupts;
     const int u = int(dof % nupts);
@@ -455,10 +477,11 @@ __global__ __launch_bounds__(256) void k_sur_finish(int N, double c0, double dc,
                                                     const float *__restrict__ q, const double *__restrict__ rho,
                                                     SurGram gm, double *__restrict__ dst, int nupts, int64_t ldim,
                                                     int64_t first, double *__restrict__ defect,
-                                                    int32_t *__restrict__ flag) {
+                                                    int32_t *__restrict__ flag, int64_t ndof) {
     __shared__ double red[256];
     const int n = N * N * N;
-    const int64_t b = blockIdx.x;
+    const int64_t b = xcd_slot(ndof);
+    if (b >= ndof) return;
     const float *qb = q + b * n;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     for (int x = threadIdx.x; x < n; x += 256) {
@@ -591,8 +614,8 @@ int fq_sur_forward(const fq_sur_shape *shape, const float *theta, const float *g
         const float *lay = theta + 2 * SW + l * SLAYER;
         const float2 *R = reinterpret_cast<const float2 *>(lay);
         hipLaunchKernelGGL(k_sur_dft, dim3(slices), dim3(256), 0, st, p.v[l], 1.0f / SN3, p.X[l]);
-        hipLaunchKernelGGL(k_sur_mix_bk<0>, dim3(SM3 / 64, unsigned((batch + 3) / 4)), dim3(256), 0, st, p.X[l], R,
-                           p.Y, batch);
+        hipLaunchKernelGGL(k_sur_mix_lds<0>, dim3(SM3 / 8, unsigned((batch + 31) / 32)), dim3(256), 0, st, p.X[l],
+                           R, p.Y, batch);
         hipLaunchKernelGGL(k_sur_idft, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, p.s);
         hipLaunchKernelGGL(k_sur_pw, dim3(nblk(pts)), dim3(256), 0, st, p.v[l], p.s, lay + SR, int(l < L - 1),
                            tr ? p.z[l] : nullptr, p.v[l + 1], pts);
@@ -615,7 +638,7 @@ int fq_sur_backward(const fq_sur_shape *shape, const float *theta, const float *
     float *gv = work, *gvin = work + vol;
     float2 *gY = reinterpret_cast<float2 *>(work + 2 * vol);
     float2 *gX = gY + batch * SW * SM3;
-    const dim3 wg(SN3 / 512, unsigned(batch));
+    const dim3 wg(unsigned(batch * (SN3 / 512)));
     const int64_t qoff = 2 * SW + L * SLAYER;
     hipLaunchKernelGGL((k_sur_wgrad<1, SW>), wg, dim3(256), 0, st, gout, p.v[L], grad + qoff, grad + qoff + SW);
     hipLaunchKernelGGL(k_sur_proj_bwd, dim3(nblk(pts)), dim3(256), 0, st, gout, theta + qoff, gv, pts);
@@ -630,8 +653,8 @@ int fq_sur_backward(const fq_sur_shape *shape, const float *theta, const float *
         hipLaunchKernelGGL(k_sur_dft, dim3(slices), dim3(256), 0, st, gv, 1.0f, gY);
         hipLaunchKernelGGL(k_sur_mix_grad, dim3(nblk(int64_t(SW) * SW * SM3)), dim3(256), 0, st, p.X[l], gY, gR,
                            batch);
-        hipLaunchKernelGGL(k_sur_mix_bk<1>, dim3(SM3 / 64, unsigned((batch + 3) / 4)), dim3(256), 0, st, gY, R, gX,
-                           batch);
+        hipLaunchKernelGGL(k_sur_mix_lds<1>, dim3(SM3 / 8, unsigned((batch + 31) / 32)), dim3(256), 0, st, gY, R,
+                           gX, batch);
         hipLaunchKernelGGL(k_sur_idft, dim3(slices), dim3(256), 0, st, gX, 1.0f / SN3, 1, gvin);
         float *tmp = gv;
         gv = gvin;
@@ -672,8 +695,8 @@ int fq_sur_gather(int32_t Nv, double cw, double scale, int32_t power, const doub
     const int64_t nparr = (int64_t(neles) + 31) / 32 * 32;
     if (ldim < int64_t(Nv) * Nv * Nv * nparr) return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_gather: ldim too small");
     if (ndof == 0) return FQ_SUCCESS;
-    hipLaunchKernelGGL(k_sur_gather, dim3(unsigned(ndof)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
-                       Nv * Nv * Nv, cw, scale, power, src, nupts, ldim, first_dof, rho_in, rho_out, dst);
+    hipLaunchKernelGGL(k_sur_gather, dim3(xcd_grid(ndof)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       Nv * Nv * Nv, cw, scale, power, src, nupts, ldim, first_dof, rho_in, rho_out, dst, ndof);
     return hip_status("fq_sur_gather");
 }
 
@@ -689,8 +712,9 @@ int fq_sur_finish(int32_t Nv, double L, double cw, double scale, double tol, con
     SurGram gm;
     for (int j = 0; j < 25; ++j) gm.ginv[j] = gram_inv[j];
     for (int j = 0; j < 5; ++j) gm.gdiag[j] = gram_diag[j];
-    hipLaunchKernelGGL(k_sur_finish, dim3(unsigned(ndof)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), Nv,
-                       -L + L / Nv, 2 * L / Nv, cw, scale, tol, q, rho, gm, dst, nupts, ldim, first_dof, defect, flag);
+    hipLaunchKernelGGL(k_sur_finish, dim3(xcd_grid(ndof)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), Nv,
+                       -L + L / Nv, 2 * L / Nv, cw, scale, tol, q, rho, gm, dst, nupts, ldim, first_dof, defect, flag,
+                       ndof);
     return hip_status("fq_sur_finish");
 }
 
