@@ -364,9 +364,10 @@ def bench_fno16(a):
     out = torch.empty_like(g)
     from frfs_amd import _lib
     from frfs_amd.surrogate import F_REF, NV
+    lib, bst = _lib.load(), be.stream_handle()
     rho = torch.empty(ndof, dtype=torch.float64, device=be.device)
-    _lib.check(_lib.load().fq_sur_gather(NV, vm.cw(), F_REF, 1, fin._as_parameter_, 1, fin.leaddim, ndof, 0, ndof,
-                                         None, rho.data_ptr(), g.data_ptr(), be.stream_handle()))
+    _lib.check(lib.fq_sur_gather(NV, vm.cw(), F_REF, 1, fin._as_parameter_, 1, fin.leaddim, ndof, 0, ndof, None,
+                                 rho.data_ptr(), g.data_ptr(), bst))
     st = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     sm.net.forward(g, out)
@@ -390,6 +391,20 @@ def bench_fno16(a):
     sm.tol = tol
     rn = ((from_aosoa(qn) - qb).pow(2).sum(1) / qb.pow(2).sum(1)).sqrt().cpu().numpy()
     dfc = stats['defect']
+    # the teacher's own conservation defect, and its projection, through the
+    # same check kernel: Q_fp64 / (rho^2 q_ref) -> defect and projected Q
+    gq = torch.empty((ndof, n), dtype=torch.float32, device=be.device)
+    _lib.check(lib.fq_sur_gather(NV, vm.cw(), sm.net.q_ref, 2, qref._as_parameter_, 1, qref.leaddim, ndof, 0, ndof,
+                                 rho.data_ptr(), None, gq.data_ptr(), bst))
+    qrp = be.matrix(fin.ioshape)
+    tdef = torch.empty(ndof, dtype=torch.float64, device=be.device)
+    tflag = torch.empty(ndof, dtype=torch.int32, device=be.device)
+    _lib.check(lib.fq_sur_finish(NV, vm.L(), vm.cw(), sm.net.q_ref, 2.0, gq.data_ptr(), rho.data_ptr(),
+                                 sm._ginv.data_ptr(), sm._gdiag.data_ptr(), qrp._as_parameter_, 1, qrp.leaddim, ndof,
+                                 0, ndof, tdef.data_ptr(), tflag.data_ptr(), bst))
+    qbp = from_aosoa(qrp)
+    rnp = ((from_aosoa(qn) - qbp).pow(2).sum(1) / qbp.pow(2).sum(1)).sqrt().cpu().numpy()
+    tdef = tdef.cpu().numpy()
     cpu = None
     if not a.no_cpu_baseline:
         from oracle import surrogate as osur
@@ -418,6 +433,11 @@ def bench_fno16(a):
         'network_projected_rel_l2_vs_fp64_all_dofs': {'median': float(np.median(rn)),
                                                       'p90': float(np.quantile(rn, 0.9)),
                                                       'max': float(rn.max())},
+        'network_projected_rel_l2_vs_fp64_projected_all_dofs': {'median': float(np.median(rnp)),
+                                                                'p90': float(np.quantile(rnp, 0.9)),
+                                                                'max': float(rnp.max())},
+        'spectral_fp64_defect': {'median': float(np.median(tdef)), 'p90': float(np.quantile(tdef, 0.9)),
+                                 'fraction_above_tol': float((tdef > sm.tol).mean())},
         'rel_l2_vs_fp64': {'network_dofs_median': float(np.median(net_r)),
                            'network_dofs_p90': float(np.quantile(net_r, 0.9)),
                            'network_dofs_max': float(net_r.max()), 'all_dofs_median': float(np.median(r))},

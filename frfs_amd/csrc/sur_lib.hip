@@ -44,6 +44,23 @@ __device__ __forceinline__ void load_tw(float2 *tw) {
     }
 }
 
+// cos(2 pi j / 16); sin(2 pi j / 16) = cos(2 pi (j - 4) / 16).  Called with
+// compile-time j (unrolled loops) the values fold into the instructions.
+__device__ __forceinline__ float cos16(int j) {
+    switch (j & 15) {
+    case 0: return 1.0f;
+    case 1: case 15: return 0.92387953251128676f;
+    case 2: case 14: return 0.70710678118654752f;
+    case 3: case 13: return 0.38268343236508977f;
+    case 4: case 12: return 0.0f;
+    case 5: case 11: return -0.38268343236508977f;
+    case 6: case 10: return -0.70710678118654752f;
+    case 7: case 9: return -0.92387953251128676f;
+    default: return -1.0f;
+    }
+}
+__device__ __forceinline__ float sin16(int j) { return cos16(j - 4); }
+
 __device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_grad(float z) {
     return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
@@ -76,28 +93,32 @@ __global__ __launch_bounds__(256) void k_sur_dft(const float *__restrict__ v, fl
             float re = 0.f, im = 0.f;
 #pragma unroll
             for (int K = 0; K < SN; ++K) {
-                const float2 w = tw[(k * K) & (SN - 1)];
-                re += a[K] * w.x;
-                im -= a[K] * w.y;
+                re += a[K] * cos16(k * K);
+                im -= a[K] * sin16(k * K);
             }
             A[t * 9 + p] = make_float2(re, im);
         }
     }
     __syncthreads();
+    {
+        // the y mode of a thread's four outputs is (t / 8) % 8 for all of them
+        const int k = kidx((t >> 3) & 7);
+        float2 wy[SN];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int o = t + 256 * r;
-        const int I = o >> 6, py = (o >> 3) & 7, pz = o & 7;
-        const int k = kidx(py);
-        float re = 0.f, im = 0.f;
+        for (int J = 0; J < SN; ++J) wy[J] = tw[(k * J) & (SN - 1)];
 #pragma unroll
-        for (int J = 0; J < SN; ++J) {
-            const float2 a = A[(I * SN + J) * 9 + pz];
-            const float2 w = tw[(k * J) & (SN - 1)];
-            re += a.x * w.x + a.y * w.y;
-            im += a.y * w.x - a.x * w.y;
+        for (int r = 0; r < 4; ++r) {
+            const int o = t + 256 * r;
+            const int I = o >> 6, pz = o & 7;
+            float re = 0.f, im = 0.f;
+#pragma unroll
+            for (int J = 0; J < SN; ++J) {
+                const float2 a = A[(I * SN + J) * 9 + pz];
+                re += a.x * wy[J].x + a.y * wy[J].y;
+                im += a.y * wy[J].x - a.x * wy[J].y;
+            }
+            Bs[o] = make_float2(re, im);
         }
-        Bs[o] = make_float2(re, im);
     }
     __syncthreads();
 #pragma unroll
@@ -145,32 +166,41 @@ __global__ __launch_bounds__(256) void k_sur_idft(const float2 *__restrict__ Y, 
         T1[o] = make_float2(re, im);
     }
     __syncthreads();
+    {
+        // the grid row J of a thread's eight outputs is (t / 8) % 16 for all of them
+        const int J = (t >> 3) & 15;
+        float2 wy[SM];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        const int o = t + 256 * r;
-        const int I = o >> 7, J = (o >> 3) & 15, pz = o & 7;
-        float re = 0.f, im = 0.f;
+        for (int py = 0; py < SM; ++py) wy[py] = tw[(kidx(py) * J) & (SN - 1)];
 #pragma unroll
-        for (int py = 0; py < SM; ++py) {
-            const float2 a = T1[(I * SM + py) * SM + pz];
-            const float2 w = tw[(kidx(py) * J) & (SN - 1)];
-            re += a.x * w.x - a.y * w.y;
-            im += a.x * w.y + a.y * w.x;
+        for (int r = 0; r < 8; ++r) {
+            const int o = t + 256 * r;
+            const int I = o >> 7, pz = o & 7;
+            float re = 0.f, im = 0.f;
+#pragma unroll
+            for (int py = 0; py < SM; ++py) {
+                const float2 a = T1[(I * SM + py) * SM + pz];
+                re += a.x * wy[py].x - a.y * wy[py].y;
+                im += a.x * wy[py].y + a.y * wy[py].x;
+            }
+            T2[(I * SN + J) * 9 + pz] = make_float2(re, im);
         }
-        T2[(I * SN + J) * 9 + pz] = make_float2(re, im);
     }
     __syncthreads();
     float *dst = out + slice * SN3;
+    // the grid column K of a thread's sixteen outputs is t % 16 for all of them
+    float2 wz[SM];
+#pragma unroll
+    for (int pz = 0; pz < SM; ++pz) wz[pz] = tw[(kidx(pz) * (t & 15)) & (SN - 1)];
 #pragma unroll
     for (int r = 0; r < SN3 / 256; ++r) {
         const int x = t + 256 * r;
-        const int I = x >> 8, J = (x >> 4) & 15, K = x & 15;
+        const int I = x >> 8, J = (x >> 4) & 15;
         float s = 0.f;
 #pragma unroll
         for (int pz = 0; pz < SM; ++pz) {
             const float2 a = T2[(I * SN + J) * 9 + pz];
-            const float2 w = tw[(kidx(pz) * K) & (SN - 1)];
-            s += a.x * w.x - a.y * w.y;
+            s += a.x * wz[pz].x - a.y * wz[pz].y;
         }
         dst[x] = (accumulate ? dst[x] : 0.f) + scale * s;
     }
