@@ -283,11 +283,9 @@ __global__ __launch_bounds__(256) void k_sur_lift(const float *__restrict__ g, c
 __global__ __launch_bounds__(256) void k_sur_pw(const float *__restrict__ v, const float *__restrict__ s,
                                                 const float *__restrict__ lw, int act, float *__restrict__ zout,
                                                 float *__restrict__ vout, int64_t total) {
-    __shared__ float Ws[SW * SW + SW];
-    const int t = threadIdx.x;
-    for (int e = t; e < SW * SW + SW; e += 256) Ws[e] = lw[e];
-    __syncthreads();
-    const int64_t idx = int64_t(blockIdx.x) * 256 + t;
+    // the weights are the same for every lane: read at uniform addresses, they
+    // arrive through scalar loads instead of LDS
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (idx >= total) return;
     const int64_t b = idx >> 12;
     const int x = int(idx & (SN3 - 1));
@@ -297,9 +295,9 @@ __global__ __launch_bounds__(256) void k_sur_pw(const float *__restrict__ v, con
 #pragma unroll 4
     for (int o = 0; o < SW; ++o) {
         const int64_t at = (b * SW + o) * SN3 + x;
-        float z = Ws[SW * SW + o] + s[at];
+        float z = lw[SW * SW + o] + s[at];
 #pragma unroll
-        for (int i = 0; i < SW; ++i) z += Ws[o * SW + i] * vi[i];
+        for (int i = 0; i < SW; ++i) z += lw[o * SW + i] * vi[i];
         if (zout) zout[at] = z;
         vout[at] = act ? gelu(z) : z;
     }
@@ -335,29 +333,26 @@ __global__ __launch_bounds__(256) void k_sur_proj_bwd(const float *__restrict__ 
 __global__ __launch_bounds__(256) void k_sur_pw_bwd(float *__restrict__ gv, const float *__restrict__ z,
                                                     const float *__restrict__ lw, int act, float *__restrict__ gvin,
                                                     int64_t total) {
-    __shared__ float Ws[SW * SW];
-    const int t = threadIdx.x;
-    for (int e = t; e < SW * SW; e += 256) Ws[e] = lw[e];
-    __syncthreads();
-    const int64_t idx = int64_t(blockIdx.x) * 256 + t;
+    // weight rows are read at uniform addresses (scalar loads), row o scaled
+    // by gz_o and added into the 32 input-channel sums
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (idx >= total) return;
     const int64_t b = idx >> 12;
     const int x = int(idx & (SN3 - 1));
-    float gz[SW];
+    float a[SW];
 #pragma unroll
+    for (int i = 0; i < SW; ++i) a[i] = 0.f;
+#pragma unroll 4
     for (int o = 0; o < SW; ++o) {
         const int64_t at = (b * SW + o) * SN3 + x;
         const float g = gv[at];
-        gz[o] = act ? g * gelu_grad(z[at]) : g;
-        gv[at] = gz[o];
-    }
-#pragma unroll 4
-    for (int i = 0; i < SW; ++i) {
-        float a = 0.f;
+        const float gz = act ? g * gelu_grad(z[at]) : g;
+        gv[at] = gz;
 #pragma unroll
-        for (int o = 0; o < SW; ++o) a += Ws[o * SW + i] * gz[o];
-        gvin[(b * SW + i) * SN3 + x] = a;
+        for (int i = 0; i < SW; ++i) a[i] += lw[o * SW + i] * gz;
     }
+#pragma unroll
+    for (int i = 0; i < SW; ++i) gvin[(b * SW + i) * SN3 + x] = a[i];
 }
 
 // G(o, i) += sum_{b, x} A(b, o, x) V(b, i, x);  gb(o) += sum_{b, x} A(b, o, x).

@@ -125,3 +125,43 @@ def test_sheared_mixture_density_and_positivity():
     r2 = (cv[0] - 0.1) ** 2 + (cv[1] + 0.2) ** 2 + (cv[2] - 0.3) ** 2
     assert np.allclose(g, 1.2 / (np.pi * 0.8) ** 1.5 * np.exp(-r2 / 0.8), rtol=1e-13)
     del torch
+
+
+def test_sur_abi_rejects_bad_shapes_and_arguments():
+    """The C ABI validates before touching the device: wrong ABI version,
+    unsupported shapes and null buffers return error codes with a message."""
+    import ctypes as C
+    from frfs_amd import _lib
+    lib = _lib.load()
+    n = C.c_int64()
+    good = _lib.fq_sur_shape(_lib.SUR_ABI_VERSION, 16, 32, 8, 4)
+    assert lib.fq_sur_sizes(C.byref(good), 64, 1, C.byref(n), None, None) == _lib.SUCCESS
+    assert n.value == osur.layout(32, 8, 4)['total']
+    act, work = C.c_int64(), C.c_int64()
+    assert lib.fq_sur_sizes(C.byref(good), 2, 1, None, C.byref(act), C.byref(work)) == _lib.SUCCESS
+    vol, md = 2 * 32 * 4096, 2 * 2 * 32 * 512
+    assert act.value == 5 * vol + 4 * vol + 4 * md + vol + md
+    assert work.value == 2 * vol + 2 * md
+    for bad, code in ((_lib.fq_sur_shape(99, 16, 32, 8, 4), _lib.ERR_INVALID),
+                      (_lib.fq_sur_shape(_lib.SUR_ABI_VERSION, 32, 32, 8, 4), _lib.ERR_UNSUPPORTED),
+                      (_lib.fq_sur_shape(_lib.SUR_ABI_VERSION, 16, 16, 8, 4), _lib.ERR_UNSUPPORTED),
+                      (_lib.fq_sur_shape(_lib.SUR_ABI_VERSION, 16, 32, 8, 9), _lib.ERR_UNSUPPORTED)):
+        assert lib.fq_sur_sizes(C.byref(bad), 1, 0, C.byref(n), None, None) == code
+        assert lib.fq_last_error()
+    assert lib.fq_sur_forward(C.byref(good), None, None, 1, 0, None, None, None) == _lib.ERR_INVALID
+    assert lib.fq_sur_forward(C.byref(good), 8, 8, _lib.SUR_MAX_BATCH + 1, 0, 8, 8, None) == _lib.ERR_INVALID
+    assert lib.fq_sur_backward(C.byref(good), 8, 8, 1, 8, None, 8, 8, None) == _lib.ERR_INVALID
+    assert lib.fq_sur_loss(-1, 8, 8, 8, 8, None) == _lib.ERR_INVALID
+    assert lib.fq_sur_adam(10, 8, 8, 8, 8, C.c_float(1e-3), C.c_float(0.9), C.c_float(0.999),
+                           C.c_float(1e-8), 0, None) == _lib.ERR_INVALID
+    assert lib.fq_sur_gather(16, 0.1, 1.0, 1, 8, 1, 10, 4, 3, 2, None, None, 8, None) == _lib.ERR_INVALID
+    assert lib.fq_sur_finish(16, 5.0, 0.1, 1.0, 1e-4, 8, 8, 8, 8, 8, 1, 10, 4, 0, 2, 8, 8,
+                             None) == _lib.ERR_INVALID
+    # zero-sized work launches nothing and succeeds
+    assert lib.fq_sur_forward(C.byref(good), 8, 8, 0, 0, 8, 8, None) == _lib.SUCCESS
+
+
+def test_stand_in_plugin_is_registered():
+    from frfs_amd.scattering import DGFSScatteringModel, subclass_where
+    from frfs_amd.surrogate import DGFSVHSGLLFNOScatteringModel
+    assert subclass_where(DGFSScatteringModel, scattering_model='vhs-gll-fno') is DGFSVHSGLLFNOScatteringModel
