@@ -359,6 +359,16 @@ def bench_fno16(a):
     def run():
         stats.update(sm.fs_batch(fin, qs))
     dt = timed(run)
+    # the same call with a defect tolerance the operator itself meets on most
+    # DOFs, so that the network serves them (the sampled check still applies)
+    loose, tol0, stats_l = 3e-2, sm.tol, {}
+    sm.tol = loose
+    ql = be.matrix(fin.ioshape)
+
+    def run_loose():
+        stats_l.update(sm.fs_batch(fin, ql))
+    dt_l = timed(run_loose)
+    sm.tol = tol0
     # the network forward alone, HIP events on the stream it runs on
     g = torch.empty((ndof, n), dtype=torch.float32, device=be.device)
     out = torch.empty_like(g)
@@ -428,6 +438,11 @@ def bench_fno16(a):
         'speedup_vs_fp64': dt_fp64 / dt, 'network_forward_ms': fwd_ms,
         'network_only_speedup_vs_fp64': dt_fp64 / (fwd_ms * 1e-3),
         'fallback_rate': float(fl.mean()),
+        'at_defect_tol_3e-2': {'dof_evals_per_s': ndof / dt_l, 'speedup_vs_fp64': dt_fp64 / dt_l,
+                               'fallback_rate': stats_l['fallback'] / ndof,
+                               'sampled_check_rel_l2_median': stats_l['check_rel_l2_median'],
+                               'rel_l2_vs_fp64_median': float(((from_aosoa(ql) - from_aosoa(qref)).pow(2).sum(1)
+                                                               / from_aosoa(qref).pow(2).sum(1)).sqrt().median())},
         'fallback_rate_at_tol': {t: float((dfc > t).mean()) for t in (1e-3, 3e-3, 1e-2, 3e-2)},
         'defect': {'median': float(np.median(dfc)), 'p90': float(np.quantile(dfc, 0.9))},
         'network_projected_rel_l2_vs_fp64_all_dofs': {'median': float(np.median(rn)),

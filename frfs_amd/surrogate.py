@@ -10,15 +10,21 @@ fallback: ``fs_batch`` runs the network on every DOF, measures each output's
 conservation defect (the largest cosine between Q and a collision invariant
 1, c, |c|^2), writes the conservation-projected network output where the
 defect is within ``tol`` and evaluates the spectral operator on the rest.
+A sampled spectral check guards each batch of up to 8192 DOFs as well: the
+spectral operator also evaluates every ``check_every``-th DOF that the
+network served, and when the median rel-L2 between the two exceeds
+``check_tol`` the whole batch is evaluated by the spectral operator.
 
 Normalisation: Q is quadratic in f, so with the density rho of f the network
 maps g = f / (rho f_ref) to Q / (rho^2 q_ref) (f_ref = pi^{-3/2}, the peak of
 the unit Maxwellian; q_ref is the RMS of Q / rho^2 over the training set).
 
 Extra configuration keys: ``[scattering-model] surrogate-weights`` (an .npz
-written by :meth:`FourierSurrogate.save`), ``surrogate-tol`` (default 1e-4)
-and ``surrogate-layers`` (default 4).  Without weights the network is the
-seeded initialisation and every DOF falls back.
+written by :meth:`FourierSurrogate.save`), ``surrogate-tol`` (default 1e-4),
+``surrogate-check-every`` (default 64; 0 turns the sampled check off),
+``surrogate-check-tol`` (default 3e-2) and ``surrogate-layers`` (default 4).
+Without weights the network is the seeded initialisation and every DOF
+falls back.
 """
 from __future__ import annotations
 
@@ -225,6 +231,9 @@ class DGFSVHSGLLFNOScatteringModel(DGFSVHSGLLScatteringModel):
         path = self.cfg.get('scattering-model', 'surrogate-weights', '')
         layers = self.cfg.getint('scattering-model', 'surrogate-layers', 4)
         self.tol = self.cfg.getfloat('scattering-model', 'surrogate-tol', 1e-4)
+        self.check_every = self.cfg.getint('scattering-model', 'surrogate-check-every', 64)
+        self.check_tol = self.cfg.getfloat('scattering-model', 'surrogate-check-tol', 3e-2)
+        self.inplace_runs = 4        # more runs of flagged DOFs than this: one compact batch
         self.net = FourierSurrogate.load(self.backend, path) if path else FourierSurrogate(self.backend, layers)
         gi, gd = gram(self.vm.cv(), self.vm.cw())
         torch = self.net.torch
@@ -249,8 +258,7 @@ class DGFSVHSGLLFNOScatteringModel(DGFSVHSGLLScatteringModel):
         st = _stream(self.backend)
         n = self.vm.vsize()
         cw, L = self.vm.cw(), self.vm.L()
-        flags_all = []
-        defects = []
+        flags_all, defects, checks = [], [], []
         for d0 in range(first_dof, first_dof + ndof, chunk):
             m = min(chunk, first_dof + ndof - d0)
             g = self.net._buf('g', m * n)[:m * n].view(m, n)
@@ -264,14 +272,62 @@ class DGFSVHSGLLFNOScatteringModel(DGFSVHSGLLScatteringModel):
             _lib.check(lib.fq_sur_finish(NV, L, cw, self.net.q_ref, self.tol, q.data_ptr(), rho.data_ptr(),
                                          self._ginv.data_ptr(), self._gdiag.data_ptr(), d_arr_out._as_parameter_,
                                          nrow, ldim, neles, d0, m, dfc.data_ptr(), flag.data_ptr(), st))
-            flags_all.append(flag[:m].cpu().numpy().copy())
+            fl = flag[:m].cpu().numpy().copy()
+            if self.check_every > 0 and not fl.all():
+                med = self._sampled_check(d_arr_in, d_arr_out, d0, m, fl)
+                checks.append(med)
+                if med > self.check_tol:
+                    fl[:] = 1                 # the whole chunk goes to the spectral operator
+            flags_all.append(fl)
             defects.append(dfc[:m].cpu().numpy().copy())
         flags = np.concatenate(flags_all) if flags_all else np.zeros(0, np.int32)
-        # spectral operator on each run of flagged DOFs
+        # spectral operator on the flagged DOFs: in place for a few contiguous
+        # runs, else gathered into one compact batch (each DOF's result does
+        # not depend on the batch it is evaluated in)
         bad = np.flatnonzero(flags)
         runs = np.split(bad, np.flatnonzero(np.diff(bad) != 1) + 1) if bad.size else []
-        for r in runs:
-            self.fs_spectral(d_arr_in, d_arr_out, first_dof + int(r[0]), int(r.size))
+        if len(runs) <= self.inplace_runs:
+            for r in runs:
+                self.fs_spectral(d_arr_in, d_arr_out, first_dof + int(r[0]), int(r.size))
+        else:
+            from .inputs import from_aosoa, to_aosoa
+            idx = self._aosoa_index(d_arr_in, first_dof + bad)
+            sin = self.backend.matrix((1, n, bad.size))
+            sout = self.backend.matrix((1, n, bad.size))
+            to_aosoa(d_arr_in.data[idx], sin)
+            self.fs_spectral(sin, sout)
+            d_arr_out.data[idx] = from_aosoa(sout)
         self.last_stats = {'ndof': int(ndof), 'fallback': int(bad.size),
-                           'defect': np.concatenate(defects) if defects else np.zeros(0)}
+                           'defect': np.concatenate(defects) if defects else np.zeros(0),
+                           'check_rel_l2_median': checks,
+                           'check_failed_chunks': int(sum(c > self.check_tol for c in checks))}
         return self.last_stats
+
+    def _aosoa_index(self, d_arr, dofs):
+        """(len(dofs), N^3) element offsets of the given DOFs in an AoSoA
+        matrix of this grid (dof = elem * nupts + upt)."""
+        torch = self.net.torch
+        nrow, ldim, _ = d_arr.traits
+        n = self.vm.vsize()
+        soa = self.backend.soasz
+        dev = self.backend.device
+        d = torch.as_tensor(np.asarray(dofs), dtype=torch.int64, device=dev)
+        e, u = d // nrow, d % nrow
+        base = u * ldim + (e // soa) * (n * soa) + e % soa
+        return base[:, None] + torch.arange(n, device=dev)[None, :] * soa
+
+    def _sampled_check(self, d_arr_in, d_arr_out, d0, m, fl):
+        """Median rel-L2, over every check_every-th of the chunk's DOFs that
+        the network served, between the written Q and the spectral operator's."""
+        from .inputs import from_aosoa, to_aosoa
+        n = self.vm.vsize()
+        pick = np.flatnonzero(fl == 0)[::self.check_every]
+        idx = self._aosoa_index(d_arr_in, d0 + pick)
+        sin = self.backend.matrix((1, n, len(pick)))
+        sout = self.backend.matrix((1, n, len(pick)))
+        to_aosoa(d_arr_in.data[idx], sin)
+        self.fs_spectral(sin, sout)
+        qs = from_aosoa(sout)
+        qn = d_arr_out.data[idx]
+        rel = ((qn - qs).pow(2).sum(1) / qs.pow(2).sum(1).clamp_min(1e-300)).sqrt()
+        return float(rel.median())

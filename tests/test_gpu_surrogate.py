@@ -219,9 +219,11 @@ def test_fallback_and_conservation(backend):
     torch.cuda.synchronize()
     assert st['fallback'] == nd and np.array_equal(q0.get(), spec.get())
     sm.tol = 2.0
+    sm.check_tol = 1e30              # the sampled spectral check always passes
     q1 = backend.matrix(fin.ioshape)
     st = sm.fs_batch(fin, q1)
-    assert st['fallback'] == 0
+    assert st['fallback'] == 0 and len(st['check_rel_l2_median']) == 1
+    st_defect = st['defect'].copy()
     cv, cw = vm.cv(), vm.cw()
     # the network's own input (the gather kernel is checked against NumPy above)
     g = torch.empty((nd, vm.vsize()), dtype=torch.float32, device='cuda')
@@ -239,6 +241,30 @@ def test_fallback_and_conservation(backend):
         assert rel_l2(got[u, :, e], rho[b] ** 2 * sm.net.q_ref * qp) <= 1e-12
         mom = phi @ got[u, :, e] * cw
         assert np.max(np.abs(mom)) <= 1e-10 * np.abs(got[u, :, e]).sum() * cw * 25
+    # the sampled spectral check fails (tolerance 0): the whole batch is the
+    # spectral operator's, bit for bit, with every DOF counted as a fallback
+    sm.check_tol = 0.0
+    sm.check_every = 4
+    q3 = backend.matrix(fin.ioshape)
+    st = sm.fs_batch(fin, q3)
+    assert st['fallback'] == nd and st['check_failed_chunks'] == 1 and st['check_rel_l2_median'][0] > 0
+    assert np.array_equal(q3.get(), spec.get())
+    sm.check_tol = 1e30
+    # a tolerance between the DOFs' defects flags a scattered subset: the
+    # flagged DOFs, gathered into one compact spectral batch, get the spectral
+    # operator's result bit for bit and the others keep the network's
+    sm.check_every = 64
+    sm.tol = float(np.median(st_defect))
+    sm.inplace_runs = 0
+    q4 = backend.matrix(fin.ioshape)
+    st = sm.fs_batch(fin, q4)
+    bad = st_defect > sm.tol
+    assert 0 < bad.sum() < nd and st['fallback'] == bad.sum()
+    a4, a1, asp = q4.get(), q1.get(), spec.get()
+    for b in range(nd):
+        e, u = divmod(b, nupts)
+        assert np.array_equal(a4[u, :, e], (asp if bad[b] else a1)[u, :, e]), b
+    sm.tol, sm.inplace_runs = 2.0, 4
     # a DOF sub-range writes only its DOFs
     q2 = backend.matrix(fin.ioshape, np.full(fin.ioshape, -1.0))
     sm.fs_batch(fin, q2, first_dof=4, ndof=6)
