@@ -54,6 +54,7 @@ class fq_params(C.Structure):
 SUR_ABI_VERSION = 1
 SUR_MAX_LAYERS = 8
 SUR_MAX_BATCH = 65536
+SUR_FUSED_FLOATS = 65697
 
 
 class fq_sur_shape(C.Structure):
@@ -97,6 +98,8 @@ SIGNATURES = {
     # include/frfs_sur.h
     'fq_sur_sizes': (C.c_int, [_vp, _i64, _i32, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]),
     'fq_sur_forward': (C.c_int, [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp]),
+    'fq_sur_fuse': (C.c_int, [_vp, _vp, _vp, _vp]),
+    'fq_sur_forward_fused': (C.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
     'fq_sur_backward': (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp]),
     'fq_sur_loss': (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
     'fq_sur_adam': (C.c_int, [_i64, _vp, _vp, _vp, _vp, C.c_float, C.c_float, C.c_float, C.c_float,

@@ -552,6 +552,147 @@ __global__ __launch_bounds__(256) void k_sur_finish(int N, double c0, double dc,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Inference with the lift folded into the first layer and the projection into
+// the last (layers >= 2).  With Ghat = DFT(g)/n (one channel):
+//   first layer: X_0(i,k) = P_i Ghat(k) + p0_i [k = 0], so
+//     Y_0(o,k) = Ghat(k) Rp(o,k) + [k = 0] c0(o),  Rp = sum_i P_i R_0(i,o,.),
+//     c0 = sum_i p0_i R_0(i,o,0);  z_0 = A g + a0 + s_0,  A = Wm_0 P,
+//     a0 = Wm_0 p0 + b_0;
+//   last layer: Qw.s = Re IDFT(sum_o Qw_o Y(o,.)) and sum_o Qw_o Y(o,k) =
+//     sum_i X(i,k) Rq(i,k), Rq = sum_o Qw_o R(i,o,.), so
+//     out = QW.v + qb + Re IDFT(Y1),  QW = Qw Wm,  qb = Qw.b + q0.
+// The folded weights (k_sur_fuse) are laid out as Rp (SW x SM3 complex), c0
+// (SW complex), Rq (SW x SM3 complex), A, a0, QW (SW each), qb.
+// ---------------------------------------------------------------------------
+constexpr int64_t FRC0 = 2LL * SW * SM3, FRQ = FRC0 + 2 * SW, FA = FRQ + 2LL * SW * SM3;
+constexpr int64_t FA0 = FA + SW, FQW = FA0 + SW, FQB = FQW + SW, FTOT = FQB + 1;
+static_assert(FTOT == FQ_SUR_FUSED_FLOATS, "folded weight layout");
+
+__global__ __launch_bounds__(256) void k_sur_fuse(const float *__restrict__ th, int L, float *__restrict__ fw) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const float *P = th, *p0 = th + SW;
+    const float *l0 = th + 2 * SW, *ll = th + 2 * SW + int64_t(L - 1) * SLAYER;
+    const float2 *R0 = reinterpret_cast<const float2 *>(l0), *RL = reinterpret_cast<const float2 *>(ll);
+    const float *W0 = l0 + SR, *b0 = l0 + SR + SW * SW, *WL = ll + SR, *bL = ll + SR + SW * SW;
+    const float *Qw = th + 2 * SW + int64_t(L) * SLAYER, q0 = Qw[SW];
+    int64_t j = idx;
+    if (j < SW * SM3) {                      // Rp(o, k)
+        const int k = int(j % SM3), o = int(j / SM3);
+        float2 a = make_float2(0.f, 0.f);
+        for (int i = 0; i < SW; ++i) {
+            const float2 r = R0[(int64_t(i) * SW + o) * SM3 + k];
+            a.x += P[i] * r.x;
+            a.y += P[i] * r.y;
+        }
+        reinterpret_cast<float2 *>(fw)[j] = a;
+        return;
+    }
+    j -= SW * SM3;
+    if (j < SW * SM3) {                      // Rq(i, k)
+        const int k = int(j % SM3), i = int(j / SM3);
+        float2 a = make_float2(0.f, 0.f);
+        for (int o = 0; o < SW; ++o) {
+            const float2 r = RL[(int64_t(i) * SW + o) * SM3 + k];
+            a.x += Qw[o] * r.x;
+            a.y += Qw[o] * r.y;
+        }
+        reinterpret_cast<float2 *>(fw + FRQ)[j] = a;
+        return;
+    }
+    j -= SW * SM3;
+    if (j < SW) {                            // c0(o), A(o), a0(o), QW(i = j)
+        const int o = int(j);
+        float2 c = make_float2(0.f, 0.f);
+        float a = 0.f, a0 = b0[o], qw = 0.f;
+        for (int i = 0; i < SW; ++i) {
+            const float2 r = R0[(int64_t(i) * SW + o) * SM3];
+            c.x += p0[i] * r.x;
+            c.y += p0[i] * r.y;
+            a += W0[o * SW + i] * P[i];
+            a0 += W0[o * SW + i] * p0[i];
+            qw += Qw[i] * WL[i * SW + o];     // QW at input channel o
+        }
+        reinterpret_cast<float2 *>(fw + FRC0)[o] = c;
+        fw[FA + o] = a;
+        fw[FA0 + o] = a0;
+        fw[FQW + o] = qw;
+        return;
+    }
+    if (j == SW) {                           // qb
+        float q = q0;
+        for (int o = 0; o < SW; ++o) q += Qw[o] * bL[o];
+        fw[FQB] = q;
+    }
+}
+
+// Y(b,o,k) = Xg(b,k) Rp(o,k) + [k = 0] c0(o)
+__global__ __launch_bounds__(256) void k_sur_mix_first(const float2 *__restrict__ Xg, const float *__restrict__ fw,
+                                                       float2 *__restrict__ Y, int64_t total) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= total) return;
+    const int k = int(idx & (SM3 - 1));
+    const int o = int((idx >> 9) & (SW - 1));
+    const int64_t b = idx >> 14;
+    const float2 x = Xg[b * SM3 + k];
+    const float2 r = reinterpret_cast<const float2 *>(fw)[o * SM3 + k];
+    float2 y = make_float2(x.x * r.x - x.y * r.y, x.x * r.y + x.y * r.x);
+    if (k == 0) {
+        const float2 c = reinterpret_cast<const float2 *>(fw + FRC0)[o];
+        y.x += c.x;
+        y.y += c.y;
+    }
+    Y[idx] = y;
+}
+
+// v_1(o) = gelu(A_o g + a0_o + s_0(o)) per point
+__global__ __launch_bounds__(256) void k_sur_pw_first(const float *__restrict__ g, const float *__restrict__ s,
+                                                      const float *__restrict__ fw, float *__restrict__ vout,
+                                                      int64_t total) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= total) return;
+    const int64_t b = idx >> 12;
+    const int x = int(idx & (SN3 - 1));
+    const float gv = g[idx];
+#pragma unroll 8
+    for (int o = 0; o < SW; ++o) {
+        const int64_t at = (b * SW + o) * SN3 + x;
+        vout[at] = gelu(fw[FA + o] * gv + fw[FA0 + o] + s[at]);
+    }
+}
+
+// Y1(b,k) = sum_i X(b,i,k) Rq(i,k)
+__global__ __launch_bounds__(256) void k_sur_mix_last(const float2 *__restrict__ X, const float *__restrict__ fw,
+                                                      float2 *__restrict__ Y1, int64_t total) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= total) return;
+    const int k = int(idx & (SM3 - 1));
+    const int64_t b = idx >> 9;
+    const float2 *Rq = reinterpret_cast<const float2 *>(fw + FRQ);
+    float2 a = make_float2(0.f, 0.f);
+    for (int i = 0; i < SW; ++i) {
+        const float2 x = X[(b * SW + i) * SM3 + k];
+        const float2 r = Rq[i * SM3 + k];
+        a.x += x.x * r.x - x.y * r.y;
+        a.y += x.x * r.y + x.y * r.x;
+    }
+    Y1[idx] = a;
+}
+
+// out(b,x) = QW . v(b,:,x) + qb + s1(b,x)
+__global__ __launch_bounds__(256) void k_sur_pw_last(const float *__restrict__ v, const float *__restrict__ s1,
+                                                     const float *__restrict__ fw, float *__restrict__ out,
+                                                     int64_t total) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= total) return;
+    const int64_t b = idx >> 12;
+    const int x = int(idx & (SN3 - 1));
+    float a = fw[FQB] + s1[idx];
+#pragma unroll 8
+    for (int i = 0; i < SW; ++i) a += fw[FQW + i] * v[(b * SW + i) * SN3 + x];
+    out[idx] = a;
+}
+
 inline unsigned nblk(int64_t n) { return unsigned((n + 255) / 256); }
 
 int check_shape(const fq_sur_shape *s, const char *who) {
@@ -647,6 +788,53 @@ int fq_sur_forward(const fq_sur_shape *shape, const float *theta, const float *g
     }
     hipLaunchKernelGGL(k_sur_proj, dim3(nblk(pts)), dim3(256), 0, st, p.v[L], theta + 2 * SW + L * SLAYER, out, pts);
     return hip_status("fq_sur_forward");
+}
+
+int fq_sur_fuse(const fq_sur_shape *shape, const float *theta, float *fused, void *stream) {
+    if (int rc = check_shape(shape, "fq_sur_fuse")) return rc;
+    if (shape->layers < 2) return fq_internal_fail(FQ_ERR_UNSUPPORTED, "fq_sur_fuse: needs at least two layers");
+    if (!theta || !fused) return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_fuse: invalid arguments");
+    hipLaunchKernelGGL(k_sur_fuse, dim3(nblk(2LL * SW * SM3 + SW + 1)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), theta, shape->layers, fused);
+    return hip_status("fq_sur_fuse");
+}
+
+int fq_sur_forward_fused(const fq_sur_shape *shape, const float *theta, const float *fused, const float *g,
+                         int64_t batch, float *acts, float *out, void *stream) {
+    if (int rc = check_shape(shape, "fq_sur_forward_fused")) return rc;
+    if (shape->layers < 2)
+        return fq_internal_fail(FQ_ERR_UNSUPPORTED, "fq_sur_forward_fused: needs at least two layers");
+    if (batch < 0 || batch > FQ_SUR_MAX_BATCH || !theta || !fused || !g || !acts || !out)
+        return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_forward_fused: invalid arguments");
+    if (batch == 0) return FQ_SUCCESS;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int L = shape->layers;
+    const ActPtrs p = act_ptrs(acts, L, batch, false);
+    const int64_t pts = batch * SN3;
+    const unsigned slices = unsigned(batch * SW);
+    // first layer from the one-channel transform of g
+    hipLaunchKernelGGL(k_sur_dft, dim3(unsigned(batch)), dim3(256), 0, st, g, 1.0f / SN3, p.X[0]);
+    hipLaunchKernelGGL(k_sur_mix_first, dim3(nblk(batch * SW * SM3)), dim3(256), 0, st, p.X[0], fused, p.Y,
+                       batch * SW * SM3);
+    hipLaunchKernelGGL(k_sur_idft, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, p.s);
+    hipLaunchKernelGGL(k_sur_pw_first, dim3(nblk(pts)), dim3(256), 0, st, g, p.s, fused, p.v[1], pts);
+    for (int l = 1; l < L - 1; ++l) {
+        const float *lay = theta + 2 * SW + l * SLAYER;
+        const float2 *R = reinterpret_cast<const float2 *>(lay);
+        hipLaunchKernelGGL(k_sur_dft, dim3(slices), dim3(256), 0, st, p.v[l], 1.0f / SN3, p.X[l]);
+        hipLaunchKernelGGL(k_sur_mix_lds<0>, dim3(SM3 / 8, unsigned((batch + 31) / 32)), dim3(256), 0, st, p.X[l],
+                           R, p.Y, batch);
+        hipLaunchKernelGGL(k_sur_idft, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, p.s);
+        hipLaunchKernelGGL(k_sur_pw, dim3(nblk(pts)), dim3(256), 0, st, p.v[l], p.s, lay + SR, 1, nullptr,
+                           p.v[l + 1], pts);
+    }
+    // last layer and projection as one output channel
+    hipLaunchKernelGGL(k_sur_dft, dim3(slices), dim3(256), 0, st, p.v[L - 1], 1.0f / SN3, p.X[L - 1]);
+    hipLaunchKernelGGL(k_sur_mix_last, dim3(nblk(batch * SM3)), dim3(256), 0, st, p.X[L - 1], fused, p.Y,
+                       batch * SM3);
+    hipLaunchKernelGGL(k_sur_idft, dim3(unsigned(batch)), dim3(256), 0, st, p.Y, 1.0f, 0, p.s);
+    hipLaunchKernelGGL(k_sur_pw_last, dim3(nblk(pts)), dim3(256), 0, st, p.v[L - 1], p.s, fused, out, pts);
+    return hip_status("fq_sur_forward_fused");
 }
 
 int fq_sur_backward(const fq_sur_shape *shape, const float *theta, const float *g, int64_t batch, float *acts,

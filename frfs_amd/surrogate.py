@@ -84,6 +84,12 @@ class FourierSurrogate:
         self.q_ref = float(q_ref)
         self._adam = None
         self._bufs = {}
+        self._fused = None           # folded inference weights, valid for theta version _fused_ver
+        self._ver, self._fused_ver = 0, -1
+
+    def invalidate(self):
+        """Call after changing theta in place (train_step does it itself)."""
+        self._ver += 1
 
     # -- sizes and buffers -------------------------------------------------
     def _sizes(self, batch, training):
@@ -105,13 +111,27 @@ class FourierSurrogate:
         return _stream(self.backend)
 
     # -- network -----------------------------------------------------------
-    def forward(self, g, out, training=False):
-        """out[b] = network(g[b]) for (batch, 16^3) float32 device tensors."""
+    def forward(self, g, out, training=False, fused=True):
+        """out[b] = network(g[b]) for (batch, 16^3) float32 device tensors.
+        Inference (training False) with two or more layers runs the folded
+        first and last layers (fq_sur_forward_fused) unless fused is False."""
         batch = g.shape[0]
         _, act, _ = self._sizes(batch, training)
         acts = self._buf('acts_train' if training else 'acts', act)
-        _lib.check(_lib.load().fq_sur_forward(C.byref(self.shape), self.theta.data_ptr(), g.data_ptr(), batch,
-                                              int(training), acts.data_ptr(), out.data_ptr(), self._stream()))
+        lib = _lib.load()
+        if not training and fused and self.layers >= 2:
+            if self._fused_ver != self._ver:
+                if self._fused is None:
+                    self._fused = self.torch.empty(_lib.SUR_FUSED_FLOATS, dtype=self.torch.float32, device=self.dev)
+                _lib.check(lib.fq_sur_fuse(C.byref(self.shape), self.theta.data_ptr(), self._fused.data_ptr(),
+                                           self._stream()))
+                self._fused_ver = self._ver
+            _lib.check(lib.fq_sur_forward_fused(C.byref(self.shape), self.theta.data_ptr(), self._fused.data_ptr(),
+                                                g.data_ptr(), batch, acts.data_ptr(), out.data_ptr(),
+                                                self._stream()))
+            return acts
+        _lib.check(lib.fq_sur_forward(C.byref(self.shape), self.theta.data_ptr(), g.data_ptr(), batch,
+                                      int(training), acts.data_ptr(), out.data_ptr(), self._stream()))
         return acts
 
     def backward(self, g, acts, gout, grad):
@@ -145,6 +165,7 @@ class FourierSurrogate:
         m, v, step = self._adam
         _lib.check(_lib.load().fq_sur_adam(self.nparams, self.theta.data_ptr(), grad.data_ptr(), m.data_ptr(),
                                            v.data_ptr(), lr, betas[0], betas[1], eps, step, self._stream()))
+        self.invalidate()
         return sums
 
     def fit(self, g_all, t_all, *, epochs=1, batch=64, lr=1e-3, lr_final=None, seed=0, time_limit=None,
@@ -175,11 +196,11 @@ class FourierSurrogate:
                 break
         return hist
 
-    def predict(self, g, chunk=8192):
+    def predict(self, g, chunk=8192, fused=True):
         """Network output for (batch, 16^3) float32 rows, in chunks."""
         out = self.torch.empty_like(g)
         for b0 in range(0, g.shape[0], chunk):
-            self.forward(g[b0:b0 + chunk], out[b0:b0 + chunk])
+            self.forward(g[b0:b0 + chunk], out[b0:b0 + chunk], fused=fused)
         return out
 
     # -- persistence -------------------------------------------------------

@@ -36,6 +36,7 @@ extern "C" {
 #define FQ_SUR_ABI_VERSION 1
 #define FQ_SUR_MAX_LAYERS 8
 #define FQ_SUR_MAX_BATCH 65536
+#define FQ_SUR_FUSED_FLOATS 65697   /* folded inference weights (fq_sur_fuse) */
 
 typedef struct fq_sur_shape {
     int32_t abi_version;   /* FQ_SUR_ABI_VERSION */
@@ -54,6 +55,17 @@ int fq_sur_sizes(const fq_sur_shape *shape, int64_t batch, int32_t training, int
 /* out[batch][16^3] = network(g[batch][16^3]).  acts: fq_sur_sizes floats. */
 int fq_sur_forward(const fq_sur_shape *shape, const float *theta, const float *g, int64_t batch,
                    int32_t training, float *acts, float *out, void *stream);
+
+/* Inference with the lift folded into the first layer and the projection
+ * into the last (layers >= 2): fq_sur_fuse computes the folded weights
+ * (FQ_SUR_FUSED_FLOATS floats) from theta; fq_sur_forward_fused then gives
+ * the output of fq_sur_forward up to rounding, transforming one channel
+ * instead of 32 in the first and last layers.  acts: the inference size
+ * (training = 0) of fq_sur_sizes.  Recompute the folded weights whenever
+ * theta changes. */
+int fq_sur_fuse(const fq_sur_shape *shape, const float *theta, float *fused, void *stream);
+int fq_sur_forward_fused(const fq_sur_shape *shape, const float *theta, const float *fused, const float *g,
+                         int64_t batch, float *acts, float *out, void *stream);
 
 /* grad += d(sum gout*out)/d theta, from the activations of a training
  * forward call on the same theta and g (grad is caller-zeroed; acts is

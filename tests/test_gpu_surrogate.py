@@ -60,7 +60,7 @@ def test_forward_matches_oracle(backend):
     g = _inputs(3, seed=21)
     net = FourierSurrogate(backend, layers=L, theta=th)
     print('[surrogate forward L=4 B=3]', flush=True)
-    out = net.predict(_dev(g)).cpu().numpy()
+    out = net.predict(_dev(g), fused=False).cpu().numpy()
     want = osur.forward(th, g, N, W, M, L)
     for b in range(3):
         assert rel_l2(out[b], want[b]) <= FWD_TOL, b
@@ -69,6 +69,36 @@ def test_forward_matches_oracle(backend):
     o2 = torch.empty_like(_dev(g))
     net.forward(_dev(g), o2, training=True)
     assert np.array_equal(o2.cpu().numpy(), out)
+    # inference with the lift and projection folded into the first and last
+    # layers: the same network up to FP32 rounding, against the oracle too
+    of = net.predict(_dev(g)).cpu().numpy()
+    for b in range(3):
+        assert rel_l2(of[b], out[b]) <= 2e-6, b
+        assert rel_l2(of[b], want[b]) <= FWD_TOL, b
+
+
+@pytest.mark.parametrize('L', [2, 3])
+def test_fused_inference_shallow(backend, L):
+    """Two and three layers: the folded first and last layers are adjacent
+    (L = 2) or around one ordinary layer (L = 3); the folded weights follow
+    theta after a training step."""
+    import torch
+    from frfs_amd.surrogate import FourierSurrogate
+    th = _theta(L, seed=40 + L)
+    g = _inputs(2, seed=50 + L)
+    net = FourierSurrogate(backend, layers=L, theta=th)
+    print(f'[surrogate fused L={L}]', flush=True)
+    of = net.predict(_dev(g)).cpu().numpy()
+    want = osur.forward(th, g, N, W, M, L)
+    for b in range(2):
+        assert rel_l2(of[b], want[b]) <= FWD_TOL, b
+    net.train_step(_dev(g), torch.zeros_like(_dev(g)), 1e-3)
+    th2 = net.theta.double().cpu().numpy()
+    of2 = net.predict(_dev(g)).cpu().numpy()
+    want2 = osur.forward(th2, g, N, W, M, L)
+    assert not np.array_equal(of2, of)
+    for b in range(2):
+        assert rel_l2(of2[b], want2[b]) <= FWD_TOL, b
 
 
 @pytest.mark.parametrize('L', [1, 4])
