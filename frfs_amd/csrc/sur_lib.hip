@@ -67,16 +67,16 @@ __device__ __forceinline__ float gelu_grad(float z) {
 }
 
 // X[slice] = scale * sum_x v[slice](x) e^{-i k.x} over the retained modes.
-// One workgroup per (batch, channel) slice; separable z, y, x stages in LDS.
+// One workgroup per (batch, channel) slice; separable z, y, x stages in LDS,
+// each thread reading its inputs once and producing all the modes of its
+// line with compile-time twiddles.
 __global__ __launch_bounds__(256) void k_sur_dft(const float *__restrict__ v, float scale, float2 *__restrict__ X) {
     __shared__ float vin[SN2 * 17];
     __shared__ float2 A[SN2 * 9];            // [I][J][pz]
     __shared__ float2 Bs[SN * SM * SM];      // [I][py][pz]
-    __shared__ float2 tw[SN];
     const int t = threadIdx.x;
     const int64_t slice = blockIdx.x;
     const float *src = v + slice * SN3;
-    load_tw(tw);
 #pragma unroll
     for (int r = 0; r < SN3 / 256; ++r) {
         const int e = t + 256 * r;
@@ -100,41 +100,44 @@ __global__ __launch_bounds__(256) void k_sur_dft(const float *__restrict__ v, fl
         }
     }
     __syncthreads();
-    {
-        // the y mode of a thread's four outputs is (t / 8) % 8 for all of them
-        const int k = kidx((t >> 3) & 7);
-        float2 wy[SN];
+    // y: thread t < 128 owns (I, pz) = (t / 8, t % 8): its 16 values once, the
+    // 8 y modes with compile-time twiddles
+    if (t < SN * SM) {
+        const int I = t >> 3, pz = t & 7;
+        float2 a[SN];
 #pragma unroll
-        for (int J = 0; J < SN; ++J) wy[J] = tw[(k * J) & (SN - 1)];
+        for (int J = 0; J < SN; ++J) a[J] = A[(I * SN + J) * 9 + pz];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int o = t + 256 * r;
-            const int I = o >> 6, pz = o & 7;
+        for (int py = 0; py < SM; ++py) {
+            const int k = kidx(py);
             float re = 0.f, im = 0.f;
 #pragma unroll
             for (int J = 0; J < SN; ++J) {
-                const float2 a = A[(I * SN + J) * 9 + pz];
-                re += a.x * wy[J].x + a.y * wy[J].y;
-                im += a.y * wy[J].x - a.x * wy[J].y;
+                const float c = cos16(k * J), sn = sin16(k * J);
+                re += a[J].x * c + a[J].y * sn;
+                im += a[J].y * c - a[J].x * sn;
             }
-            Bs[o] = make_float2(re, im);
+            Bs[(I * SM + py) * SM + pz] = make_float2(re, im);
         }
     }
     __syncthreads();
+    // x: thread t < 64 owns (py, pz) = (t / 8, t % 8): 16 values, the 8 x modes
+    if (t < SM * SM) {
+        float2 a[SN];
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-        const int o = t + 256 * r;
-        const int px = o >> 6, py = (o >> 3) & 7, pz = o & 7;
-        const int k = kidx(px);
-        float re = 0.f, im = 0.f;
+        for (int I = 0; I < SN; ++I) a[I] = Bs[I * SM * SM + t];
 #pragma unroll
-        for (int I = 0; I < SN; ++I) {
-            const float2 a = Bs[(I * SM + py) * SM + pz];
-            const float2 w = tw[(k * I) & (SN - 1)];
-            re += a.x * w.x + a.y * w.y;
-            im += a.y * w.x - a.x * w.y;
+        for (int px = 0; px < SM; ++px) {
+            const int k = kidx(px);
+            float re = 0.f, im = 0.f;
+#pragma unroll
+            for (int I = 0; I < SN; ++I) {
+                const float c = cos16(k * I), sn = sin16(k * I);
+                re += a[I].x * c + a[I].y * sn;
+                im += a[I].y * c - a[I].x * sn;
+            }
+            X[slice * SM3 + px * SM * SM + t] = make_float2(scale * re, scale * im);
         }
-        X[slice * SM3 + o] = make_float2(scale * re, scale * im);
     }
 }
 
@@ -187,22 +190,32 @@ __global__ __launch_bounds__(256) void k_sur_idft(const float2 *__restrict__ Y, 
         }
     }
     __syncthreads();
-    float *dst = out + slice * SN3;
-    // the grid column K of a thread's sixteen outputs is t % 16 for all of them
-    float2 wz[SM];
+    // z: thread t owns grid row (I, J) = t: its 8 modes from LDS once, the 16
+    // outputs with compile-time twiddles, written as four 16-byte stores
+    float2 a[SM];
 #pragma unroll
-    for (int pz = 0; pz < SM; ++pz) wz[pz] = tw[(kidx(pz) * (t & 15)) & (SN - 1)];
+    for (int pz = 0; pz < SM; ++pz) a[pz] = T2[t * 9 + pz];
+    float4 *dst = reinterpret_cast<float4 *>(out + slice * SN3 + t * SN);
 #pragma unroll
-    for (int r = 0; r < SN3 / 256; ++r) {
-        const int x = t + 256 * r;
-        const int I = x >> 8, J = (x >> 4) & 15;
-        float s = 0.f;
+    for (int q = 0; q < SN / 4; ++q) {
+        float o[4];
 #pragma unroll
-        for (int pz = 0; pz < SM; ++pz) {
-            const float2 a = T2[(I * SN + J) * 9 + pz];
-            s += a.x * wz[pz].x - a.y * wz[pz].y;
+        for (int j = 0; j < 4; ++j) {
+            const int K = 4 * q + j;
+            float s = 0.f;
+#pragma unroll
+            for (int pz = 0; pz < SM; ++pz) s += a[pz].x * cos16(kidx(pz) * K) - a[pz].y * sin16(kidx(pz) * K);
+            o[j] = scale * s;
         }
-        dst[x] = (accumulate ? dst[x] : 0.f) + scale * s;
+        float4 w = make_float4(o[0], o[1], o[2], o[3]);
+        if (accumulate) {
+            const float4 p0 = dst[q];
+            w.x += p0.x;
+            w.y += p0.y;
+            w.z += p0.z;
+            w.w += p0.w;
+        }
+        dst[q] = w;
     }
 }
 
