@@ -35,15 +35,6 @@ constexpr int64_t SLAYER = SR + SW * SW + SW;        // floats of one layer
 
 __device__ __forceinline__ int kidx(int p) { return p < SM / 2 ? p : p + SN - SM; }
 
-// tw[j] = (cos, sin)(2 pi j / 16)
-__device__ __forceinline__ void load_tw(float2 *tw) {
-    if (threadIdx.x < SN) {
-        float s, c;
-        sincospif(float(threadIdx.x) / (SN / 2), &s, &c);
-        tw[threadIdx.x] = make_float2(c, s);
-    }
-}
-
 // cos(2 pi j / 16); sin(2 pi j / 16) = cos(2 pi (j - 4) / 16).  Called with
 // compile-time j (unrolled loops) the values fold into the instructions.
 __device__ __forceinline__ float cos16(int j) {
@@ -147,44 +138,44 @@ __global__ __launch_bounds__(256) void k_sur_idft(const float2 *__restrict__ Y, 
     __shared__ float2 Ys[SM3];               // [px][py][pz]
     __shared__ float2 T1[SN * SM * SM];      // [I][py][pz]
     __shared__ float2 T2[SN2 * 9];           // [I][J][pz]
-    __shared__ float2 tw[SN];
     const int t = threadIdx.x;
     const int64_t slice = blockIdx.x;
-    load_tw(tw);
     Ys[t] = Y[slice * SM3 + t];
     Ys[t + 256] = Y[slice * SM3 + t + 256];
     __syncthreads();
+    // x: thread t < 64 owns (py, pz) = (t / 8, t % 8): its 8 modes once, the 16
+    // grid values I with compile-time twiddles
+    if (t < SM * SM) {
+        float2 a[SM];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int o = t + 256 * r;
-        const int I = o >> 6, py = (o >> 3) & 7, pz = o & 7;
-        float re = 0.f, im = 0.f;
+        for (int px = 0; px < SM; ++px) a[px] = Ys[px * SM * SM + t];
 #pragma unroll
-        for (int px = 0; px < SM; ++px) {
-            const float2 a = Ys[(px * SM + py) * SM + pz];
-            const float2 w = tw[(kidx(px) * I) & (SN - 1)];
-            re += a.x * w.x - a.y * w.y;
-            im += a.x * w.y + a.y * w.x;
+        for (int I = 0; I < SN; ++I) {
+            float re = 0.f, im = 0.f;
+#pragma unroll
+            for (int px = 0; px < SM; ++px) {
+                const float c = cos16(kidx(px) * I), sn = sin16(kidx(px) * I);
+                re += a[px].x * c - a[px].y * sn;
+                im += a[px].x * sn + a[px].y * c;
+            }
+            T1[I * SM * SM + t] = make_float2(re, im);
         }
-        T1[o] = make_float2(re, im);
     }
     __syncthreads();
-    {
-        // the grid row J of a thread's eight outputs is (t / 8) % 16 for all of them
-        const int J = (t >> 3) & 15;
-        float2 wy[SM];
+    // y: thread t < 128 owns (I, pz) = (t / 8, t % 8): 8 values, the 16 rows J
+    if (t < SN * SM) {
+        const int I = t >> 3, pz = t & 7;
+        float2 a[SM];
 #pragma unroll
-        for (int py = 0; py < SM; ++py) wy[py] = tw[(kidx(py) * J) & (SN - 1)];
+        for (int py = 0; py < SM; ++py) a[py] = T1[(I * SM + py) * SM + pz];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const int o = t + 256 * r;
-            const int I = o >> 7, pz = o & 7;
+        for (int J = 0; J < SN; ++J) {
             float re = 0.f, im = 0.f;
 #pragma unroll
             for (int py = 0; py < SM; ++py) {
-                const float2 a = T1[(I * SM + py) * SM + pz];
-                re += a.x * wy[py].x - a.y * wy[py].y;
-                im += a.x * wy[py].y + a.y * wy[py].x;
+                const float c = cos16(kidx(py) * J), sn = sin16(kidx(py) * J);
+                re += a[py].x * c - a[py].y * sn;
+                im += a[py].x * sn + a[py].y * c;
             }
             T2[(I * SN + J) * 9 + pz] = make_float2(re, im);
         }
