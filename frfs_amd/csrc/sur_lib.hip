@@ -21,6 +21,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 #include "../../include/frfs_sur.h"
 
@@ -53,6 +54,13 @@ __device__ __forceinline__ float cos16(int j) {
 __device__ __forceinline__ float sin16(int j) { return cos16(j - 4); }
 
 __device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// Activation storage: FP32, or FP16 for inference (arithmetic stays FP32).
+__device__ __forceinline__ float ldA(const float *p, int64_t i) { return p[i]; }
+__device__ __forceinline__ float ldA(const _Float16 *p, int64_t i) { return float(p[i]); }
+__device__ __forceinline__ void stA(float *p, int64_t i, float v) { p[i] = v; }
+__device__ __forceinline__ void stA(_Float16 *p, int64_t i, float v) { p[i] = _Float16(v); }
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+
 __device__ __forceinline__ float gelu_grad(float z) {
     return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
 }
@@ -61,17 +69,18 @@ __device__ __forceinline__ float gelu_grad(float z) {
 // One workgroup per (batch, channel) slice; separable z, y, x stages in LDS,
 // each thread reading its inputs once and producing all the modes of its
 // line with compile-time twiddles.
-__global__ __launch_bounds__(256) void k_sur_dft(const float *__restrict__ v, float scale, float2 *__restrict__ X) {
+template <typename TA>
+__global__ __launch_bounds__(256) void k_sur_dft(const TA *__restrict__ v, float scale, float2 *__restrict__ X) {
     __shared__ float vin[SN2 * 17];
     __shared__ float2 A[SN2 * 9];            // [I][J][pz]
     __shared__ float2 Bs[SN * SM * SM];      // [I][py][pz]
     const int t = threadIdx.x;
     const int64_t slice = blockIdx.x;
-    const float *src = v + slice * SN3;
+    const TA *src = v + slice * SN3;
 #pragma unroll
     for (int r = 0; r < SN3 / 256; ++r) {
         const int e = t + 256 * r;
-        vin[(e >> 4) * 17 + (e & 15)] = src[e];
+        vin[(e >> 4) * 17 + (e & 15)] = ldA(src, e);
     }
     __syncthreads();
     {
@@ -133,8 +142,9 @@ __global__ __launch_bounds__(256) void k_sur_dft(const float *__restrict__ v, fl
 }
 
 // out[slice](x) = (accumulate ? out : 0) + scale * Re sum_k Y[slice](k) e^{+i k.x}.
+template <typename TA>
 __global__ __launch_bounds__(256) void k_sur_idft(const float2 *__restrict__ Y, float scale, int accumulate,
-                                                  float *__restrict__ out) {
+                                                  TA *__restrict__ out) {
     __shared__ float2 Ys[SM3];               // [px][py][pz]
     __shared__ float2 T1[SN * SM * SM];      // [I][py][pz]
     __shared__ float2 T2[SN2 * 9];           // [I][J][pz]
@@ -186,27 +196,37 @@ __global__ __launch_bounds__(256) void k_sur_idft(const float2 *__restrict__ Y, 
     float2 a[SM];
 #pragma unroll
     for (int pz = 0; pz < SM; ++pz) a[pz] = T2[t * 9 + pz];
-    float4 *dst = reinterpret_cast<float4 *>(out + slice * SN3 + t * SN);
+    float o[SN];
 #pragma unroll
-    for (int q = 0; q < SN / 4; ++q) {
-        float o[4];
+    for (int K = 0; K < SN; ++K) {
+        float sum = 0.f;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int K = 4 * q + j;
-            float s = 0.f;
+        for (int pz = 0; pz < SM; ++pz) sum += a[pz].x * cos16(kidx(pz) * K) - a[pz].y * sin16(kidx(pz) * K);
+        o[K] = scale * sum;
+    }
+    if constexpr (std::is_same<TA, float>::value) {
+        float4 *dst = reinterpret_cast<float4 *>(out + slice * SN3 + t * SN);
 #pragma unroll
-            for (int pz = 0; pz < SM; ++pz) s += a[pz].x * cos16(kidx(pz) * K) - a[pz].y * sin16(kidx(pz) * K);
-            o[j] = scale * s;
+        for (int q = 0; q < SN / 4; ++q) {
+            float4 w = make_float4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+            if (accumulate) {
+                const float4 p0 = dst[q];
+                w.x += p0.x;
+                w.y += p0.y;
+                w.z += p0.z;
+                w.w += p0.w;
+            }
+            dst[q] = w;
         }
-        float4 w = make_float4(o[0], o[1], o[2], o[3]);
-        if (accumulate) {
-            const float4 p0 = dst[q];
-            w.x += p0.x;
-            w.y += p0.y;
-            w.z += p0.z;
-            w.w += p0.w;
+    } else {                                  // FP16 inference output (never accumulated)
+        half8 *dst = reinterpret_cast<half8 *>(out + slice * SN3 + t * SN);
+#pragma unroll
+        for (int q = 0; q < SN / 8; ++q) {
+            half8 w;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = _Float16(o[8 * q + j]);
+            dst[q] = w;
         }
-        dst[q] = w;
     }
 }
 
@@ -284,9 +304,10 @@ __global__ __launch_bounds__(256) void k_sur_lift(const float *__restrict__ g, c
 }
 
 // z = Wm v + b + s; vout = act ? gelu(z) : z (z stored when zout is set).
-__global__ __launch_bounds__(256) void k_sur_pw(const float *__restrict__ v, const float *__restrict__ s,
+template <typename TA>
+__global__ __launch_bounds__(256) void k_sur_pw(const TA *__restrict__ v, const TA *__restrict__ s,
                                                 const float *__restrict__ lw, int act, float *__restrict__ zout,
-                                                float *__restrict__ vout, int64_t total) {
+                                                TA *__restrict__ vout, int64_t total) {
     // the weights are the same for every lane: read at uniform addresses, they
     // arrive through scalar loads instead of LDS
     const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
@@ -295,15 +316,15 @@ __global__ __launch_bounds__(256) void k_sur_pw(const float *__restrict__ v, con
     const int x = int(idx & (SN3 - 1));
     float vi[SW];
 #pragma unroll
-    for (int i = 0; i < SW; ++i) vi[i] = v[(b * SW + i) * SN3 + x];
+    for (int i = 0; i < SW; ++i) vi[i] = ldA(v, (b * SW + i) * SN3 + x);
 #pragma unroll 4
     for (int o = 0; o < SW; ++o) {
         const int64_t at = (b * SW + o) * SN3 + x;
-        float z = lw[SW * SW + o] + s[at];
+        float z = lw[SW * SW + o] + ldA(s, at);
 #pragma unroll
         for (int i = 0; i < SW; ++i) z += lw[o * SW + i] * vi[i];
         if (zout) zout[at] = z;
-        vout[at] = act ? gelu(z) : z;
+        stA(vout, at, act ? gelu(z) : z);
     }
 }
 
@@ -650,8 +671,9 @@ __global__ __launch_bounds__(256) void k_sur_mix_first(const float2 *__restrict_
 }
 
 // v_1(o) = gelu(A_o g + a0_o + s_0(o)) per point
-__global__ __launch_bounds__(256) void k_sur_pw_first(const float *__restrict__ g, const float *__restrict__ s,
-                                                      const float *__restrict__ fw, float *__restrict__ vout,
+template <typename TA>
+__global__ __launch_bounds__(256) void k_sur_pw_first(const float *__restrict__ g, const TA *__restrict__ s,
+                                                      const float *__restrict__ fw, TA *__restrict__ vout,
                                                       int64_t total) {
     const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (idx >= total) return;
@@ -661,7 +683,7 @@ __global__ __launch_bounds__(256) void k_sur_pw_first(const float *__restrict__ 
 #pragma unroll 8
     for (int o = 0; o < SW; ++o) {
         const int64_t at = (b * SW + o) * SN3 + x;
-        vout[at] = gelu(fw[FA + o] * gv + fw[FA0 + o] + s[at]);
+        stA(vout, at, gelu(fw[FA + o] * gv + fw[FA0 + o] + ldA(s, at)));
     }
 }
 
@@ -684,16 +706,17 @@ __global__ __launch_bounds__(256) void k_sur_mix_last(const float2 *__restrict__
 }
 
 // out(b,x) = QW . v(b,:,x) + qb + s1(b,x)
-__global__ __launch_bounds__(256) void k_sur_pw_last(const float *__restrict__ v, const float *__restrict__ s1,
+template <typename TA>
+__global__ __launch_bounds__(256) void k_sur_pw_last(const TA *__restrict__ v, const TA *__restrict__ s1,
                                                      const float *__restrict__ fw, float *__restrict__ out,
                                                      int64_t total) {
     const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (idx >= total) return;
     const int64_t b = idx >> 12;
     const int x = int(idx & (SN3 - 1));
-    float a = fw[FQB] + s1[idx];
+    float a = fw[FQB] + ldA(s1, idx);
 #pragma unroll 8
-    for (int i = 0; i < SW; ++i) a += fw[FQW + i] * v[(b * SW + i) * SN3 + x];
+    for (int i = 0; i < SW; ++i) a += fw[FQW + i] * ldA(v, (b * SW + i) * SN3 + x);
     out[idx] = a;
 }
 
@@ -752,6 +775,40 @@ ActPtrs act_ptrs(float *acts, int L, int64_t B, bool training) {
     return p;
 }
 
+// The folded inference pass with activations stored as A (FP32 or FP16) in
+// the FP32-sized regions of the inference activation layout.
+template <typename A>
+void forward_fused(int L, const float *theta, const float *fused, const float *g, int64_t batch, float *acts,
+                   float *out, hipStream_t st) {
+    const ActPtrs p = act_ptrs(acts, L, batch, false);
+    A *v[2] = {reinterpret_cast<A *>(p.v[0]), reinterpret_cast<A *>(p.v[1])};
+    A *sa = reinterpret_cast<A *>(p.s);
+    const int64_t pts = batch * SN3;
+    const unsigned slices = unsigned(batch * SW);
+    // first layer from the one-channel transform of g
+    hipLaunchKernelGGL(k_sur_dft<float>, dim3(unsigned(batch)), dim3(256), 0, st, g, 1.0f / SN3, p.X[0]);
+    hipLaunchKernelGGL(k_sur_mix_first, dim3(nblk(batch * SW * SM3)), dim3(256), 0, st, p.X[0], fused, p.Y,
+                       batch * SW * SM3);
+    hipLaunchKernelGGL(k_sur_idft<A>, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, sa);
+    hipLaunchKernelGGL(k_sur_pw_first<A>, dim3(nblk(pts)), dim3(256), 0, st, g, sa, fused, v[1], pts);
+    for (int l = 1; l < L - 1; ++l) {
+        const float *lay = theta + 2 * SW + l * SLAYER;
+        const float2 *R = reinterpret_cast<const float2 *>(lay);
+        hipLaunchKernelGGL(k_sur_dft<A>, dim3(slices), dim3(256), 0, st, v[l & 1], 1.0f / SN3, p.X[l]);
+        hipLaunchKernelGGL(k_sur_mix_lds<0>, dim3(SM3 / 8, unsigned((batch + 31) / 32)), dim3(256), 0, st, p.X[l],
+                           R, p.Y, batch);
+        hipLaunchKernelGGL(k_sur_idft<A>, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, sa);
+        hipLaunchKernelGGL(k_sur_pw<A>, dim3(nblk(pts)), dim3(256), 0, st, v[l & 1], sa, lay + SR, 1, nullptr,
+                           v[(l + 1) & 1], pts);
+    }
+    // last layer and projection as one output channel
+    hipLaunchKernelGGL(k_sur_dft<A>, dim3(slices), dim3(256), 0, st, v[(L - 1) & 1], 1.0f / SN3, p.X[L - 1]);
+    hipLaunchKernelGGL(k_sur_mix_last, dim3(nblk(batch * SM3)), dim3(256), 0, st, p.X[L - 1], fused, p.Y,
+                       batch * SM3);
+    hipLaunchKernelGGL(k_sur_idft<A>, dim3(unsigned(batch)), dim3(256), 0, st, p.Y, 1.0f, 0, sa);
+    hipLaunchKernelGGL(k_sur_pw_last<A>, dim3(nblk(pts)), dim3(256), 0, st, v[(L - 1) & 1], sa, fused, out, pts);
+}
+
 }  // namespace
 
 extern "C" {
@@ -783,11 +840,11 @@ int fq_sur_forward(const fq_sur_shape *shape, const float *theta, const float *g
     for (int l = 0; l < L; ++l) {
         const float *lay = theta + 2 * SW + l * SLAYER;
         const float2 *R = reinterpret_cast<const float2 *>(lay);
-        hipLaunchKernelGGL(k_sur_dft, dim3(slices), dim3(256), 0, st, p.v[l], 1.0f / SN3, p.X[l]);
+        hipLaunchKernelGGL(k_sur_dft<float>, dim3(slices), dim3(256), 0, st, p.v[l], 1.0f / SN3, p.X[l]);
         hipLaunchKernelGGL(k_sur_mix_lds<0>, dim3(SM3 / 8, unsigned((batch + 31) / 32)), dim3(256), 0, st, p.X[l],
                            R, p.Y, batch);
-        hipLaunchKernelGGL(k_sur_idft, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, p.s);
-        hipLaunchKernelGGL(k_sur_pw, dim3(nblk(pts)), dim3(256), 0, st, p.v[l], p.s, lay + SR, int(l < L - 1),
+        hipLaunchKernelGGL(k_sur_idft<float>, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, p.s);
+        hipLaunchKernelGGL(k_sur_pw<float>, dim3(nblk(pts)), dim3(256), 0, st, p.v[l], p.s, lay + SR, int(l < L - 1),
                            tr ? p.z[l] : nullptr, p.v[l + 1], pts);
     }
     hipLaunchKernelGGL(k_sur_proj, dim3(nblk(pts)), dim3(256), 0, st, p.v[L], theta + 2 * SW + L * SLAYER, out, pts);
@@ -804,7 +861,7 @@ int fq_sur_fuse(const fq_sur_shape *shape, const float *theta, float *fused, voi
 }
 
 int fq_sur_forward_fused(const fq_sur_shape *shape, const float *theta, const float *fused, const float *g,
-                         int64_t batch, float *acts, float *out, void *stream) {
+                         int64_t batch, int32_t half_acts, float *acts, float *out, void *stream) {
     if (int rc = check_shape(shape, "fq_sur_forward_fused")) return rc;
     if (shape->layers < 2)
         return fq_internal_fail(FQ_ERR_UNSUPPORTED, "fq_sur_forward_fused: needs at least two layers");
@@ -812,32 +869,8 @@ int fq_sur_forward_fused(const fq_sur_shape *shape, const float *theta, const fl
         return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_forward_fused: invalid arguments");
     if (batch == 0) return FQ_SUCCESS;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const int L = shape->layers;
-    const ActPtrs p = act_ptrs(acts, L, batch, false);
-    const int64_t pts = batch * SN3;
-    const unsigned slices = unsigned(batch * SW);
-    // first layer from the one-channel transform of g
-    hipLaunchKernelGGL(k_sur_dft, dim3(unsigned(batch)), dim3(256), 0, st, g, 1.0f / SN3, p.X[0]);
-    hipLaunchKernelGGL(k_sur_mix_first, dim3(nblk(batch * SW * SM3)), dim3(256), 0, st, p.X[0], fused, p.Y,
-                       batch * SW * SM3);
-    hipLaunchKernelGGL(k_sur_idft, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, p.s);
-    hipLaunchKernelGGL(k_sur_pw_first, dim3(nblk(pts)), dim3(256), 0, st, g, p.s, fused, p.v[1], pts);
-    for (int l = 1; l < L - 1; ++l) {
-        const float *lay = theta + 2 * SW + l * SLAYER;
-        const float2 *R = reinterpret_cast<const float2 *>(lay);
-        hipLaunchKernelGGL(k_sur_dft, dim3(slices), dim3(256), 0, st, p.v[l], 1.0f / SN3, p.X[l]);
-        hipLaunchKernelGGL(k_sur_mix_lds<0>, dim3(SM3 / 8, unsigned((batch + 31) / 32)), dim3(256), 0, st, p.X[l],
-                           R, p.Y, batch);
-        hipLaunchKernelGGL(k_sur_idft, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, p.s);
-        hipLaunchKernelGGL(k_sur_pw, dim3(nblk(pts)), dim3(256), 0, st, p.v[l], p.s, lay + SR, 1, nullptr,
-                           p.v[l + 1], pts);
-    }
-    // last layer and projection as one output channel
-    hipLaunchKernelGGL(k_sur_dft, dim3(slices), dim3(256), 0, st, p.v[L - 1], 1.0f / SN3, p.X[L - 1]);
-    hipLaunchKernelGGL(k_sur_mix_last, dim3(nblk(batch * SM3)), dim3(256), 0, st, p.X[L - 1], fused, p.Y,
-                       batch * SM3);
-    hipLaunchKernelGGL(k_sur_idft, dim3(unsigned(batch)), dim3(256), 0, st, p.Y, 1.0f, 0, p.s);
-    hipLaunchKernelGGL(k_sur_pw_last, dim3(nblk(pts)), dim3(256), 0, st, p.v[L - 1], p.s, fused, out, pts);
+    if (half_acts) forward_fused<_Float16>(shape->layers, theta, fused, g, batch, acts, out, st);
+    else forward_fused<float>(shape->layers, theta, fused, g, batch, acts, out, st);
     return hip_status("fq_sur_forward_fused");
 }
 
@@ -867,12 +900,12 @@ int fq_sur_backward(const fq_sur_shape *shape, const float *theta, const float *
                            int(l < L - 1), gvin, pts);
         hipLaunchKernelGGL((k_sur_wgrad<SW, SW>), wg, dim3(256), 0, st, gv, p.v[l], grad + lo + SR,
                            grad + lo + SR + SW * SW);
-        hipLaunchKernelGGL(k_sur_dft, dim3(slices), dim3(256), 0, st, gv, 1.0f, gY);
+        hipLaunchKernelGGL(k_sur_dft<float>, dim3(slices), dim3(256), 0, st, gv, 1.0f, gY);
         hipLaunchKernelGGL(k_sur_mix_grad, dim3(nblk(int64_t(SW) * SW * SM3)), dim3(256), 0, st, p.X[l], gY, gR,
                            batch);
         hipLaunchKernelGGL(k_sur_mix_lds<1>, dim3(SM3 / 8, unsigned((batch + 31) / 32)), dim3(256), 0, st, gY, R,
                            gX, batch);
-        hipLaunchKernelGGL(k_sur_idft, dim3(slices), dim3(256), 0, st, gX, 1.0f / SN3, 1, gvin);
+        hipLaunchKernelGGL(k_sur_idft<float>, dim3(slices), dim3(256), 0, st, gX, 1.0f / SN3, 1, gvin);
         float *tmp = gv;
         gv = gvin;
         gvin = tmp;

@@ -22,7 +22,9 @@ the unit Maxwellian; q_ref is the RMS of Q / rho^2 over the training set).
 Extra configuration keys: ``[scattering-model] surrogate-weights`` (an .npz
 written by :meth:`FourierSurrogate.save`), ``surrogate-tol`` (default 1e-4),
 ``surrogate-check-every`` (default 64; 0 turns the sampled check off),
-``surrogate-check-tol`` (default 3e-2) and ``surrogate-layers`` (default 4).
+``surrogate-check-tol`` (default 3e-2), ``surrogate-activations`` (``fp32``,
+the default, or ``fp16``: activations stored in FP16 between the kernels of
+the inference pass, arithmetic in FP32) and ``surrogate-layers`` (default 4).
 Without weights the network is the seeded initialisation and every DOF
 falls back.
 """
@@ -86,6 +88,7 @@ class FourierSurrogate:
         self._bufs = {}
         self._fused = None           # folded inference weights, valid for theta version _fused_ver
         self._ver, self._fused_ver = 0, -1
+        self.half_acts = False       # FP16 activation storage in folded inference
 
     def invalidate(self):
         """Call after changing theta in place (train_step does it itself)."""
@@ -111,10 +114,12 @@ class FourierSurrogate:
         return _stream(self.backend)
 
     # -- network -----------------------------------------------------------
-    def forward(self, g, out, training=False, fused=True):
+    def forward(self, g, out, training=False, fused=True, half=None):
         """out[b] = network(g[b]) for (batch, 16^3) float32 device tensors.
         Inference (training False) with two or more layers runs the folded
-        first and last layers (fq_sur_forward_fused) unless fused is False."""
+        first and last layers (fq_sur_forward_fused) unless fused is False;
+        half (default: self.half_acts) stores its activations in FP16."""
+        half = self.half_acts if half is None else half
         batch = g.shape[0]
         _, act, _ = self._sizes(batch, training)
         acts = self._buf('acts_train' if training else 'acts', act)
@@ -127,8 +132,8 @@ class FourierSurrogate:
                                            self._stream()))
                 self._fused_ver = self._ver
             _lib.check(lib.fq_sur_forward_fused(C.byref(self.shape), self.theta.data_ptr(), self._fused.data_ptr(),
-                                                g.data_ptr(), batch, acts.data_ptr(), out.data_ptr(),
-                                                self._stream()))
+                                                g.data_ptr(), batch, int(bool(half)), acts.data_ptr(),
+                                                out.data_ptr(), self._stream()))
             return acts
         _lib.check(lib.fq_sur_forward(C.byref(self.shape), self.theta.data_ptr(), g.data_ptr(), batch,
                                       int(training), acts.data_ptr(), out.data_ptr(), self._stream()))
@@ -196,11 +201,11 @@ class FourierSurrogate:
                 break
         return hist
 
-    def predict(self, g, chunk=8192, fused=True):
+    def predict(self, g, chunk=8192, fused=True, half=None):
         """Network output for (batch, 16^3) float32 rows, in chunks."""
         out = self.torch.empty_like(g)
         for b0 in range(0, g.shape[0], chunk):
-            self.forward(g[b0:b0 + chunk], out[b0:b0 + chunk], fused=fused)
+            self.forward(g[b0:b0 + chunk], out[b0:b0 + chunk], fused=fused, half=half)
         return out
 
     # -- persistence -------------------------------------------------------
@@ -256,6 +261,10 @@ class DGFSVHSGLLFNOScatteringModel(DGFSVHSGLLScatteringModel):
         self.check_tol = self.cfg.getfloat('scattering-model', 'surrogate-check-tol', 3e-2)
         self.inplace_runs = 4        # more runs of flagged DOFs than this: one compact batch
         self.net = FourierSurrogate.load(self.backend, path) if path else FourierSurrogate(self.backend, layers)
+        acts = self.cfg.get('scattering-model', 'surrogate-activations', 'fp32')
+        if acts not in ('fp32', 'fp16'):
+            raise ValueError("[scattering-model] surrogate-activations must be 'fp32' or 'fp16'")
+        self.net.half_acts = acts == 'fp16'
         gi, gd = gram(self.vm.cv(), self.vm.cw())
         torch = self.net.torch
         self._ginv = torch.tensor(gi.reshape(-1), dtype=torch.float64, device=self.backend.device)

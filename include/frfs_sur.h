@@ -62,10 +62,12 @@ int fq_sur_forward(const fq_sur_shape *shape, const float *theta, const float *g
  * the output of fq_sur_forward up to rounding, transforming one channel
  * instead of 32 in the first and last layers.  acts: the inference size
  * (training = 0) of fq_sur_sizes.  Recompute the folded weights whenever
- * theta changes. */
+ * theta changes.  half_acts != 0 stores the activations between kernels in
+ * FP16 (arithmetic stays FP32; half the activation bytes, about 1e-3
+ * relative change of the output). */
 int fq_sur_fuse(const fq_sur_shape *shape, const float *theta, float *fused, void *stream);
 int fq_sur_forward_fused(const fq_sur_shape *shape, const float *theta, const float *fused, const float *g,
-                         int64_t batch, float *acts, float *out, void *stream);
+                         int64_t batch, int32_t half_acts, float *acts, float *out, void *stream);
 
 /* grad += d(sum gout*out)/d theta, from the activations of a training
  * forward call on the same theta and g (grad is caller-zeroed; acts is

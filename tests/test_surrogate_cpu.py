@@ -161,11 +161,11 @@ def test_sur_abi_rejects_bad_shapes_and_arguments():
     one = _lib.fq_sur_shape(_lib.SUR_ABI_VERSION, 16, 32, 8, 1)
     assert lib.fq_sur_fuse(C.byref(one), 8, 8, None) == _lib.ERR_UNSUPPORTED
     assert lib.fq_sur_fuse(C.byref(good), None, 8, None) == _lib.ERR_INVALID
-    assert lib.fq_sur_forward_fused(C.byref(one), 8, 8, 8, 1, 8, 8, None) == _lib.ERR_UNSUPPORTED
-    assert lib.fq_sur_forward_fused(C.byref(good), 8, None, 8, 1, 8, 8, None) == _lib.ERR_INVALID
+    assert lib.fq_sur_forward_fused(C.byref(one), 8, 8, 8, 1, 0, 8, 8, None) == _lib.ERR_UNSUPPORTED
+    assert lib.fq_sur_forward_fused(C.byref(good), 8, None, 8, 1, 0, 8, 8, None) == _lib.ERR_INVALID
     # zero-sized work launches nothing and succeeds
     assert lib.fq_sur_forward(C.byref(good), 8, 8, 0, 0, 8, 8, None) == _lib.SUCCESS
-    assert lib.fq_sur_forward_fused(C.byref(good), 8, 8, 8, 0, 8, 8, None) == _lib.SUCCESS
+    assert lib.fq_sur_forward_fused(C.byref(good), 8, 8, 8, 0, 1, 8, 8, None) == _lib.SUCCESS
 
 
 def test_stand_in_plugin_is_registered():
