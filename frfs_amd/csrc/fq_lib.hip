@@ -898,7 +898,7 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
     const cpx<T> *__restrict__ FT, cpx<T> *__restrict__ P, const cpx<T> *__restrict__ ph,
     const cpx<T> *__restrict__ zc, const DevProd *__restrict__ prods, const int *__restrict__ g_p0,
     const int *__restrict__ g_np, const int *__restrict__ item_src, int ngroups, int nsrc,
-    const cpx<T> *__restrict__ NY, int nprods, const cpx<T> *__restrict__ zq) {
+    const cpx<T> *__restrict__ NY, int nprods, const cpx<T> *__restrict__ zq, const int *__restrict__ pconj) {
     using PG = ProdGeo<N>;
     using P1 = Prod1Geo<N>;
     using G3 = Geo<N>;
@@ -930,7 +930,9 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
         if constexpr (PACK) {
             cpx<T> e[N];
             __syncthreads();                               // LDS free
-            if constexpr (CONJ) {
+            // conjugate products (one source, opposite phases) take the cheaper
+            // z-stage: every product (CONJ) or those marked in pconj
+            if (CONJ || (pconj && pconj[pi])) {
                 const cpx<T> *zqp = zq + (int64_t(pi) * 4 + G) * (3 + 2 * PG::S);
                 if (G & 1) zstage_conj<N, T, P1::THREADS, true>(src1, zqp, ph1, lds);
                 else zstage_conj<N, T, P1::THREADS, false>(src1, zqp, ph1, lds);
@@ -1624,6 +1626,7 @@ struct fq_handle {
     void *FJ = nullptr;          // [chunk][sources][N^2] cpx<T> spectrum slice J = N/2 for k_nyq
     void *nyv = nullptr;         // [products][2][3][N][N] cpx<T> k_nyq weights
     bool pack_conj = false;      // every product: one source, conjugate phases (zstage_conj)
+    int *pconj = nullptr;        // general FP32 plans: per product, 1 = conjugate pair (zstage_conj)
     void *zq = nullptr;          // [products][4][3 + 2 N/4] cpx<T> zstage_conj coefficients
     cpx<float> *F32 = nullptr;   // FP32 copy of F (mixed, fused)
     // scratch
@@ -1692,10 +1695,15 @@ template <typename T> int upload_nyq(fq_handle *h) {
     const fq::Plan &P = h->P;
     const int N = P.N, H = N / 2;
     const size_t N2 = size_t(N) * N;
-    bool conj = true;
-    for (auto &pr : P.prods) {
-        conj = conj && P.items[pr.i1].src == P.items[pr.i2].src;
-        for (int d = 0; d < 3; ++d) conj = conj && P.items[pr.i2].th[d] == -P.items[pr.i1].th[d];
+    std::vector<int> pc(P.prods.size());
+    bool conj = true, anyconj = false;
+    for (size_t p = 0; p < P.prods.size(); ++p) {
+        const auto &pr = P.prods[p];
+        bool c = P.items[pr.i1].src == P.items[pr.i2].src;
+        for (int d = 0; d < 3; ++d) c = c && P.items[pr.i2].th[d] == -P.items[pr.i1].th[d];
+        pc[p] = c ? 1 : 0;
+        conj = conj && c;
+        anyconj = anyconj || c;
     }
     h->pack_conj = conj;
     // the FP64 (two-lane) product kernel packs conjugate pairs only
@@ -1735,7 +1743,14 @@ template <typename T> int upload_nyq(fq_handle *h) {
     rc = upload(&dv, v);
     h->nyv = dv;
     }
-    if (rc != FQ_SUCCESS || !conj) return rc;
+    if (rc != FQ_SUCCESS || !anyconj) return rc;
+    if (!conj) {
+        // general FP32 plan with some conjugate products: mark them for zstage_conj
+        int *dp = nullptr;
+        rc = upload(&dp, pc);
+        h->pconj = dp;
+        if (rc != FQ_SUCCESS) return rc;
+    }
     // zstage_conj coefficients of item 1 of every product, per vz-group G:
     // beta_b = e^{i th S b'} (b' = b, or b - 4 for b >= 2) times the sign of
     // w_4^{bG} (its factor i, for odd bG, is applied in the kernel);
@@ -1946,7 +1961,7 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
                 const cpx<T> *zq = reinterpret_cast<const cpx<T> *>(h->zq);
                 pre();
                 hipLaunchKernelGGL(kern, grid, blk1, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0, h->g_np,
-                                   h->item_src, ngroups, nsrc, NY, nprods, zq);
+                                   h->item_src, ngroups, nsrc, NY, nprods, zq, h->pconj);
                 post(7);
             }
         }
@@ -2255,7 +2270,8 @@ int fq_destroy(fq_handle *h) {
     for (void *p : {h->ph, (void *)h->item_src, (void *)h->prods, (void *)h->g_p0, (void *)h->g_np,
                     (void *)h->c_group, (void *)h->t_c0, (void *)h->t_cn, (void *)h->l_src, (void *)h->t_l0,
                     (void *)h->t_ln, (void *)h->W, (void *)h->ginv, (void *)h->pmass, (void *)h->F, h->Z,
-                    h->Pg, h->zc, (void *)h->F32, h->NY, h->FJ, (void *)h->nyv, (void *)h->zq})
+                    h->Pg, h->zc, (void *)h->F32, h->NY, h->FJ, (void *)h->nyv, (void *)h->zq,
+                    (void *)h->pconj})
         if (p) (void)hipFree(p);
     (void)hipThreadExchangeStreamCaptureMode(&mode);
     delete h;
