@@ -233,7 +233,7 @@ def bench_multi(a):
 
 
 def bench_vhs16(a):
-    """The survey's first configuration: plain FP64 vhs-gll (omega = 0.5),
+    """The CPU-runnable configuration: plain FP64 vhs-gll (omega = 0.5),
     16^3 grid on [-5,5]^3, M = 6 half-sphere directions, Nrho = 8, 64 DOFs of
     the seeded two-Maxwellian mixture, checked against the oracle on every
     DOF (the oracle also gives the CPU rate on the same DOFs)."""
