@@ -60,6 +60,20 @@ __device__ __forceinline__ float ldA(const _Float16 *p, int64_t i) { return floa
 __device__ __forceinline__ void stA(float *p, int64_t i, float v) { p[i] = v; }
 __device__ __forceinline__ void stA(_Float16 *p, int64_t i, float v) { p[i] = _Float16(v); }
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+// two neighbouring values (even index)
+__device__ __forceinline__ float2 ld2(const float *p, int64_t i) { return *reinterpret_cast<const float2 *>(p + i); }
+__device__ __forceinline__ float2 ld2(const _Float16 *p, int64_t i) {
+    const half2v h = *reinterpret_cast<const half2v *>(p + i);
+    return make_float2(float(h[0]), float(h[1]));
+}
+__device__ __forceinline__ void st2(float *p, int64_t i, float2 v) { *reinterpret_cast<float2 *>(p + i) = v; }
+__device__ __forceinline__ void st2(_Float16 *p, int64_t i, float2 v) {
+    half2v h;
+    h[0] = _Float16(v.x);
+    h[1] = _Float16(v.y);
+    *reinterpret_cast<half2v *>(p + i) = h;
+}
 
 __device__ __forceinline__ float gelu_grad(float z) {
     return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
@@ -304,27 +318,32 @@ __global__ __launch_bounds__(256) void k_sur_lift(const float *__restrict__ g, c
 }
 
 // z = Wm v + b + s; vout = act ? gelu(z) : z (z stored when zout is set).
+// One thread per two neighbouring grid points (total = points, even).
 template <typename TA>
 __global__ __launch_bounds__(256) void k_sur_pw(const TA *__restrict__ v, const TA *__restrict__ s,
                                                 const float *__restrict__ lw, int act, float *__restrict__ zout,
                                                 TA *__restrict__ vout, int64_t total) {
     // the weights are the same for every lane: read at uniform addresses, they
     // arrive through scalar loads instead of LDS
-    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const int64_t idx = 2 * (int64_t(blockIdx.x) * 256 + threadIdx.x);
     if (idx >= total) return;
     const int64_t b = idx >> 12;
     const int x = int(idx & (SN3 - 1));
-    float vi[SW];
+    float2 vi[SW];
 #pragma unroll
-    for (int i = 0; i < SW; ++i) vi[i] = ldA(v, (b * SW + i) * SN3 + x);
+    for (int i = 0; i < SW; ++i) vi[i] = ld2(v, (b * SW + i) * SN3 + x);
 #pragma unroll 4
     for (int o = 0; o < SW; ++o) {
         const int64_t at = (b * SW + o) * SN3 + x;
-        float z = lw[SW * SW + o] + ldA(s, at);
+        const float2 sv = ld2(s, at);
+        float z0 = lw[SW * SW + o] + sv.x, z1 = lw[SW * SW + o] + sv.y;
 #pragma unroll
-        for (int i = 0; i < SW; ++i) z += lw[o * SW + i] * vi[i];
-        if (zout) zout[at] = z;
-        stA(vout, at, act ? gelu(z) : z);
+        for (int i = 0; i < SW; ++i) {
+            z0 += lw[o * SW + i] * vi[i].x;
+            z1 += lw[o * SW + i] * vi[i].y;
+        }
+        if (zout) st2(zout, at, make_float2(z0, z1));
+        st2(vout, at, act ? make_float2(gelu(z0), gelu(z1)) : make_float2(z0, z1));
     }
 }
 
@@ -675,15 +694,17 @@ template <typename TA>
 __global__ __launch_bounds__(256) void k_sur_pw_first(const float *__restrict__ g, const TA *__restrict__ s,
                                                       const float *__restrict__ fw, TA *__restrict__ vout,
                                                       int64_t total) {
-    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const int64_t idx = 2 * (int64_t(blockIdx.x) * 256 + threadIdx.x);   // two points per thread
     if (idx >= total) return;
     const int64_t b = idx >> 12;
     const int x = int(idx & (SN3 - 1));
-    const float gv = g[idx];
+    const float2 gv = ld2(g, idx);
 #pragma unroll 8
     for (int o = 0; o < SW; ++o) {
         const int64_t at = (b * SW + o) * SN3 + x;
-        stA(vout, at, gelu(fw[FA + o] * gv + fw[FA0 + o] + ldA(s, at)));
+        const float2 sv = ld2(s, at);
+        st2(vout, at, make_float2(gelu(fw[FA + o] * gv.x + fw[FA0 + o] + sv.x),
+                                  gelu(fw[FA + o] * gv.y + fw[FA0 + o] + sv.y)));
     }
 }
 
@@ -790,7 +811,7 @@ void forward_fused(int L, const float *theta, const float *fused, const float *g
     hipLaunchKernelGGL(k_sur_mix_first, dim3(nblk(batch * SW * SM3)), dim3(256), 0, st, p.X[0], fused, p.Y,
                        batch * SW * SM3);
     hipLaunchKernelGGL(k_sur_idft<A>, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, sa);
-    hipLaunchKernelGGL(k_sur_pw_first<A>, dim3(nblk(pts)), dim3(256), 0, st, g, sa, fused, v[1], pts);
+    hipLaunchKernelGGL(k_sur_pw_first<A>, dim3(nblk(pts / 2)), dim3(256), 0, st, g, sa, fused, v[1], pts);
     for (int l = 1; l < L - 1; ++l) {
         const float *lay = theta + 2 * SW + l * SLAYER;
         const float2 *R = reinterpret_cast<const float2 *>(lay);
@@ -798,7 +819,7 @@ void forward_fused(int L, const float *theta, const float *fused, const float *g
         hipLaunchKernelGGL(k_sur_mix_lds<0>, dim3(SM3 / 8, unsigned((batch + 31) / 32)), dim3(256), 0, st, p.X[l],
                            R, p.Y, batch);
         hipLaunchKernelGGL(k_sur_idft<A>, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, sa);
-        hipLaunchKernelGGL(k_sur_pw<A>, dim3(nblk(pts)), dim3(256), 0, st, v[l & 1], sa, lay + SR, 1, nullptr,
+        hipLaunchKernelGGL(k_sur_pw<A>, dim3(nblk(pts / 2)), dim3(256), 0, st, v[l & 1], sa, lay + SR, 1, nullptr,
                            v[(l + 1) & 1], pts);
     }
     // last layer and projection as one output channel
@@ -844,7 +865,7 @@ int fq_sur_forward(const fq_sur_shape *shape, const float *theta, const float *g
         hipLaunchKernelGGL(k_sur_mix_lds<0>, dim3(SM3 / 8, unsigned((batch + 31) / 32)), dim3(256), 0, st, p.X[l],
                            R, p.Y, batch);
         hipLaunchKernelGGL(k_sur_idft<float>, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, p.s);
-        hipLaunchKernelGGL(k_sur_pw<float>, dim3(nblk(pts)), dim3(256), 0, st, p.v[l], p.s, lay + SR, int(l < L - 1),
+        hipLaunchKernelGGL(k_sur_pw<float>, dim3(nblk(pts / 2)), dim3(256), 0, st, p.v[l], p.s, lay + SR, int(l < L - 1),
                            tr ? p.z[l] : nullptr, p.v[l + 1], pts);
     }
     hipLaunchKernelGGL(k_sur_proj, dim3(nblk(pts)), dim3(256), 0, st, p.v[L], theta + 2 * SW + L * SLAYER, out, pts);
