@@ -387,16 +387,20 @@ def bench_fno16(a):
     e1.record(st)
     torch.cuda.synchronize()
     fwd_ms = e0.elapsed_time(e1) / a.steps
-    # the same with FP16 activation storage
-    out16 = torch.empty_like(g)
-    sm.net.forward(g, out16, half=True)
-    e0.record(st)
-    for _ in range(a.steps):
-        sm.net.forward(g, out16, half=True)
-    e1.record(st)
-    torch.cuda.synchronize()
-    fwd16_ms = e0.elapsed_time(e1) / a.steps
-    r16 = ((out16 - out).pow(2).sum(1) / out.pow(2).sum(1)).sqrt().cpu().numpy()
+    # the same with 16-bit activation storage
+    acts16 = {}
+    for kind in ('fp16', 'bf16'):
+        out16 = torch.empty_like(g)
+        sm.net.forward(g, out16, acts=kind)
+        e0.record(st)
+        for _ in range(a.steps):
+            sm.net.forward(g, out16, acts=kind)
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms16 = e0.elapsed_time(e1) / a.steps
+        r16 = ((out16 - out).pow(2).sum(1) / out.pow(2).sum(1)).sqrt().cpu().numpy()
+        acts16[kind] = {'network_forward_ms': ms16, 'speedup_vs_fp64': dt_fp64 / (ms16 * 1e-3),
+                        'rel_l2_vs_fp32_activations': {'median': float(np.median(r16)), 'max': float(r16.max())}}
     qa, qb = from_aosoa(qs), from_aosoa(qref)
     r = ((qa - qb).pow(2).sum(1) / qb.pow(2).sum(1)).sqrt().cpu().numpy()
     fl = stats.get('defect', np.zeros(ndof)) > sm.tol
@@ -447,9 +451,7 @@ def bench_fno16(a):
         'spectral_fp64_dof_evals_per_s': ndof / dt_fp64, 'spectral_mixed_dof_evals_per_s': ndof / dt_mixed,
         'speedup_vs_fp64': dt_fp64 / dt, 'network_forward_ms': fwd_ms,
         'network_only_speedup_vs_fp64': dt_fp64 / (fwd_ms * 1e-3),
-        'network_forward_ms_fp16_activations': fwd16_ms,
-        'fp16_activations_speedup_vs_fp64': dt_fp64 / (fwd16_ms * 1e-3),
-        'fp16_vs_fp32_activations_rel_l2': {'median': float(np.median(r16)), 'max': float(r16.max())},
+        'network_16bit_activations': acts16,
         'fallback_rate': float(fl.mean()),
         'at_defect_tol_3e-2': {'dof_evals_per_s': ndof / dt_l, 'speedup_vs_fp64': dt_fp64 / dt_l,
                                'fallback_rate': stats_l['fallback'] / ndof,

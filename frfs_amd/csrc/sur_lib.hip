@@ -54,26 +54,24 @@ __device__ __forceinline__ float cos16(int j) {
 __device__ __forceinline__ float sin16(int j) { return cos16(j - 4); }
 
 __device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
-// Activation storage: FP32, or FP16 for inference (arithmetic stays FP32).
-__device__ __forceinline__ float ldA(const float *p, int64_t i) { return p[i]; }
-__device__ __forceinline__ float ldA(const _Float16 *p, int64_t i) { return float(p[i]); }
-__device__ __forceinline__ void stA(float *p, int64_t i, float v) { p[i] = v; }
-__device__ __forceinline__ void stA(_Float16 *p, int64_t i, float v) { p[i] = _Float16(v); }
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef _Float16 half2v __attribute__((ext_vector_type(2)));
-// two neighbouring values (even index)
-__device__ __forceinline__ float2 ld2(const float *p, int64_t i) { return *reinterpret_cast<const float2 *>(p + i); }
-__device__ __forceinline__ float2 ld2(const _Float16 *p, int64_t i) {
-    const half2v h = *reinterpret_cast<const half2v *>(p + i);
-    return make_float2(float(h[0]), float(h[1]));
+// Activation storage: FP32, or FP16 / BF16 for inference (arithmetic stays
+// FP32; conversions round to nearest even).
+template <typename TA> __device__ __forceinline__ float ldA(const TA *p, int64_t i) { return float(p[i]); }
+template <typename TA> __device__ __forceinline__ void stA(TA *p, int64_t i, float v) { p[i] = TA(v); }
+// two neighbouring values (even index), one 4- or 8-byte access
+template <typename TA> struct alignas(2 * sizeof(TA)) Pair { TA a, b; };
+template <typename TA> __device__ __forceinline__ float2 ld2(const TA *p, int64_t i) {
+    const Pair<TA> h = *reinterpret_cast<const Pair<TA> *>(p + i);
+    return make_float2(float(h.a), float(h.b));
 }
-__device__ __forceinline__ void st2(float *p, int64_t i, float2 v) { *reinterpret_cast<float2 *>(p + i) = v; }
-__device__ __forceinline__ void st2(_Float16 *p, int64_t i, float2 v) {
-    half2v h;
-    h[0] = _Float16(v.x);
-    h[1] = _Float16(v.y);
-    *reinterpret_cast<half2v *>(p + i) = h;
+template <typename TA> __device__ __forceinline__ void st2(TA *p, int64_t i, float2 v) {
+    Pair<TA> h;
+    h.a = TA(v.x);
+    h.b = TA(v.y);
+    *reinterpret_cast<Pair<TA> *>(p + i) = h;
 }
+// eight 16-bit values, one 16-byte store
+template <typename TA> struct alignas(16) Pack8 { TA v[8]; };
 
 __device__ __forceinline__ float gelu_grad(float z) {
     return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
@@ -232,13 +230,14 @@ __global__ __launch_bounds__(256) void k_sur_idft(const float2 *__restrict__ Y, 
             }
             dst[q] = w;
         }
-    } else {                                  // FP16 inference output (never accumulated)
-        half8 *dst = reinterpret_cast<half8 *>(out + slice * SN3 + t * SN);
+    } else {                                  // 16-bit inference output (never accumulated)
+        static_assert(sizeof(TA) == 2, "16-bit storage");
+        Pack8<TA> *dst = reinterpret_cast<Pack8<TA> *>(out + slice * SN3 + t * SN);
 #pragma unroll
         for (int q = 0; q < SN / 8; ++q) {
-            half8 w;
+            Pack8<TA> w;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) w[j] = _Float16(o[8 * q + j]);
+            for (int j = 0; j < 8; ++j) w.v[j] = TA(o[8 * q + j]);
             dst[q] = w;
         }
     }
@@ -882,15 +881,17 @@ int fq_sur_fuse(const fq_sur_shape *shape, const float *theta, float *fused, voi
 }
 
 int fq_sur_forward_fused(const fq_sur_shape *shape, const float *theta, const float *fused, const float *g,
-                         int64_t batch, int32_t half_acts, float *acts, float *out, void *stream) {
+                         int64_t batch, int32_t acts_type, float *acts, float *out, void *stream) {
     if (int rc = check_shape(shape, "fq_sur_forward_fused")) return rc;
     if (shape->layers < 2)
         return fq_internal_fail(FQ_ERR_UNSUPPORTED, "fq_sur_forward_fused: needs at least two layers");
-    if (batch < 0 || batch > FQ_SUR_MAX_BATCH || !theta || !fused || !g || !acts || !out)
+    if (batch < 0 || batch > FQ_SUR_MAX_BATCH || !theta || !fused || !g || !acts || !out ||
+        acts_type < FQ_SUR_ACTS_FP32 || acts_type > FQ_SUR_ACTS_BF16)
         return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_forward_fused: invalid arguments");
     if (batch == 0) return FQ_SUCCESS;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    if (half_acts) forward_fused<_Float16>(shape->layers, theta, fused, g, batch, acts, out, st);
+    if (acts_type == FQ_SUR_ACTS_FP16) forward_fused<_Float16>(shape->layers, theta, fused, g, batch, acts, out, st);
+    else if (acts_type == FQ_SUR_ACTS_BF16) forward_fused<__bf16>(shape->layers, theta, fused, g, batch, acts, out, st);
     else forward_fused<float>(shape->layers, theta, fused, g, batch, acts, out, st);
     return hip_status("fq_sur_forward_fused");
 }

@@ -23,8 +23,9 @@ Extra configuration keys: ``[scattering-model] surrogate-weights`` (an .npz
 written by :meth:`FourierSurrogate.save`), ``surrogate-tol`` (default 1e-4),
 ``surrogate-check-every`` (default 64; 0 turns the sampled check off),
 ``surrogate-check-tol`` (default 3e-2), ``surrogate-activations`` (``fp32``,
-the default, or ``fp16``: activations stored in FP16 between the kernels of
-the inference pass, arithmetic in FP32) and ``surrogate-layers`` (default 4).
+the default, ``fp16`` or ``bf16``: the storage of the activations between the
+kernels of the inference pass, arithmetic in FP32) and ``surrogate-layers``
+(default 4).
 Without weights the network is the seeded initialisation and every DOF
 falls back.
 """
@@ -41,6 +42,7 @@ from .scattering import DGFSVHSGLLScatteringModel, _stream
 
 F_REF = math.pi ** -1.5
 NV, WIDTH, MODES = 16, 32, 8
+ACTS = {'fp32': 0, 'fp16': 1, 'bf16': 2}      # FQ_SUR_ACTS_*
 
 
 def gram(cv, cw):
@@ -88,7 +90,7 @@ class FourierSurrogate:
         self._bufs = {}
         self._fused = None           # folded inference weights, valid for theta version _fused_ver
         self._ver, self._fused_ver = 0, -1
-        self.half_acts = False       # FP16 activation storage in folded inference
+        self.acts = 'fp32'           # activation storage of folded inference: fp32, fp16 or bf16
 
     def invalidate(self):
         """Call after changing theta in place (train_step does it itself)."""
@@ -114,15 +116,16 @@ class FourierSurrogate:
         return _stream(self.backend)
 
     # -- network -----------------------------------------------------------
-    def forward(self, g, out, training=False, fused=True, half=None):
+    def forward(self, g, out, training=False, fused=True, acts=None):
         """out[b] = network(g[b]) for (batch, 16^3) float32 device tensors.
         Inference (training False) with two or more layers runs the folded
         first and last layers (fq_sur_forward_fused) unless fused is False;
-        half (default: self.half_acts) stores its activations in FP16."""
-        half = self.half_acts if half is None else half
+        acts (default: self.acts) is its activation storage, 'fp32', 'fp16'
+        or 'bf16'."""
+        acts_type = ACTS[self.acts if acts is None else acts]
         batch = g.shape[0]
         _, act, _ = self._sizes(batch, training)
-        acts = self._buf('acts_train' if training else 'acts', act)
+        buf = self._buf('acts_train' if training else 'acts', act)
         lib = _lib.load()
         if not training and fused and self.layers >= 2:
             if self._fused_ver != self._ver:
@@ -132,12 +135,12 @@ class FourierSurrogate:
                                            self._stream()))
                 self._fused_ver = self._ver
             _lib.check(lib.fq_sur_forward_fused(C.byref(self.shape), self.theta.data_ptr(), self._fused.data_ptr(),
-                                                g.data_ptr(), batch, int(bool(half)), acts.data_ptr(),
+                                                g.data_ptr(), batch, acts_type, buf.data_ptr(),
                                                 out.data_ptr(), self._stream()))
-            return acts
+            return buf
         _lib.check(lib.fq_sur_forward(C.byref(self.shape), self.theta.data_ptr(), g.data_ptr(), batch,
-                                      int(training), acts.data_ptr(), out.data_ptr(), self._stream()))
-        return acts
+                                      int(training), buf.data_ptr(), out.data_ptr(), self._stream()))
+        return buf
 
     def backward(self, g, acts, gout, grad):
         batch = g.shape[0]
@@ -201,11 +204,11 @@ class FourierSurrogate:
                 break
         return hist
 
-    def predict(self, g, chunk=8192, fused=True, half=None):
+    def predict(self, g, chunk=8192, fused=True, acts=None):
         """Network output for (batch, 16^3) float32 rows, in chunks."""
         out = self.torch.empty_like(g)
         for b0 in range(0, g.shape[0], chunk):
-            self.forward(g[b0:b0 + chunk], out[b0:b0 + chunk], fused=fused, half=half)
+            self.forward(g[b0:b0 + chunk], out[b0:b0 + chunk], fused=fused, acts=acts)
         return out
 
     # -- persistence -------------------------------------------------------
@@ -262,9 +265,9 @@ class DGFSVHSGLLFNOScatteringModel(DGFSVHSGLLScatteringModel):
         self.inplace_runs = 4        # more runs of flagged DOFs than this: one compact batch
         self.net = FourierSurrogate.load(self.backend, path) if path else FourierSurrogate(self.backend, layers)
         acts = self.cfg.get('scattering-model', 'surrogate-activations', 'fp32')
-        if acts not in ('fp32', 'fp16'):
-            raise ValueError("[scattering-model] surrogate-activations must be 'fp32' or 'fp16'")
-        self.net.half_acts = acts == 'fp16'
+        if acts not in ACTS:
+            raise ValueError("[scattering-model] surrogate-activations must be 'fp32', 'fp16' or 'bf16'")
+        self.net.acts = acts
         gi, gd = gram(self.vm.cv(), self.vm.cw())
         torch = self.net.torch
         self._ginv = torch.tensor(gi.reshape(-1), dtype=torch.float64, device=self.backend.device)

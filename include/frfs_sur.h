@@ -37,6 +37,9 @@ extern "C" {
 #define FQ_SUR_MAX_LAYERS 8
 #define FQ_SUR_MAX_BATCH 65536
 #define FQ_SUR_FUSED_FLOATS 65697   /* folded inference weights (fq_sur_fuse) */
+#define FQ_SUR_ACTS_FP32 0          /* activation storage of fq_sur_forward_fused */
+#define FQ_SUR_ACTS_FP16 1
+#define FQ_SUR_ACTS_BF16 2
 
 typedef struct fq_sur_shape {
     int32_t abi_version;   /* FQ_SUR_ABI_VERSION */
@@ -62,12 +65,12 @@ int fq_sur_forward(const fq_sur_shape *shape, const float *theta, const float *g
  * the output of fq_sur_forward up to rounding, transforming one channel
  * instead of 32 in the first and last layers.  acts: the inference size
  * (training = 0) of fq_sur_sizes.  Recompute the folded weights whenever
- * theta changes.  half_acts != 0 stores the activations between kernels in
- * FP16 (arithmetic stays FP32; half the activation bytes, about 1e-3
- * relative change of the output). */
+ * theta changes.  acts_type FQ_SUR_ACTS_FP16 or _BF16 stores the activations
+ * between kernels in 16 bits (arithmetic stays FP32; half the activation
+ * bytes; about 2e-3 (FP16) or 1e-2 (BF16) relative change of the output). */
 int fq_sur_fuse(const fq_sur_shape *shape, const float *theta, float *fused, void *stream);
 int fq_sur_forward_fused(const fq_sur_shape *shape, const float *theta, const float *fused, const float *g,
-                         int64_t batch, int32_t half_acts, float *acts, float *out, void *stream);
+                         int64_t batch, int32_t acts_type, float *acts, float *out, void *stream);
 
 /* grad += d(sum gout*out)/d theta, from the activations of a training
  * forward call on the same theta and g (grad is caller-zeroed; acts is

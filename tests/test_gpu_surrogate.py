@@ -336,18 +336,20 @@ def test_training_gain(backend):
     torch.cuda.synchronize()
 
 
-def test_fp16_activation_storage(backend):
-    """Folded inference with the activations stored in FP16 between kernels:
-    within 2e-3 (rel-L2 per DOF) of the FP32 storage and 3e-3 of the oracle."""
+@pytest.mark.parametrize('acts,tol', [('fp16', 2e-3), ('bf16', 3e-2)])
+def test_16bit_activation_storage(backend, acts, tol):
+    """Folded inference with the activations stored in 16 bits between
+    kernels: within tol (rel-L2 per DOF) of the FP32 storage and 1.5 tol of
+    the oracle (FP16 keeps 11 significant bits, BF16 8)."""
     from frfs_amd.surrogate import FourierSurrogate
     L = 4
     th = _theta(L, seed=17)
     g = _inputs(3, seed=27)
     net = FourierSurrogate(backend, layers=L, theta=th)
-    print('[surrogate fp16 activations]', flush=True)
-    o32 = net.predict(_dev(g), half=False).cpu().numpy()
-    o16 = net.predict(_dev(g), half=True).cpu().numpy()
+    print(f'[surrogate {acts} activations]', flush=True)
+    o32 = net.predict(_dev(g), acts='fp32').cpu().numpy()
+    o16 = net.predict(_dev(g), acts=acts).cpu().numpy()
     want = osur.forward(th, g, N, W, M, L)
     for b in range(3):
-        assert 0 < rel_l2(o16[b], o32[b]) <= 2e-3, b
-        assert rel_l2(o16[b], want[b]) <= 3e-3, b
+        assert 0 < rel_l2(o16[b], o32[b]) <= tol, b
+        assert rel_l2(o16[b], want[b]) <= 1.5 * tol, b

@@ -163,6 +163,7 @@ def test_sur_abi_rejects_bad_shapes_and_arguments():
     assert lib.fq_sur_fuse(C.byref(good), None, 8, None) == _lib.ERR_INVALID
     assert lib.fq_sur_forward_fused(C.byref(one), 8, 8, 8, 1, 0, 8, 8, None) == _lib.ERR_UNSUPPORTED
     assert lib.fq_sur_forward_fused(C.byref(good), 8, None, 8, 1, 0, 8, 8, None) == _lib.ERR_INVALID
+    assert lib.fq_sur_forward_fused(C.byref(good), 8, 8, 8, 1, 3, 8, 8, None) == _lib.ERR_INVALID
     # zero-sized work launches nothing and succeeds
     assert lib.fq_sur_forward(C.byref(good), 8, 8, 0, 0, 8, 8, None) == _lib.SUCCESS
     assert lib.fq_sur_forward_fused(C.byref(good), 8, 8, 8, 0, 1, 8, 8, None) == _lib.SUCCESS
