@@ -59,6 +59,8 @@ def parse():
     ap.add_argument('--fno-epochs', type=int, default=20)
     ap.add_argument('--fno-train-seconds', type=float, default=150.0)
     ap.add_argument('--fno-tol', type=float, default=1e-4)
+    ap.add_argument('--fno-train-acts', default='fp32', choices=['fp32', 'fp16', 'bf16'],
+                    help='fno16: activation storage while training')
     ap.add_argument('--fno-dofs', type=int, default=8192)
     ap.add_argument('--calib-copy-mib', type=int, default=0,
                     help='run one device-to-device copy of this size first (counter calibration)')
@@ -325,7 +327,8 @@ def bench_fno16(a):
         rep = {'weights': os.path.basename(a.fno_weights)}
     else:
         sm, rep = ts.train(be, pairs=a.fno_pairs, epochs=a.fno_epochs, time_limit=a.fno_train_seconds,
-                           seed=a.seed, log=lambda m: print(m, file=sys.stderr, flush=True))
+                           seed=a.seed, train_acts=a.fno_train_acts,
+                           log=lambda m: print(m, file=sys.stderr, flush=True))
         sm.tol = a.fno_tol
         vm = sm.vm
         if a.fno_save:

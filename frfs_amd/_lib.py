@@ -97,10 +97,10 @@ SIGNATURES = {
     'fq_plan_export': (C.c_int, [_vp] + [_vp] * 10),
     # include/frfs_sur.h
     'fq_sur_sizes': (C.c_int, [_vp, _i64, _i32, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]),
-    'fq_sur_forward': (C.c_int, [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp]),
+    'fq_sur_forward': (C.c_int, [_vp, _vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp]),
     'fq_sur_fuse': (C.c_int, [_vp, _vp, _vp, _vp]),
     'fq_sur_forward_fused': (C.c_int, [_vp, _vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp]),
-    'fq_sur_backward': (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp]),
+    'fq_sur_backward': (C.c_int, [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
     'fq_sur_loss': (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
     'fq_sur_adam': (C.c_int, [_i64, _vp, _vp, _vp, _vp, C.c_float, C.c_float, C.c_float, C.c_float,
                               _i32, _vp]),

@@ -305,22 +305,23 @@ __global__ __launch_bounds__(256) void k_sur_mix_lds(const float2 *__restrict__ 
 }
 
 // v_0(b, c, x) = P_c g(b, x) + p0_c
+template <typename TA>
 __global__ __launch_bounds__(256) void k_sur_lift(const float *__restrict__ g, const float *__restrict__ th,
-                                                  float *__restrict__ v, int64_t total) {
+                                                  TA *__restrict__ v, int64_t total) {
     const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (idx >= total) return;
     const int64_t b = idx >> 12;
     const int x = int(idx & (SN3 - 1));
     const float gv = g[idx];
 #pragma unroll 8
-    for (int c = 0; c < SW; ++c) v[(b * SW + c) * SN3 + x] = th[c] * gv + th[SW + c];
+    for (int c = 0; c < SW; ++c) stA(v, (b * SW + c) * SN3 + x, th[c] * gv + th[SW + c]);
 }
 
 // z = Wm v + b + s; vout = act ? gelu(z) : z (z stored when zout is set).
 // One thread per two neighbouring grid points (total = points, even).
 template <typename TA>
 __global__ __launch_bounds__(256) void k_sur_pw(const TA *__restrict__ v, const TA *__restrict__ s,
-                                                const float *__restrict__ lw, int act, float *__restrict__ zout,
+                                                const float *__restrict__ lw, int act, TA *__restrict__ zout,
                                                 TA *__restrict__ vout, int64_t total) {
     // the weights are the same for every lane: read at uniform addresses, they
     // arrive through scalar loads instead of LDS
@@ -347,7 +348,8 @@ __global__ __launch_bounds__(256) void k_sur_pw(const TA *__restrict__ v, const 
 }
 
 // out(b, x) = Qw . v_L(b, :, x) + q0
-__global__ __launch_bounds__(256) void k_sur_proj(const float *__restrict__ v, const float *__restrict__ qw,
+template <typename TA>
+__global__ __launch_bounds__(256) void k_sur_proj(const TA *__restrict__ v, const float *__restrict__ qw,
                                                   float *__restrict__ out, int64_t total) {
     const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (idx >= total) return;
@@ -355,7 +357,7 @@ __global__ __launch_bounds__(256) void k_sur_proj(const float *__restrict__ v, c
     const int x = int(idx & (SN3 - 1));
     float s = qw[SW];
 #pragma unroll 8
-    for (int c = 0; c < SW; ++c) s += qw[c] * v[(b * SW + c) * SN3 + x];
+    for (int c = 0; c < SW; ++c) s += qw[c] * ldA(v, (b * SW + c) * SN3 + x);
     out[idx] = s;
 }
 
@@ -373,7 +375,8 @@ __global__ __launch_bounds__(256) void k_sur_proj_bwd(const float *__restrict__ 
 
 // gz = gv * gelu'(z) (or gv for the last layer), written over gv;
 // gvin = Wm^T gz.
-__global__ __launch_bounds__(256) void k_sur_pw_bwd(float *__restrict__ gv, const float *__restrict__ z,
+template <typename TA>
+__global__ __launch_bounds__(256) void k_sur_pw_bwd(float *__restrict__ gv, const TA *__restrict__ z,
                                                     const float *__restrict__ lw, int act, float *__restrict__ gvin,
                                                     int64_t total) {
     // weight rows are read at uniform addresses (scalar loads), row o scaled
@@ -389,7 +392,7 @@ __global__ __launch_bounds__(256) void k_sur_pw_bwd(float *__restrict__ gv, cons
     for (int o = 0; o < SW; ++o) {
         const int64_t at = (b * SW + o) * SN3 + x;
         const float g = gv[at];
-        const float gz = act ? g * gelu_grad(z[at]) : g;
+        const float gz = act ? g * gelu_grad(ldA(z, at)) : g;
         gv[at] = gz;
 #pragma unroll
         for (int i = 0; i < SW; ++i) a[i] += lw[o * SW + i] * gz;
@@ -402,8 +405,8 @@ __global__ __launch_bounds__(256) void k_sur_pw_bwd(float *__restrict__ gv, cons
 // Grid 8 * batch: workgroup w sums the 512 points from (w % 8) * 512 of DOF
 // w / 8 in LDS tiles of 64.
 static_assert(SN3 / 512 == 8, "k_sur_wgrad splits a DOF into eight 512-point tiles");
-template <int COUT, int CIN>
-__global__ __launch_bounds__(256) void k_sur_wgrad(const float *__restrict__ A, const float *__restrict__ V,
+template <int COUT, int CIN, typename TV>
+__global__ __launch_bounds__(256) void k_sur_wgrad(const float *__restrict__ A, const TV *__restrict__ V,
                                                    float *__restrict__ G, float *__restrict__ gb) {
     constexpr int NO = COUT * CIN;
     constexpr int PER = (NO + 255) / 256;
@@ -418,7 +421,7 @@ __global__ __launch_bounds__(256) void k_sur_wgrad(const float *__restrict__ A, 
     float accb = 0.f;
     for (int p0 = x0; p0 < x0 + 512; p0 += 64) {
         for (int e = t; e < COUT * 64; e += 256) As[e >> 6][e & 63] = A[(b * COUT + (e >> 6)) * SN3 + p0 + (e & 63)];
-        for (int e = t; e < CIN * 64; e += 256) Vs[e >> 6][e & 63] = V[(b * CIN + (e >> 6)) * SN3 + p0 + (e & 63)];
+        for (int e = t; e < CIN * 64; e += 256) Vs[e >> 6][e & 63] = ldA(V, (b * CIN + (e >> 6)) * SN3 + p0 + (e & 63));
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
@@ -829,6 +832,68 @@ void forward_fused(int L, const float *theta, const float *fused, const float *g
     hipLaunchKernelGGL(k_sur_pw_last<A>, dim3(nblk(pts)), dim3(256), 0, st, v[(L - 1) & 1], sa, fused, out, pts);
 }
 
+// The unfolded pass (training keeps v_l, z_l and X_l) with activations
+// stored as A in the FP32-sized regions of the activation layout.
+template <typename A>
+void forward_impl(int L, const float *theta, const float *g, int64_t batch, bool tr, float *acts, float *out,
+                  hipStream_t st) {
+    const ActPtrs p = act_ptrs(acts, L, batch, tr);
+    const int64_t pts = batch * SN3;
+    const unsigned slices = unsigned(batch * SW);
+    A *sa = reinterpret_cast<A *>(p.s);
+    hipLaunchKernelGGL(k_sur_lift<A>, dim3(nblk(pts)), dim3(256), 0, st, g, theta, reinterpret_cast<A *>(p.v[0]), pts);
+    for (int l = 0; l < L; ++l) {
+        const float *lay = theta + 2 * SW + l * SLAYER;
+        const float2 *R = reinterpret_cast<const float2 *>(lay);
+        A *vl = reinterpret_cast<A *>(p.v[l]), *vn = reinterpret_cast<A *>(p.v[l + 1]);
+        hipLaunchKernelGGL(k_sur_dft<A>, dim3(slices), dim3(256), 0, st, vl, 1.0f / SN3, p.X[l]);
+        hipLaunchKernelGGL(k_sur_mix_lds<0>, dim3(SM3 / 8, unsigned((batch + 31) / 32)), dim3(256), 0, st, p.X[l],
+                           R, p.Y, batch);
+        hipLaunchKernelGGL(k_sur_idft<A>, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, sa);
+        hipLaunchKernelGGL(k_sur_pw<A>, dim3(nblk(pts / 2)), dim3(256), 0, st, vl, sa, lay + SR, int(l < L - 1),
+                           tr ? reinterpret_cast<A *>(p.z[l]) : nullptr, vn, pts);
+    }
+    hipLaunchKernelGGL(k_sur_proj<A>, dim3(nblk(pts)), dim3(256), 0, st, reinterpret_cast<const A *>(p.v[L]),
+                       theta + 2 * SW + L * SLAYER, out, pts);
+}
+
+// Gradients from the activations of a training forward_impl<A>; the
+// gradients themselves (gv, gz, the mode arrays) stay FP32.
+template <typename A>
+void backward_impl(int L, const float *theta, const float *g, int64_t batch, float *acts, const float *gout,
+                   float *grad, float *work, hipStream_t st) {
+    const ActPtrs p = act_ptrs(acts, L, batch, true);
+    const int64_t pts = batch * SN3, vol = batch * SW * SN3;
+    const unsigned slices = unsigned(batch * SW);
+    float *gv = work, *gvin = work + vol;
+    float2 *gY = reinterpret_cast<float2 *>(work + 2 * vol);
+    float2 *gX = gY + batch * SW * SM3;
+    const dim3 wg(unsigned(batch * (SN3 / 512)));
+    const int64_t qoff = 2 * SW + L * SLAYER;
+    hipLaunchKernelGGL((k_sur_wgrad<1, SW, A>), wg, dim3(256), 0, st, gout, reinterpret_cast<const A *>(p.v[L]),
+                       grad + qoff, grad + qoff + SW);
+    hipLaunchKernelGGL(k_sur_proj_bwd, dim3(nblk(pts)), dim3(256), 0, st, gout, theta + qoff, gv, pts);
+    for (int l = L - 1; l >= 0; --l) {
+        const int64_t lo = 2 * SW + l * SLAYER;
+        const float2 *R = reinterpret_cast<const float2 *>(theta + lo);
+        float2 *gR = reinterpret_cast<float2 *>(grad + lo);
+        hipLaunchKernelGGL(k_sur_pw_bwd<A>, dim3(nblk(pts)), dim3(256), 0, st, gv,
+                           reinterpret_cast<const A *>(p.z[l]), theta + lo + SR, int(l < L - 1), gvin, pts);
+        hipLaunchKernelGGL((k_sur_wgrad<SW, SW, A>), wg, dim3(256), 0, st, gv, reinterpret_cast<const A *>(p.v[l]),
+                           grad + lo + SR, grad + lo + SR + SW * SW);
+        hipLaunchKernelGGL(k_sur_dft<float>, dim3(slices), dim3(256), 0, st, gv, 1.0f, gY);
+        hipLaunchKernelGGL(k_sur_mix_grad, dim3(nblk(int64_t(SW) * SW * SM3)), dim3(256), 0, st, p.X[l], gY, gR,
+                           batch);
+        hipLaunchKernelGGL(k_sur_mix_lds<1>, dim3(SM3 / 8, unsigned((batch + 31) / 32)), dim3(256), 0, st, gY, R,
+                           gX, batch);
+        hipLaunchKernelGGL(k_sur_idft<float>, dim3(slices), dim3(256), 0, st, gX, 1.0f / SN3, 1, gvin);
+        float *tmp = gv;
+        gv = gvin;
+        gvin = tmp;
+    }
+    hipLaunchKernelGGL((k_sur_wgrad<SW, 1, float>), wg, dim3(256), 0, st, gv, g, grad, grad + SW);
+}
+
 }  // namespace
 
 extern "C" {
@@ -845,29 +910,17 @@ int fq_sur_sizes(const fq_sur_shape *shape, int64_t batch, int32_t training, int
 }
 
 int fq_sur_forward(const fq_sur_shape *shape, const float *theta, const float *g, int64_t batch, int32_t training,
-                   float *acts, float *out, void *stream) {
+                   int32_t acts_type, float *acts, float *out, void *stream) {
     if (int rc = check_shape(shape, "fq_sur_forward")) return rc;
-    if (batch < 0 || batch > FQ_SUR_MAX_BATCH || !theta || !g || !acts || !out)
+    if (batch < 0 || batch > FQ_SUR_MAX_BATCH || !theta || !g || !acts || !out ||
+        acts_type < FQ_SUR_ACTS_FP32 || acts_type > FQ_SUR_ACTS_BF16)
         return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_forward: invalid arguments");
     if (batch == 0) return FQ_SUCCESS;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const int L = shape->layers;
     const bool tr = training != 0;
-    const ActPtrs p = act_ptrs(acts, L, batch, tr);
-    const int64_t pts = batch * SN3;
-    const unsigned slices = unsigned(batch * SW);
-    hipLaunchKernelGGL(k_sur_lift, dim3(nblk(pts)), dim3(256), 0, st, g, theta, p.v[0], pts);
-    for (int l = 0; l < L; ++l) {
-        const float *lay = theta + 2 * SW + l * SLAYER;
-        const float2 *R = reinterpret_cast<const float2 *>(lay);
-        hipLaunchKernelGGL(k_sur_dft<float>, dim3(slices), dim3(256), 0, st, p.v[l], 1.0f / SN3, p.X[l]);
-        hipLaunchKernelGGL(k_sur_mix_lds<0>, dim3(SM3 / 8, unsigned((batch + 31) / 32)), dim3(256), 0, st, p.X[l],
-                           R, p.Y, batch);
-        hipLaunchKernelGGL(k_sur_idft<float>, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, p.s);
-        hipLaunchKernelGGL(k_sur_pw<float>, dim3(nblk(pts / 2)), dim3(256), 0, st, p.v[l], p.s, lay + SR, int(l < L - 1),
-                           tr ? p.z[l] : nullptr, p.v[l + 1], pts);
-    }
-    hipLaunchKernelGGL(k_sur_proj, dim3(nblk(pts)), dim3(256), 0, st, p.v[L], theta + 2 * SW + L * SLAYER, out, pts);
+    if (acts_type == FQ_SUR_ACTS_FP16) forward_impl<_Float16>(shape->layers, theta, g, batch, tr, acts, out, st);
+    else if (acts_type == FQ_SUR_ACTS_BF16) forward_impl<__bf16>(shape->layers, theta, g, batch, tr, acts, out, st);
+    else forward_impl<float>(shape->layers, theta, g, batch, tr, acts, out, st);
     return hip_status("fq_sur_forward");
 }
 
@@ -896,43 +949,17 @@ int fq_sur_forward_fused(const fq_sur_shape *shape, const float *theta, const fl
     return hip_status("fq_sur_forward_fused");
 }
 
-int fq_sur_backward(const fq_sur_shape *shape, const float *theta, const float *g, int64_t batch, float *acts,
-                    const float *gout, float *grad, float *work, void *stream) {
+int fq_sur_backward(const fq_sur_shape *shape, const float *theta, const float *g, int64_t batch, int32_t acts_type,
+                    float *acts, const float *gout, float *grad, float *work, void *stream) {
     if (int rc = check_shape(shape, "fq_sur_backward")) return rc;
-    if (batch < 0 || batch > FQ_SUR_MAX_BATCH || !theta || !g || !acts || !gout || !grad || !work)
+    if (batch < 0 || batch > FQ_SUR_MAX_BATCH || !theta || !g || !acts || !gout || !grad || !work ||
+        acts_type < FQ_SUR_ACTS_FP32 || acts_type > FQ_SUR_ACTS_BF16)
         return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_backward: invalid arguments");
     if (batch == 0) return FQ_SUCCESS;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const int L = shape->layers;
-    const ActPtrs p = act_ptrs(acts, L, batch, true);
-    const int64_t pts = batch * SN3, vol = batch * SW * SN3;
-    const unsigned slices = unsigned(batch * SW);
-    float *gv = work, *gvin = work + vol;
-    float2 *gY = reinterpret_cast<float2 *>(work + 2 * vol);
-    float2 *gX = gY + batch * SW * SM3;
-    const dim3 wg(unsigned(batch * (SN3 / 512)));
-    const int64_t qoff = 2 * SW + L * SLAYER;
-    hipLaunchKernelGGL((k_sur_wgrad<1, SW>), wg, dim3(256), 0, st, gout, p.v[L], grad + qoff, grad + qoff + SW);
-    hipLaunchKernelGGL(k_sur_proj_bwd, dim3(nblk(pts)), dim3(256), 0, st, gout, theta + qoff, gv, pts);
-    for (int l = L - 1; l >= 0; --l) {
-        const int64_t lo = 2 * SW + l * SLAYER;
-        const float2 *R = reinterpret_cast<const float2 *>(theta + lo);
-        float2 *gR = reinterpret_cast<float2 *>(grad + lo);
-        hipLaunchKernelGGL(k_sur_pw_bwd, dim3(nblk(pts)), dim3(256), 0, st, gv, p.z[l], theta + lo + SR,
-                           int(l < L - 1), gvin, pts);
-        hipLaunchKernelGGL((k_sur_wgrad<SW, SW>), wg, dim3(256), 0, st, gv, p.v[l], grad + lo + SR,
-                           grad + lo + SR + SW * SW);
-        hipLaunchKernelGGL(k_sur_dft<float>, dim3(slices), dim3(256), 0, st, gv, 1.0f, gY);
-        hipLaunchKernelGGL(k_sur_mix_grad, dim3(nblk(int64_t(SW) * SW * SM3)), dim3(256), 0, st, p.X[l], gY, gR,
-                           batch);
-        hipLaunchKernelGGL(k_sur_mix_lds<1>, dim3(SM3 / 8, unsigned((batch + 31) / 32)), dim3(256), 0, st, gY, R,
-                           gX, batch);
-        hipLaunchKernelGGL(k_sur_idft<float>, dim3(slices), dim3(256), 0, st, gX, 1.0f / SN3, 1, gvin);
-        float *tmp = gv;
-        gv = gvin;
-        gvin = tmp;
-    }
-    hipLaunchKernelGGL((k_sur_wgrad<SW, 1>), wg, dim3(256), 0, st, gv, g, grad, grad + SW);
+    if (acts_type == FQ_SUR_ACTS_FP16) backward_impl<_Float16>(shape->layers, theta, g, batch, acts, gout, grad, work, st);
+    else if (acts_type == FQ_SUR_ACTS_BF16) backward_impl<__bf16>(shape->layers, theta, g, batch, acts, gout, grad, work, st);
+    else backward_impl<float>(shape->layers, theta, g, batch, acts, gout, grad, work, st);
     return hip_status("fq_sur_backward");
 }
 

@@ -90,7 +90,8 @@ class FourierSurrogate:
         self._bufs = {}
         self._fused = None           # folded inference weights, valid for theta version _fused_ver
         self._ver, self._fused_ver = 0, -1
-        self.acts = 'fp32'           # activation storage of folded inference: fp32, fp16 or bf16
+        self.acts = 'fp32'           # activation storage of inference: fp32, fp16 or bf16
+        self.train_acts = 'fp32'     # activation storage of training (arithmetic and gradients FP32)
 
     def invalidate(self):
         """Call after changing theta in place (train_step does it itself)."""
@@ -120,9 +121,9 @@ class FourierSurrogate:
         """out[b] = network(g[b]) for (batch, 16^3) float32 device tensors.
         Inference (training False) with two or more layers runs the folded
         first and last layers (fq_sur_forward_fused) unless fused is False;
-        acts (default: self.acts) is its activation storage, 'fp32', 'fp16'
-        or 'bf16'."""
-        acts_type = ACTS[self.acts if acts is None else acts]
+        acts is the activation storage, 'fp32', 'fp16' or 'bf16' (default:
+        self.train_acts for training, self.acts for inference)."""
+        acts_type = ACTS[acts if acts is not None else (self.train_acts if training else self.acts)]
         batch = g.shape[0]
         _, act, _ = self._sizes(batch, training)
         buf = self._buf('acts_train' if training else 'acts', act)
@@ -139,16 +140,19 @@ class FourierSurrogate:
                                                 out.data_ptr(), self._stream()))
             return buf
         _lib.check(lib.fq_sur_forward(C.byref(self.shape), self.theta.data_ptr(), g.data_ptr(), batch,
-                                      int(training), buf.data_ptr(), out.data_ptr(), self._stream()))
+                                      int(training), acts_type, buf.data_ptr(), out.data_ptr(), self._stream()))
         return buf
 
-    def backward(self, g, acts, gout, grad):
+    def backward(self, g, acts, gout, grad, acts_kind=None):
+        """grad += gradient, from the activations of forward(training=True)
+        with the same storage (default self.train_acts)."""
         batch = g.shape[0]
         _, _, work = self._sizes(batch, True)
         w = self._buf('work', work)
+        kind = ACTS[self.train_acts if acts_kind is None else acts_kind]
         _lib.check(_lib.load().fq_sur_backward(C.byref(self.shape), self.theta.data_ptr(), g.data_ptr(), batch,
-                                               acts.data_ptr(), gout.data_ptr(), grad.data_ptr(), w.data_ptr(),
-                                               self._stream()))
+                                               kind, acts.data_ptr(), gout.data_ptr(), grad.data_ptr(),
+                                               w.data_ptr(), self._stream()))
 
     def gradient(self, g, target):
         """(loss sums [sum d^2, sum t^2], grad) of the mean-square loss."""

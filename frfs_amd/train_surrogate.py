@@ -54,14 +54,17 @@ def rel_l2_rows(p, t):
 
 
 def train(backend, *, pairs=200000, heldout=2048, epochs=20, batch=64, lr=2e-3, seed=0, time_limit=None,
-          layers=4, log=print):
-    """Returns (model, report).  The model's network is trained in place."""
+          layers=4, train_acts='fp32', log=print):
+    """Returns (model, report).  The model's network is trained in place;
+    train_acts is the activation storage of training ('fp32' or 'bf16':
+    BF16 storage with FP32 arithmetic, gradients and weights)."""
     import torch
     from .surrogate import DGFSVHSGLLFNOScatteringModel
     from .velocitymesh import DGFSVelocityMesh
     cfg = config(layers=layers)
     vm = DGFSVelocityMesh(backend, cfg)
     sm = DGFSVHSGLLFNOScatteringModel(backend, cfg, vm)
+    sm.net.train_acts = train_acts
     t0 = time.time()
     g, t, _ = make_pairs(sm, pairs + heldout, seed)
     torch.cuda.synchronize()
@@ -77,7 +80,8 @@ def train(backend, *, pairs=200000, heldout=2048, epochs=20, batch=64, lr=2e-3, 
     t_fit = time.time() - t0
     r = rel_l2_rows(sm.net.predict(g[pairs:]), t[pairs:])
     rep = {'pairs': pairs, 'heldout': heldout, 'epochs_run': len(hist), 'batch': batch, 'lr': lr,
-           'layers': layers, 'generate_s': round(t_gen, 2), 'train_s': round(t_fit, 2),
+           'layers': layers, 'train_acts': train_acts, 'generate_s': round(t_gen, 2),
+           'train_s': round(t_fit, 2),
            'train_rel_sq_err': hist[-1] if hist else None,
            'heldout_rel_l2_untrained_median': before,
            'heldout_rel_l2_median': float(r.median()), 'heldout_rel_l2_p90': float(r.quantile(0.9)),
@@ -94,6 +98,7 @@ def main():
     ap.add_argument('--batch', type=int, default=64)
     ap.add_argument('--lr', type=float, default=2e-3)
     ap.add_argument('--layers', type=int, default=4)
+    ap.add_argument('--train-acts', default='fp32', choices=['fp32', 'fp16', 'bf16'])
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--time-limit', type=float, default=None, help='seconds; stops after the running epoch')
     ap.add_argument('--out', required=True)
@@ -101,7 +106,7 @@ def main():
     from .backend import HIPBackend
     be = HIPBackend()
     sm, rep = train(be, pairs=a.pairs, heldout=a.heldout, epochs=a.epochs, batch=a.batch, lr=a.lr,
-                    seed=a.seed, time_limit=a.time_limit, layers=a.layers,
+                    seed=a.seed, time_limit=a.time_limit, layers=a.layers, train_acts=a.train_acts,
                     log=lambda s: print(s, flush=True))
     sm.net.save(a.out)
     print(json.dumps(rep), flush=True)

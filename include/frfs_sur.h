@@ -55,9 +55,13 @@ typedef struct fq_sur_shape {
 int fq_sur_sizes(const fq_sur_shape *shape, int64_t batch, int32_t training, int64_t *nparams,
                  int64_t *act_size, int64_t *work_size);
 
-/* out[batch][16^3] = network(g[batch][16^3]).  acts: fq_sur_sizes floats. */
+/* out[batch][16^3] = network(g[batch][16^3]).  acts: fq_sur_sizes floats.
+ * acts_type (FQ_SUR_ACTS_*) is the storage of the activations; arithmetic
+ * and the gradients stay FP32 (BF16 storage with FP32 accumulation for
+ * training, for example).  A training call and the fq_sur_backward call that
+ * uses its activations take the same acts_type. */
 int fq_sur_forward(const fq_sur_shape *shape, const float *theta, const float *g, int64_t batch,
-                   int32_t training, float *acts, float *out, void *stream);
+                   int32_t training, int32_t acts_type, float *acts, float *out, void *stream);
 
 /* Inference with the lift folded into the first layer and the projection
  * into the last (layers >= 2): fq_sur_fuse computes the folded weights
@@ -76,7 +80,7 @@ int fq_sur_forward_fused(const fq_sur_shape *shape, const float *theta, const fl
  * forward call on the same theta and g (grad is caller-zeroed; acts is
  * overwritten only in its scratch part). */
 int fq_sur_backward(const fq_sur_shape *shape, const float *theta, const float *g, int64_t batch,
-                    float *acts, const float *gout, float *grad, float *work, void *stream);
+                    int32_t acts_type, float *acts, const float *gout, float *grad, float *work, void *stream);
 
 /* gout = 2 (out - target) / count;  sums[0] += sum (out - target)^2,
  * sums[1] += sum target^2 (sums caller-zeroed, FP64). */

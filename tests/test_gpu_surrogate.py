@@ -101,8 +101,11 @@ def test_fused_inference_shallow(backend, L):
         assert rel_l2(of2[b], want2[b]) <= FWD_TOL, b
 
 
-@pytest.mark.parametrize('L', [1, 4])
-def test_backward_matches_oracle(backend, L):
+@pytest.mark.parametrize('L,acts,tol', [(1, 'fp32', GRAD_TOL), (4, 'fp32', GRAD_TOL), (4, 'bf16', 5e-2)])
+def test_backward_matches_oracle(backend, L, acts, tol):
+    """FP32 activation storage: every parameter block within 1e-4 of the
+    oracle.  BF16 storage (training in BF16 with FP32 accumulation): within
+    5e-2, the rounding of 8 significant bits carried through four layers."""
     import torch
     from frfs_amd.surrogate import FourierSurrogate
     th = _theta(L, seed=L)
@@ -110,23 +113,28 @@ def test_backward_matches_oracle(backend, L):
     rng = np.random.default_rng(L)
     go = rng.standard_normal(g.shape).astype(np.float32).astype(np.float64)
     net = FourierSurrogate(backend, layers=L, theta=th)
-    print(f'[surrogate backward L={L}]', flush=True)
+    net.train_acts = acts
+    print(f'[surrogate backward L={L} {acts}]', flush=True)
     gd = _dev(g)
     out = torch.empty_like(gd)
-    acts = net.forward(gd, out, training=True)
+    acts_buf = net.forward(gd, out, training=True)
     grad = torch.zeros(net.nparams, dtype=torch.float32, device='cuda')
-    net.backward(gd, acts, _dev(go), grad)
+    net.backward(gd, acts_buf, _dev(go), grad)
     got = grad.cpu().numpy().astype(np.float64)
     want = osur.backward(th, g, go, N, W, M, L)
+    if acts != 'fp32':
+        fw = osur.forward(th, g, N, W, M, L)
+        for b in range(2):
+            assert rel_l2(out.cpu().numpy()[b], fw[b]) <= tol, b
     off = osur.layout(W, M, L)
     blocks = [('P', W), ('p0', W), ('Qw', W), ('q0', 1)]
     blocks += [(f'R{l}', 2 * W * W * M ** 3) for l in range(L)]
     blocks += [(f'W{l}', W * W) for l in range(L)] + [(f'b{l}', W) for l in range(L)]
     for name, size in blocks:
         a = off[name]
-        assert rel_l2(got[a:a + size], want[a:a + size]) <= GRAD_TOL, name
+        assert rel_l2(got[a:a + size], want[a:a + size]) <= tol, name
     # the gradient accumulates into a caller-zeroed vector
-    net.backward(gd, acts, _dev(go), grad)
+    net.backward(gd, acts_buf, _dev(go), grad)
     assert rel_l2(grad.cpu().numpy(), 2 * got) <= 1e-6
 
 
@@ -307,15 +315,18 @@ def test_fallback_and_conservation(backend):
             assert np.all(a[u, :, e] == -1.0)
 
 
-def test_training_gain(backend):
+@pytest.mark.parametrize('acts', ['fp32', 'bf16'])
+def test_training_gain(backend, acts):
     """What the trained network gains over the method without it: 512
     training pairs from the 16^3 vhs-gll operator, 64 held out.  Without the
     network (Q = 0) the held-out rel-L2 is 1, the untrained network's is
-    larger; a short training run must bring it well below both."""
+    larger; a short training run, with FP32 or BF16 activation storage, must
+    bring it well below both."""
     import torch
     from frfs_amd.inputs import shear_mixture_device, shear_mixture_params
     from frfs_amd.surrogate import training_pairs
     sm, vm = _fno_model(backend, 1e-4)
+    sm.net.train_acts = acts
     f = shear_mixture_device(vm.cv(), shear_mixture_params(576, 40), device='cuda')
     g, t, _ = training_pairs(sm, f)
     qref = float(t[:512].pow(2).mean().sqrt())
@@ -326,10 +337,10 @@ def test_training_gain(backend):
         p = sm.net.predict(g[512:])
         return float(((p - t[512:]).pow(2).sum(1) / t[512:].pow(2).sum(1)).sqrt().median())
     before = held_out()
-    print(f'[surrogate training] untrained held-out median rel-L2 {before:.3e}', flush=True)
+    print(f'[surrogate training {acts}] untrained held-out median rel-L2 {before:.3e}', flush=True)
     hist = sm.net.fit(g[:512], t[:512], epochs=40, batch=32, lr=2e-3, seed=1)
     after = held_out()
-    print(f'[surrogate training] trained held-out median rel-L2 {after:.3e} '
+    print(f'[surrogate training {acts}] trained held-out median rel-L2 {after:.3e} '
           f'(train rel. sq. error {hist[0]:.3e} -> {hist[-1]:.3e})', flush=True)
     assert hist[-1] < 0.1 * hist[0]
     assert after < 0.5 and after < 0.5 * before

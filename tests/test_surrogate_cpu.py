@@ -148,9 +148,10 @@ def test_sur_abi_rejects_bad_shapes_and_arguments():
                       (_lib.fq_sur_shape(_lib.SUR_ABI_VERSION, 16, 32, 8, 9), _lib.ERR_UNSUPPORTED)):
         assert lib.fq_sur_sizes(C.byref(bad), 1, 0, C.byref(n), None, None) == code
         assert lib.fq_last_error()
-    assert lib.fq_sur_forward(C.byref(good), None, None, 1, 0, None, None, None) == _lib.ERR_INVALID
-    assert lib.fq_sur_forward(C.byref(good), 8, 8, _lib.SUR_MAX_BATCH + 1, 0, 8, 8, None) == _lib.ERR_INVALID
-    assert lib.fq_sur_backward(C.byref(good), 8, 8, 1, 8, None, 8, 8, None) == _lib.ERR_INVALID
+    assert lib.fq_sur_forward(C.byref(good), None, None, 1, 0, 0, None, None, None) == _lib.ERR_INVALID
+    assert lib.fq_sur_forward(C.byref(good), 8, 8, _lib.SUR_MAX_BATCH + 1, 0, 0, 8, 8, None) == _lib.ERR_INVALID
+    assert lib.fq_sur_forward(C.byref(good), 8, 8, 1, 1, 5, 8, 8, None) == _lib.ERR_INVALID
+    assert lib.fq_sur_backward(C.byref(good), 8, 8, 1, 0, 8, None, 8, 8, None) == _lib.ERR_INVALID
     assert lib.fq_sur_loss(-1, 8, 8, 8, 8, None) == _lib.ERR_INVALID
     assert lib.fq_sur_adam(10, 8, 8, 8, 8, C.c_float(1e-3), C.c_float(0.9), C.c_float(0.999),
                            C.c_float(1e-8), 0, None) == _lib.ERR_INVALID
@@ -165,7 +166,7 @@ def test_sur_abi_rejects_bad_shapes_and_arguments():
     assert lib.fq_sur_forward_fused(C.byref(good), 8, None, 8, 1, 0, 8, 8, None) == _lib.ERR_INVALID
     assert lib.fq_sur_forward_fused(C.byref(good), 8, 8, 8, 1, 3, 8, 8, None) == _lib.ERR_INVALID
     # zero-sized work launches nothing and succeeds
-    assert lib.fq_sur_forward(C.byref(good), 8, 8, 0, 0, 8, 8, None) == _lib.SUCCESS
+    assert lib.fq_sur_forward(C.byref(good), 8, 8, 0, 0, 2, 8, 8, None) == _lib.SUCCESS
     assert lib.fq_sur_forward_fused(C.byref(good), 8, 8, 8, 0, 1, 8, 8, None) == _lib.SUCCESS
 
 
