@@ -201,7 +201,7 @@ __global__ __launch_bounds__(256) void k_fwd_z(AosoaPtrs ptr, Dof dof, cpx<doubl
     using G = Geo<N>;
     constexpr int E = G::E;
     const int lt = threadIdx.x % G::LPP;
-    const int64_t unit = xcd_block() * (blockDim.x / G::LPP) + threadIdx.x / G::LPP;
+    const int64_t unit = int64_t(blockIdx.x) * (blockDim.x / G::LPP) + threadIdx.x / G::LPP;
     if (unit >= nunits) return;   // whole unit exits together (no barriers here)
     const int I = int(unit % N);
     const int64_t ds = unit / N;
@@ -1919,7 +1919,7 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     int64_t u;
     u = int64_t(D) * nsrc * N;
     pre();
-    hipLaunchKernelGGL(k_fwd_z<N>, dim3(xcd8(blocks(u))), bt, 0, st, ptr, dof, T0, nsrc, u);
+    hipLaunchKernelGGL(k_fwd_z<N>, dim3(blocks(u)), bt, 0, st, ptr, dof, T0, nsrc, u);
     post(0);
     // the fused FP32 path reads an FP32 copy of F, written by k_fwd_xy itself
     const bool want32 = h->fused && std::is_same<T, float>::value;
