@@ -285,8 +285,14 @@ __global__ __launch_bounds__(256) void k_sur_mix_lds(const float2 *__restrict__ 
     float2 acc[SW];
 #pragma unroll
     for (int j = 0; j < SW; ++j) acc[j] = make_float2(0.f, 0.f);
+    // all 32 inputs in flight at once: occupancy is set by the LDS tile (two
+    // waves per SIMD), so the registers are free
+    float2 xs[SW];
+#pragma unroll
+    for (int j = 0; j < SW; ++j) xs[j] = a[(b * SW + j) * SM3 + k];
+#pragma unroll
     for (int j = 0; j < SW; ++j) {
-        const float2 x = a[(b * SW + j) * SM3 + k];
+        const float2 x = xs[j];
 #pragma unroll
         for (int q = 0; q < SW; ++q) {
             if constexpr (MODE == 0) {      // j = i, q = o
