@@ -726,7 +726,9 @@ __global__ __launch_bounds__(ProdGeo<N>::THREADS, (sizeof(T) == 4 ? 4 : 2)) void
     // Nyquist corrections as in k_prod1
     constexpr int NYL = PACK ? N * N + 2 * PG::S * N : 0;
     __shared__ cpx<T> lds[PG::S * PG::PLANE + NYL];
-    const int64_t unit = blockIdx.x;
+    // XCD-contiguous order when the grid allows it: the 4 * ngroups
+    // workgroups of a DOF then share one L2 for its spectrum and corrections
+    const int64_t unit = (gridDim.x & 7) == 0 ? xcd_block() : int64_t(blockIdx.x);
     const int G = int(unit % 4);
     const int g = int((unit / 4) % ngroups);
     const int slot = int(unit / (4 * int64_t(ngroups)));
@@ -907,7 +909,9 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
     // K piece [vy][vx], J piece [k][vx] and I piece [k][vy] for vz = G + 4k
     constexpr int NYL = PACK ? N * N + 2 * PG::S * N : 0;
     __shared__ cpx<T> lds[PG::S * PG::PLANE + NYL];
-    const int64_t unit = blockIdx.x;
+    // XCD-contiguous order when the grid allows it: the 4 * ngroups
+    // workgroups of a DOF then share one L2 for its spectrum and corrections
+    const int64_t unit = (gridDim.x & 7) == 0 ? xcd_block() : int64_t(blockIdx.x);
     const int G = int(unit % 4);
     const int g = int((unit / 4) % ngroups);
     const int slot = int(unit / (4 * int64_t(ngroups)));
