@@ -1,6 +1,6 @@
 #!/usr/bin/env python3
 """Benchmark: Q(f,f) DOF-evals/s on one MI355X, mixed (FP32 transforms, FP64
-accumulation) vs plain FP64, at the workload
+accumulation and FP64 conservation projection) vs plain FP64, at the workload
 
     vss-gll, 32^3 velocity grid on [-6,6]^3, M = 12 (full sphere),
     Nrho = 16, omega = 0.74, alpha = 1.4, Mpre = 32, 4096 DOFs of seeded
@@ -8,6 +8,9 @@ accumulation) vs plain FP64, at the workload
 
 A step is one evaluation of Q for all DOFs (inputs resident in HBM).  Prints
 one JSON line.  ``--gpus N`` with N > 1 exits: this build runs on one GPU.
+Other workloads (``--workload``): vhs16 (plain FP64, 16^3, 64 DOFs), vss32x2
+(two species, four pairs), rhs24 (SSP-RK2 step with FR advection) and fno16
+(the learned stand-in against the spectral operator on 16^3, 8192 DOFs).
 """
 import argparse
 import json
