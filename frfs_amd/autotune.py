@@ -8,10 +8,12 @@ between lanes); and the largest batch the scratch budget allows or a smaller
 one.
 ``tune`` builds one handle per candidate on the current device, times each on
 the caller's own inputs (one untimed call, then ``reps`` timed calls between
-device synchronisations), keeps the fastest handle and releases the others.
-At most two handles exist at a time.  Decisions are cached for the process,
-keyed by the plan shape and the DOF count, so a later model with the same key
-builds its handle from the cache without timing.
+device synchronisations) and releases it before the next is built, so at most
+one handle (one scratch allocation) exists at a time; the fastest schedule is
+then built once more and returned.  The caller's ``max-chunk`` is an upper
+bound on every candidate's batch.  Decisions are cached for the process,
+keyed by the plan shape, the DOF count and that bound, so a later model with
+the same key builds its handle from the cache without timing.
 """
 import time
 
@@ -32,6 +34,14 @@ def _sync():
     torch.cuda.synchronize()
 
 
+def _bounded(cand, limit):
+    """The candidate with its batch capped by the caller's limit (0: none)."""
+    mc = cand['max_chunk']
+    if limit > 0:
+        mc = limit if mc <= 0 else min(mc, limit)
+    return {**cand, 'max_chunk': mc}
+
+
 def _apply(params, cand):
     c = params.c
     fl = c.flags & ~(_lib.FLAG_UNFUSED | _lib.FLAG_TWO_LANE)
@@ -47,38 +57,44 @@ def tune(params, run, key, reps=2, candidates=CANDIDATES):
     """Return ``(handle, report)``; ``run(handle)`` evaluates the operator on
     the caller's inputs.  ``params`` is left with the chosen schedule."""
     orig = (params.c.flags, params.c.max_chunk)
+    limit = int(orig[1])
+    key = (key, limit)
     if key in _CACHE:
         cand = _CACHE[key]
         _apply(params, cand)
         return _lib.Handle(params), {'cached': True, 'choice': dict(cand)}
-    best, times = None, []
+    best, times, seen = None, [], []
     for cand in candidates:
+        cand = _bounded(cand, limit)
+        if cand in seen:                           # two candidates the limit made equal
+            continue
+        seen.append(cand)
         _apply(params, cand)
         try:
             h = _lib.Handle(params)
         except _lib.FQError as e:                 # e.g. scratch allocation failed
             times.append({**cand, 'error': str(e)})
             continue
-        run(h)
-        _sync()
-        t0 = time.perf_counter()
-        for _ in range(reps):
+        try:
             run(h)
-        _sync()
-        dt = (time.perf_counter() - t0) / reps
-        times.append({**cand, 'ms': dt * 1e3, 'chunk': h.chunk_dofs()})
+            _sync()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                run(h)
+            _sync()
+            dt = (time.perf_counter() - t0) / reps
+            times.append({**cand, 'ms': dt * 1e3, 'chunk': h.chunk_dofs()})
+        finally:
+            h.close()                              # one scratch allocation at a time
         if best is None or dt < best[0]:
-            if best is not None:
-                best[2].close()
-            best = (dt, cand, h)
-        else:
-            h.close()
+            best = (dt, cand)
     if best is None:
         params.c.flags, params.c.max_chunk = orig
         raise _lib.FQError(_lib.ERR_ALLOC, 'no schedule could be built: ' + str(times))
     _apply(params, best[1])
+    h = _lib.Handle(params)
     _CACHE[key] = dict(best[1])
-    return best[2], {'cached': False, 'choice': dict(best[1]), 'timings': times}
+    return h, {'cached': False, 'choice': dict(best[1]), 'timings': times}
 
 
 def clear_cache():

@@ -366,6 +366,8 @@ __global__ __launch_bounds__(kPlaneThreads) void k_plane(const cpx<T> *Z, cpx<T>
     __shared__ cpx<T> lds[UPB][N * G::LDSROW];
     const int u = threadIdx.x / G::LPP;
     const int lt = threadIdx.x % G::LPP;
+    // the launch pads the grid to a multiple of 8, so xcd_block() is a
+    // bijection of the grid; padded units fail `valid`
     const int64_t unit = xcd_block() * UPB + u;
     const bool valid = unit < nunits;
     const int vz = int(unit % N);
@@ -2006,10 +2008,10 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     for (int g = 0; g < ngroups; ++g) multi |= P.group_np[g] > 1;
     pre();
     if (multi)
-        hipLaunchKernelGGL((k_plane<N, T, true>), dim3(pblocks(u)), pt, 0, st, Z, Pg, ph, h->prods, h->g_p0,
+        hipLaunchKernelGGL((k_plane<N, T, true>), dim3(xcd8(pblocks(u))), pt, 0, st, Z, Pg, ph, h->prods, h->g_p0,
                            h->g_np, nitems, ngroups, u);
     else
-        hipLaunchKernelGGL((k_plane<N, T, false>), dim3(pblocks(u)), pt, 0, st, Z, Pg, ph, h->prods, h->g_p0,
+        hipLaunchKernelGGL((k_plane<N, T, false>), dim3(xcd8(pblocks(u))), pt, 0, st, Z, Pg, ph, h->prods, h->g_p0,
                            h->g_np, nitems, ngroups, u);
     post(3);
     }
@@ -2024,7 +2026,7 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
                            h->t_cn, h->l_src, h->t_l0, h->t_ln, int(P.contribs.size()), Y, ngroups, nsrc, ntgt, D, u);
     post(4);
     pre();
-    hipLaunchKernelGGL(k_out<N>, dim3(xcd8(pblocks(u))), pt, 0, st, Y, ptr, dof, ntgt, u);
+    hipLaunchKernelGGL(k_out<N>, dim3(pblocks(u)), pt, 0, st, Y, ptr, dof, ntgt, u);
     post(5);
     if (h->flags & FQ_FLAG_PROJECT) {
         pre();

@@ -74,6 +74,10 @@ def _stream(backend):
 
 
 class DGFSScatteringModel(metaclass=ABCMeta):
+    # fs_batch only enqueues device work (no host syncs, allocations or
+    # host-side branches on results), so a step may be captured in a HIP graph
+    capture_safe = True
+
     def __init__(self, backend, cfg, velocitymesh):
         self.backend = backend
         self.cfg = cfg
@@ -206,6 +210,8 @@ class DGFSVHSGLLNodalScatteringModel(DGFSVHSGLLScatteringModel):
 
 
 class DGFSBiScatteringModel(metaclass=ABCMeta):
+    capture_safe = True
+
     def __init__(self, backend, cfg, velocitymesh):
         self.backend = backend
         self.cfg = cfg

@@ -256,6 +256,8 @@ def training_pairs(model, f, chunk=8192):
 class DGFSVHSGLLFNOScatteringModel(DGFSVHSGLLScatteringModel):
     """vhs-gll with the learned stand-in in front of the spectral operator."""
     scattering_model = 'vhs-gll-fno'
+    # fs_batch reads fallback flags on the host and branches on them
+    capture_safe = False
 
     def perform_precomputation(self):
         super().perform_precomputation()
@@ -287,6 +289,11 @@ class DGFSVHSGLLFNOScatteringModel(DGFSVHSGLLScatteringModel):
         DGFSVHSGLLScatteringModel.fs_batch(self, d_arr_in, d_arr_out, first_dof, ndof)
 
     def fs_batch(self, d_arr_in, d_arr_out, first_dof=0, ndof=None, chunk=8192):
+        if d_arr_in._as_parameter_ == d_arr_out._as_parameter_:
+            # the fallback and the sampled check read f after the network's Q
+            # has been written, so Q may not alias f here
+            raise ValueError('vhs-gll-fno: d_arr_out must not alias d_arr_in (in-place evaluation '
+                             'is not supported by the learned stand-in)')
         torch = self.net.torch
         nrow, ldim, _ = d_arr_in.traits
         neles = d_arr_in.ioshape[2]

@@ -128,6 +128,10 @@ class BaseStepper:
         torch = self.system.backend.torch
         g = self._graphs.get(dt)
         if g is None:
+            if not getattr(self.system.sm, 'capture_safe', True):
+                raise ValueError(f'scattering model {self.system.sm.scattering_model!r} synchronises with '
+                                 'the host inside fs_batch and cannot be captured in a HIP graph; '
+                                 'use graph=False')
             # one-time schedule tuning runs eagerly, never inside a capture
             self.system.prepare()
             # release unreachable handles now rather than from a collection

@@ -41,7 +41,9 @@ def test_tune_picks_fastest_and_caches(monkeypatch):
     h, rep = autotune.tune(p, lambda h: time.sleep(h.cost), key=('k', 1), reps=1)
     assert rep['choice'] == {'fused': 0, 'max_chunk': 0, 'two_lane': 0}
     assert not h.closed
-    assert [x.closed for x in _FakeHandle.made] == [True, True, False]   # slower ones released
+    # every timed handle is released before the next is built; the winner is rebuilt
+    assert [x.closed for x in _FakeHandle.made] == [True, True, True, False]
+    assert _FakeHandle.made[-1] is h
     assert any('error' in t for t in rep['timings'])                 # failed candidate reported
     assert p.c.flags == 1 | 2 and p.c.max_chunk == 0                 # params keep the choice
     h2, rep2 = autotune.tune(_P(), lambda h: 1 / 0, key=('k', 1))    # cached: nothing is run
@@ -60,6 +62,30 @@ def test_tune_raises_when_nothing_builds(monkeypatch):
     with pytest.raises(_lib.FQError):
         autotune.tune(p, lambda h: None, key=('none',))
     assert p.c.flags == 1 and p.c.max_chunk == 0                     # restored
+
+
+def test_user_max_chunk_bounds_every_candidate(monkeypatch):
+    """[scattering-model] max-chunk is an upper bound: unbounded candidates
+    take it, larger ones are capped, and candidates it makes equal run once."""
+    from frfs_amd import _lib, autotune
+    seen = []
+
+    class H(_FakeHandle):
+        def __init__(self, params):
+            seen.append((params.c.flags & 6, params.c.max_chunk))
+            self.cost, self.closed = 0.0, False
+
+    monkeypatch.setattr(_lib, 'Handle', H)
+    monkeypatch.setattr(autotune, '_sync', lambda: None)
+    autotune.clear_cache()
+    p = _P()
+    p.c.max_chunk = 256
+    h, rep = autotune.tune(p, lambda h: None, key=('bounded',), reps=1)
+    assert all(mc == 256 for _, mc in seen)
+    # fused/auto and fused/512 collapse into one candidate under the bound
+    assert len(rep['timings']) == len(autotune.CANDIDATES) - 1
+    assert p.c.max_chunk == 256
+    autotune.clear_cache()
 
 
 @pytest.mark.gpu

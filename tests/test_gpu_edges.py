@@ -94,3 +94,22 @@ def test_range_errors(backend):
         sm.fs_batch(fin, q, first_dof=2, ndof=3)     # past nupts * neles
     with pytest.raises(FQError):
         sm.fs_batch(fin, q, first_dof=-1, ndof=1)
+
+
+@pytest.mark.parametrize('N,Nrho,dev,per_dof', [(16, 5, 5.0, True), (8, 4, 4.0, True), (24, 3, 5.0, False)])
+def test_staged_path_odd_grids(backend, N, Nrho, dev, per_dof):
+    """The staged product path (fused = 0) on launch grids that are not a
+    multiple of 8: per-DOF fs (one DOF per call) at 16^3 with Nrho = 5 and
+    at 8^3 with Nrho = 4, and an odd DOF count times groups at 24^3.  Every
+    (DOF, group, plane) unit must run exactly once."""
+    from oracle import fastspectral as fsp
+    from tests.cases import oracle_vhs, rel_l2
+    from tests.test_gpu_parity import _run_vhs
+    cfg = vhs_cfg(N, Nrho, 6, dev, fused=0)
+    vm, prm, tab = oracle_vhs(cfg)
+    f = fsp.two_maxwellian_batch(vm['cv'], 3, seed=N + Nrho)
+    print(f'[staged N={N} Nrho={Nrho} per_dof={per_dof}]', flush=True)
+    Q, _ = _run_vhs(backend, cfg, f, 1, 3, per_dof=per_dof)
+    errs = [rel_l2(Q[d], fsp.vhs_fs(f[d], vm, tab, prm['prefac'])) for d in range(3)]
+    print(N, Nrho, max(errs))
+    assert max(errs) <= 1e-12

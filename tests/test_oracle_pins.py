@@ -277,3 +277,95 @@ def test_moments_recover_maxwellian_parameters():
     assert abs(m[0] - 1.3) < 1e-8 and np.allclose(m[1:4], u, atol=1e-8) and abs(m[4] - T) < 1e-8
     assert np.allclose(m[5:8], 0, atol=1e-8)
     assert np.allclose(m[8:14], [1.3 * T, 0, 0, 1.3 * T, 0, 1.3 * T], atol=1e-8)
+
+
+# ---------------------------------------------------------------- VSS alpha != 1
+# The variable-soft-sphere kernel enters the oracle twice: the angular factor
+# (1 + sigma.sigma')^eta with eta = alpha - 1 in the b and c tables
+# (vss-gll.mako:80-85) and the prefactor alpha / (... 2^{1+alpha} ...)
+# (frfs/solvers/dgfsbi/scattering.py:86-88).  Two facts of the mathematics fix
+# both for alpha != 1.
+
+def _vss_alpha(alpha, N=8, Nrho=4, Mpre=192):
+    cfg = vss_cfg(N, Nrho, 12, 5.0, alpha={'00': alpha}, omega={'00': 0.74}, Mpre=Mpre)
+    return oracle_vss(cfg, 1)
+
+
+def test_vss_total_cross_section_independent_of_alpha():
+    """prefac(alpha) * c(l = 0) does not depend on alpha: over the sphere
+    int (1 + sigma.sigma')^{alpha-1} dsigma' = 2 pi 2^alpha / alpha, which the
+    prefactor's alpha / 2^{1+alpha} cancels.  A slip eta -> alpha moves the
+    product by about 17%, 2^{1+alpha} -> 2^{2+alpha} by a factor 2^{alpha-1}."""
+    vals = []
+    for alpha in (1.0, 1.4, 2.0):
+        vm, prm, tab = _vss_alpha(alpha)
+        c0 = tab['c']['00'][0]                     # l = (0, 0, 0) is the first index
+        assert abs(c0.imag) < 1e-12 * abs(c0.real)
+        vals.append(prm['prefac']['00'] * c0.real)
+    vals = np.array(vals)
+    # measured 9e-5 (the Mpre = 192 design integrates (1 + t)^0.4 to that accuracy)
+    assert np.max(np.abs(vals - vals[0])) < 1e-3 * abs(vals[0]), vals
+
+
+def test_vss_b_table_alpha2_closed_form():
+    """alpha = 2 (eta = 1): b[x, y](l) = 4 pi qz^{gamma+2} [j0(kappa) -
+    i (sigma_y . l_hat) j1(kappa)] with kappa = m_q/(m_p+m_q) (pi/L) qz_x |l|,
+    the spherical average of (1 + sigma_y.sigma') e^{-i kappa sigma'.l_hat},
+    which the degree-19 design integrates exactly to rounding for |kappa| < 2."""
+    vm, prm, tab = _vss_alpha(2.0)
+    l = fsp.lvec(8)
+    ln = np.linalg.norm(l, axis=0)
+    b = tab['b']['00']
+    g = prm['gamma']['00']
+    checked = 0
+    for x in range(vm['Nrho']):
+        kap = 0.5 * np.pi / vm['L'] * vm['qz'][x] * ln
+        sel = np.where((kap < 2.0) & (ln > 0))[0]
+        lhat = l[:, sel] / ln[sel]
+        for y in range(12):
+            sl = vm['sz'][y] @ lhat
+            want = 4 * np.pi * vm['qz'][x] ** (g + 2) * (sp.spherical_jn(0, kap[sel])
+                                                         - 1j * sl * sp.spherical_jn(1, kap[sel]))
+            assert np.max(np.abs(b[x, y, sel] - want)) < 1e-10 * np.max(np.abs(want))
+            checked += len(sel)
+        # l = 0: b = 4 pi qz^{gamma+2}
+        assert abs(b[x, 0, 0] - 4 * np.pi * vm['qz'][x] ** (g + 2)) < 1e-12 * abs(b[x, 0, 0])
+    assert checked > 1000
+
+
+@pytest.mark.parametrize('alpha', [1.4, 2.0])
+def test_vss_c_table_is_weighted_b_at_twice_l(alpha):
+    """c(l) = sum_{x,y} qw_x sw b[x, y](2 l) for one species: the loss table
+    is the gain table's angular sum without the mass factor m_q/(m_p+m_q) =
+    1/2, weighted by the radial and spherical quadrature (fac_c = qw sw fac_b,
+    vss-gll.mako:53-95).  This ties c to b, which the closed forms above and
+    the alpha = 1 plane-wave average pin; a missing qw, sw or mass factor, or
+    a c built from the wrong angular factor, fails it."""
+    N = 16
+    vm, prm, tab = _vss_alpha(alpha, N=N, Nrho=3, Mpre=48)
+    l = fsp.lvec(N)
+    b, c = tab['b']['00'], tab['c']['00']
+    sel = np.where(np.all((l >= -N // 4) & (l < N // 4), axis=0))[0]
+    l2 = 2 * l[:, sel]
+    idx2 = (((l2[0] % N) * N + (l2[1] % N)) * N + (l2[2] % N)).astype(int)
+    want = np.zeros(len(sel), dtype=np.complex128)
+    for x in range(vm['Nrho']):
+        for y in range(12):
+            want += vm['qw'][x] * vm['sw'] * b[x, y, idx2]
+    assert len(sel) == (N // 2) ** 3
+    assert np.max(np.abs(c[sel] - want)) < 1e-12 * np.max(np.abs(want))
+
+
+def test_bi_mesh_radius_reduces_to_single_species():
+    """The two-species support radius S (frfs/solvers/dgfsbi/velocitymesh.py:72-75)
+    at mass ratio 1 is the single-species S = 2L/(3 + sqrt 2)
+    (frfs/solvers/dgfs/velocitymesh.py:63); a heavier partner shrinks it."""
+    m0 = ARGON['molarMass0']
+    one = fsp.velocity_mesh(8, 4, 6, dev=5.0, **ARGON)
+    bi1 = fsp.velocity_mesh(8, 4, 12, dev=5.0, masses=[m0, m0], **ARGON)
+    bi2 = fsp.velocity_mesh(8, 4, 12, dev=5.0, masses=[m0, 2 * m0], **ARGON)
+    assert abs(bi1['S'] - one['S']) < 1e-15 * one['S']
+    assert abs(one['S'] - 2 * one['L'] / (3 + np.sqrt(2))) < 1e-15
+    # mass ratio 2: the first bound 2L/(8/3 + sqrt 3 + 1) is the smaller one
+    assert abs(bi2['S'] - 2 * bi2['L'] / (8 / 3 + np.sqrt(3) + 1)) < 1e-14 * bi2['S']
+    assert bi2['S'] < bi1['S']
