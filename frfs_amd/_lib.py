@@ -27,6 +27,8 @@ FLAG_PROJECT = 1
 FLAG_UNFUSED = 2
 FLAG_TWO_LANE = 4
 FLAG_NO_PACK = 8
+FLAG_NO_SCRATCH = 16
+NOALLOC_SCRATCH_BYTES = 256 << 20
 
 _dp = C.POINTER(C.c_double)
 
@@ -79,6 +81,7 @@ SIGNATURES = {
     'fq_get_plan_info': (C.c_int, [_vp, C.POINTER(fq_plan_info)]),
     'fq_scratch_bytes': (C.c_int, [_vp, C.POINTER(_i64)]),
     'fq_eval': (C.c_int, [_vp, _vp, _vp, _i32, _i64, _i32, _i64, _i64, _vp]),
+    'fq_eval_noalloc': (C.c_int, [_vp, _vp, _vp, _i32, _i64, _i32, _i64, _i64, _vp]),
     'fq_eval_multi': (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), _i32, _i64, _i32, _i64, _i64, _vp]),
     'fq_eval_bilinear': (C.c_int, [_vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64, _i64, _vp]),
     'fq_profile_enable': (C.c_int, [_vp, C.c_int]),
@@ -248,6 +251,9 @@ class Handle:
 
     def eval(self, f_ptr, q_ptr, nupts, ldim, neles, first_dof, ndof, stream=0):
         check(load().fq_eval(self._h, f_ptr, q_ptr, nupts, ldim, neles, first_dof, ndof, stream))
+
+    def eval_noalloc(self, f_ptr, q_ptr, nupts, ldim, neles, first_dof, ndof, stream=0):
+        check(load().fq_eval_noalloc(self._h, f_ptr, q_ptr, nupts, ldim, neles, first_dof, ndof, stream))
 
     def eval_multi(self, f_ptrs, q_ptrs, nupts, ldim, neles, first_dof, ndof, stream=0):
         n = len(f_ptrs)

@@ -31,12 +31,23 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/frfs_fq.h"
 #include "fft.hpp"
 #include "plan.hpp"
+
+// ---------------------------------------------------------------------------
+// Static scratch of fq_eval_noalloc (declared in frfs_fq.h): a file-scope
+// device array, one copy per GPU, held until the process ends.  Host code
+// never uses its name as an address (on the host it is only a stand-in); the
+// device address comes from hipGetSymbolAddress, resolved once per device
+// with that device current (at fq_create) and kept per device.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[FQ_NOALLOC_SCRATCH_BYTES];
 
 using fq::cpx;
 
@@ -1608,6 +1619,20 @@ struct fq_plan {
     fq::Plan P;
 };
 
+// Device scratch of one internal batch: the forward spectra F (FP64) and
+// their FP32 copy, Z (staged inverse transforms, also T0 and Y), the
+// product volumes Pg and the Nyquist-plane data NY/FJ of the pair-packed
+// product kernels.  Every kernel writes what it later reads, so the memory
+// needs no clearing between calls.
+struct Scratch {
+    cpx<double> *F = nullptr;
+    void *Z = nullptr;
+    void *Pg = nullptr;
+    cpx<float> *F32 = nullptr;
+    void *NY = nullptr;
+    void *FJ = nullptr;
+};
+
 struct fq_handle {
     fq::Plan P;
     int precision = 0;
@@ -1640,6 +1665,10 @@ struct fq_handle {
     void *Z = nullptr;           // also T0 and Y
     void *Pg = nullptr;
     size_t scratch = 0;
+    // per-DOF bytes of each scratch region (for carving the static array of
+    // fq_eval_noalloc) and that array's device address on this handle's device
+    size_t per_dof_F = 0, per_dof_Z = 0, per_dof_P = 0, per_dof_F32 = 0, per_dof_NY = 0, per_dof_FJ = 0;
+    unsigned char *noalloc = nullptr;
     // optional per-kernel HIP-event timing
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -1892,17 +1921,17 @@ int build_projection(fq_handle *h) {
     return rc;
 }
 
-template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, const Dof &dof, int D,
-                                           hipStream_t st) {
+template <int N, typename T> int run_chunk(fq_handle *h, const Scratch &sc, const AosoaPtrs &ptr, const Dof &dof,
+                                           int D, bool prof, hipStream_t st) {
     using G = Geo<N>;
     const fq::Plan &P = h->P;
     const int nsrc = P.nsrc, ntgt = P.ntgt, nitems = int(P.items.size()), ngroups = P.ngroups;
     constexpr int upb = kPencilThreads / G::LPP;
     constexpr int upp = kPlaneThreads / G::LPP;
-    cpx<double> *T0 = reinterpret_cast<cpx<double> *>(h->Z);
-    cpx<T> *Z = reinterpret_cast<cpx<T> *>(h->Z);
-    cpx<T> *Pg = reinterpret_cast<cpx<T> *>(h->Pg);
-    cpx<double> *Y = reinterpret_cast<cpx<double> *>(h->Z);
+    cpx<double> *T0 = reinterpret_cast<cpx<double> *>(sc.Z);
+    cpx<T> *Z = reinterpret_cast<cpx<T> *>(sc.Z);
+    cpx<T> *Pg = reinterpret_cast<cpx<T> *>(sc.Pg);
+    cpx<double> *Y = reinterpret_cast<cpx<double> *>(sc.Z);
     const cpx<T> *ph = reinterpret_cast<const cpx<T> *>(h->ph);
     // HIP-event brackets around each launch when profiling is on
     auto evget = [&]() {
@@ -1913,10 +1942,10 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     };
     std::pair<hipEvent_t, hipEvent_t> cur{};
     auto pre = [&]() {
-        if (h->prof) { cur = {evget(), evget()}; (void)hipEventRecord(cur.first, st); }
+        if (prof) { cur = {evget(), evget()}; (void)hipEventRecord(cur.first, st); }
     };
     auto post = [&](int k) {
-        if (h->prof) { (void)hipEventRecord(cur.second, st); h->ev_rec.push_back({k, cur}); }
+        if (prof) { (void)hipEventRecord(cur.second, st); h->ev_rec.push_back({k, cur}); }
     };
     auto blocks = [&](int64_t units) { return unsigned((units + upb - 1) / upb); };
     auto pblocks = [&](int64_t units) { return unsigned((units + upp - 1) / upp); };
@@ -1930,13 +1959,13 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     // the fused FP32 path reads an FP32 copy of F, written by k_fwd_xy itself
     const bool want32 = h->fused && std::is_same<T, float>::value;
     pre();
-    cpx<double> *FJ64 = h->pack && sizeof(T) == 8 ? reinterpret_cast<cpx<double> *>(h->FJ) : nullptr;
-    cpx<float> *FJ32 = h->pack && sizeof(T) == 4 ? reinterpret_cast<cpx<float> *>(h->FJ) : nullptr;
-    hipLaunchKernelGGL(k_fwd_xy<N>, dim3(pblocks(u)), pt, 0, st, T0, h->F, want32 ? h->F32 : nullptr, FJ64, FJ32, u);
+    cpx<double> *FJ64 = h->pack && sizeof(T) == 8 ? reinterpret_cast<cpx<double> *>(sc.FJ) : nullptr;
+    cpx<float> *FJ32 = h->pack && sizeof(T) == 4 ? reinterpret_cast<cpx<float> *>(sc.FJ) : nullptr;
+    hipLaunchKernelGGL(k_fwd_xy<N>, dim3(pblocks(u)), pt, 0, st, T0, sc.F, want32 ? sc.F32 : nullptr, FJ64, FJ32, u);
     post(1);
     if (h->fused) {
-        const cpx<T> *FT = reinterpret_cast<const cpx<T> *>(want32 ? static_cast<void *>(h->F32)
-                                                                   : static_cast<void *>(h->F));
+        const cpx<T> *FT = reinterpret_cast<const cpx<T> *>(want32 ? static_cast<void *>(sc.F32)
+                                                                   : static_cast<void *>(sc.F));
         const cpx<T> *zc = reinterpret_cast<const cpx<T> *>(h->zc);
         const dim3 grid(unsigned(int64_t(D) * ngroups * 4));
         bool multi = false;
@@ -1948,13 +1977,13 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
             if (one_lane) {
                 const dim3 blk1(Prod1Geo<N>::THREADS);
                 const int nprods = int(P.prods.size());
-                cpx<T> *NY = reinterpret_cast<cpx<T> *>(h->NY);
+                cpx<T> *NY = reinterpret_cast<cpx<T> *>(sc.NY);
                 const bool cj = h->pack_conj;
                 if (h->pack) {
                     auto knyq = cj ? k_nyq<N, T, true> : k_nyq<N, T, false>;
                     pre();
                     hipLaunchKernelGGL(knyq, dim3(unsigned(int64_t(D) * nprods)), dim3(96), 0, st, FT,
-                                       reinterpret_cast<const cpx<T> *>(h->FJ), NY,
+                                       reinterpret_cast<const cpx<T> *>(sc.FJ), NY,
                                        reinterpret_cast<const cpx<T> *>(h->nyv), ph, h->prods, h->item_src, nprods,
                                        nsrc);
                     post(8);
@@ -1977,7 +2006,7 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
             // keeps the FP64 kernel within 256 VGPRs without spills
             const bool m2 = multi || sizeof(T) == 8;
             const int nprods = int(P.prods.size());
-            cpx<T> *NY = reinterpret_cast<cpx<T> *>(h->NY);
+            cpx<T> *NY = reinterpret_cast<cpx<T> *>(sc.NY);
             const cpx<T> *zq = reinterpret_cast<const cpx<T> *>(h->zq);
             using KP = decltype(&k_prod<N, T, true, false>);
             KP kern = m2 ? k_prod<N, T, true, false> : k_prod<N, T, false, false>;
@@ -1985,7 +2014,7 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
                 if (h->pack) {
                     pre();
                     hipLaunchKernelGGL((k_nyq<N, T, true>), dim3(unsigned(int64_t(D) * nprods)), dim3(96), 0, st, FT,
-                                       reinterpret_cast<const cpx<T> *>(h->FJ), NY, nullptr, ph, h->prods,
+                                       reinterpret_cast<const cpx<T> *>(sc.FJ), NY, nullptr, ph, h->prods,
                                        h->item_src, nprods, nsrc);
                     post(8);
                     // packed: one item stage per product, so the single-product
@@ -2001,7 +2030,7 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     } else {
     u = int64_t(D) * nitems * N;
     pre();
-    hipLaunchKernelGGL((k_inv_z<N, T>), dim3(blocks(u)), bt, 0, st, h->F, Z, ph, h->item_src, nitems, nsrc, u);
+    hipLaunchKernelGGL((k_inv_z<N, T>), dim3(blocks(u)), bt, 0, st, sc.F, Z, ph, h->item_src, nitems, nsrc, u);
     post(2);
     u = int64_t(D) * ngroups * N;
     bool multi = false;
@@ -2019,10 +2048,10 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     pre();
     if constexpr (N == 32)
         hipLaunchKernelGGL((k_accum_lds<N, T>), dim3(unsigned(int64_t((D + 3) / 4) * ntgt * N)), dim3(256), 0, st,
-                           Pg, h->F, h->W, h->c_group, h->t_c0, h->t_cn, h->l_src, h->t_l0, h->t_ln,
+                           Pg, sc.F, h->W, h->c_group, h->t_c0, h->t_cn, h->l_src, h->t_l0, h->t_ln,
                            int(P.contribs.size()), Y, ngroups, nsrc, ntgt, D);
     else
-        hipLaunchKernelGGL((k_accum<N, T>), dim3(blocks(u)), bt, 0, st, Pg, h->F, h->W, h->c_group, h->t_c0,
+        hipLaunchKernelGGL((k_accum<N, T>), dim3(blocks(u)), bt, 0, st, Pg, sc.F, h->W, h->c_group, h->t_c0,
                            h->t_cn, h->l_src, h->t_l0, h->t_ln, int(P.contribs.size()), Y, ngroups, nsrc, ntgt, D, u);
     post(4);
     pre();
@@ -2039,13 +2068,18 @@ template <int N, typename T> int run_chunk(fq_handle *h, const AosoaPtrs &ptr, c
     return FQ_SUCCESS;
 }
 
-template <int N> int dispatch_prec(fq_handle *h, const AosoaPtrs &ptr, const Dof &dof, int D, hipStream_t st) {
-    return h->precision == FQ_PREC_MIXED ? run_chunk<N, float>(h, ptr, dof, D, st)
-                                         : run_chunk<N, double>(h, ptr, dof, D, st);
+template <int N>
+int dispatch_prec(fq_handle *h, const Scratch &sc, const AosoaPtrs &ptr, const Dof &dof, int D, bool prof,
+                  hipStream_t st) {
+    return h->precision == FQ_PREC_MIXED ? run_chunk<N, float>(h, sc, ptr, dof, D, prof, st)
+                                         : run_chunk<N, double>(h, sc, ptr, dof, D, prof, st);
 }
 
-int eval_impl(fq_handle *h, const AosoaPtrs &ptr, int32_t nupts, int64_t ldim, int32_t neles,
-              int64_t first_dof, int64_t ndof, void *stream) {
+// Batches of at most `chunk` DOFs on the scratch `sc`; prof = per-kernel
+// HIP-event brackets (which create events, so the no-alloc entry point
+// passes false).
+int eval_impl(fq_handle *h, const Scratch &sc, int64_t chunk, bool prof, const AosoaPtrs &ptr, int32_t nupts,
+              int64_t ldim, int32_t neles, int64_t first_dof, int64_t ndof, void *stream) {
     if (!h) return fail(FQ_ERR_INVALID, "null handle");
     if (nupts <= 0 || neles <= 0 || ndof < 0 || first_dof < 0)
         return fail(FQ_ERR_INVALID, "invalid DOF range or shape");
@@ -2058,23 +2092,81 @@ int eval_impl(fq_handle *h, const AosoaPtrs &ptr, int32_t nupts, int64_t ldim, i
     HIPCHK(hipGetDevice(&dev));
     if (dev != h->device) return fail(FQ_ERR_INVALID, "handle was created on another device");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    // equal batches of at most h->chunk DOFs (no short tail batch)
-    const int64_t nbatch = (ndof + h->chunk - 1) / h->chunk;
+    if (chunk <= 0) return fail(FQ_ERR_INVALID, "no scratch for this call");
+    // equal batches of at most chunk DOFs (no short tail batch)
+    const int64_t nbatch = (ndof + chunk - 1) / chunk;
     const int64_t per = (ndof + nbatch - 1) / nbatch;
     for (int64_t c0 = 0; c0 < ndof; c0 += per) {
         const int D = int(std::min<int64_t>(per, ndof - c0));
         Dof dof{first_dof + c0, nupts, 32, ldim, n};
         int rc;
         switch (h->P.N) {
-        case 8: rc = dispatch_prec<8>(h, ptr, dof, D, st); break;
-        case 16: rc = dispatch_prec<16>(h, ptr, dof, D, st); break;
-        case 24: rc = dispatch_prec<24>(h, ptr, dof, D, st); break;
-        case 32: rc = dispatch_prec<32>(h, ptr, dof, D, st); break;
+        case 8: rc = dispatch_prec<8>(h, sc, ptr, dof, D, prof, st); break;
+        case 16: rc = dispatch_prec<16>(h, sc, ptr, dof, D, prof, st); break;
+        case 24: rc = dispatch_prec<24>(h, sc, ptr, dof, D, prof, st); break;
+        case 32: rc = dispatch_prec<32>(h, sc, ptr, dof, D, prof, st); break;
         default: rc = fail(FQ_ERR_UNSUPPORTED, "grid size not built");
         }
         if (rc != FQ_SUCCESS) return rc;
     }
     return FQ_SUCCESS;
+}
+
+// The evaluation entry points on the handle's own scratch.
+int eval_own(fq_handle *h, const AosoaPtrs &ptr, int32_t nupts, int64_t ldim, int32_t neles, int64_t first_dof,
+             int64_t ndof, void *stream) {
+    if (h->flags & FQ_FLAG_NO_SCRATCH)
+        return fail(FQ_ERR_INVALID, "handle created with FQ_FLAG_NO_SCRATCH has no scratch; use fq_eval_noalloc");
+    Scratch sc;
+    sc.F = h->F; sc.Z = h->Z; sc.Pg = h->Pg; sc.F32 = h->F32; sc.NY = h->NY; sc.FJ = h->FJ;
+    return eval_impl(h, sc, h->chunk, h->prof, ptr, nupts, ldim, neles, first_dof, ndof, stream);
+}
+
+std::mutex g_noalloc_mu;
+std::map<int, unsigned char *> g_noalloc_ptr;     // device -> address of `scratch`
+
+// Address of the static scratch on the current device `dev` (nullptr on failure).
+unsigned char *noalloc_resolve(int dev) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    auto it = g_noalloc_ptr.find(dev);
+    if (it != g_noalloc_ptr.end()) return it->second;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    g_noalloc_ptr[dev] = static_cast<unsigned char *>(p);
+    return static_cast<unsigned char *>(p);
+}
+
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Bytes of the static scratch that a batch of c DOFs occupies.
+size_t noalloc_bytes(const fq_handle *h, int64_t c) {
+    const size_t cc = size_t(c);
+    return align256(h->per_dof_F * cc) + align256(h->per_dof_Z * cc) + align256(h->per_dof_P * cc) +
+           align256(h->per_dof_F32 * cc) + align256(h->per_dof_NY * cc) + align256(h->per_dof_FJ * cc);
+}
+
+// Carves the regions of a batch of c DOFs out of the static scratch at base.
+Scratch noalloc_carve(const fq_handle *h, unsigned char *base, int64_t c) {
+    const size_t cc = size_t(c);
+    Scratch sc;
+    size_t o = 0;
+    auto take = [&](size_t per) -> void * {
+        if (per == 0) return nullptr;
+        void *p = base + o;
+        o += align256(per * cc);
+        return p;
+    };
+    sc.F = static_cast<cpx<double> *>(take(h->per_dof_F));
+    sc.Z = take(h->per_dof_Z);
+    sc.Pg = take(h->per_dof_P);
+    sc.F32 = static_cast<cpx<float> *>(take(h->per_dof_F32));
+    sc.NY = take(h->per_dof_NY);
+    sc.FJ = take(h->per_dof_FJ);
+    return sc;
 }
 
 }  // namespace
@@ -2196,9 +2288,17 @@ int fq_create(const fq_params *prm, fq_handle **out) {
                                        size_t(2 * P.ntgt) * n * sizeof(cpx<double>)});
     const size_t per_dof_P = size_t(P.ngroups) * n * tsz;
     const size_t per_dof_F32 = h->precision == FQ_PREC_MIXED ? size_t(P.nsrc) * n * sizeof(cpx<float>) : 0;
-    // Nyquist-plane corrections of the pair-packed product kernel
-    const size_t per_dof_NY = h->pack ? (P.prods.size() * 3 + P.nsrc) * size_t(N) * N * tsz : 0;
-    const size_t per_dof = per_dof_F + per_dof_Z + per_dof_P + per_dof_F32 + per_dof_NY;
+    // Nyquist-plane corrections of the pair-packed product kernel and the
+    // spectrum slices they are computed from
+    const size_t per_dof_NY = h->pack ? P.prods.size() * 3 * size_t(N) * N * tsz : 0;
+    const size_t per_dof_FJ = h->pack ? size_t(P.nsrc) * N * N * tsz : 0;
+    const size_t per_dof = per_dof_F + per_dof_Z + per_dof_P + per_dof_F32 + per_dof_NY + per_dof_FJ;
+    h->per_dof_F = per_dof_F; h->per_dof_Z = per_dof_Z; h->per_dof_P = per_dof_P;
+    h->per_dof_F32 = per_dof_F32; h->per_dof_NY = per_dof_NY; h->per_dof_FJ = per_dof_FJ;
+    // the static scratch of fq_eval_noalloc, resolved with this device current
+    h->noalloc = noalloc_resolve(h->device);
+    if (rc == FQ_SUCCESS && (h->flags & FQ_FLAG_NO_SCRATCH) && !h->noalloc)
+        rc = fail(FQ_ERR_HIP, "hipGetSymbolAddress of the static scratch failed");
     int64_t chunk = prm->max_chunk;
     if (chunk <= 0) {
         const size_t budget = size_t(32) << 30;   // 32 GiB default scratch budget (of 288 GB HBM)
@@ -2206,7 +2306,9 @@ int fq_create(const fq_params *prm, fq_handle **out) {
         chunk = std::min<int64_t>(chunk, 4096);
     }
     h->chunk = chunk;
-    if (rc == FQ_SUCCESS) {
+    if (rc == FQ_SUCCESS && (h->flags & FQ_FLAG_NO_SCRATCH)) {
+        h->chunk = 0;                      // evaluation only through fq_eval_noalloc
+    } else if (rc == FQ_SUCCESS) {
         hipError_t e1 = hipMalloc(&h->F, per_dof_F * chunk);
         hipError_t e2 = hipMalloc(&h->Z, per_dof_Z * chunk);
         hipError_t e3 = hipMalloc(&h->Pg, per_dof_P * chunk);
@@ -2215,8 +2317,8 @@ int fq_create(const fq_params *prm, fq_handle **out) {
             e4 = hipMalloc(&h->F32, size_t(P.nsrc) * n * sizeof(cpx<float>) * chunk);
         hipError_t e5 = hipSuccess;
         if (h->pack) {
-            e5 = hipMalloc(&h->NY, P.prods.size() * 3 * size_t(N) * N * tsz * chunk);
-            if (e5 == hipSuccess) e5 = hipMalloc(&h->FJ, size_t(P.nsrc) * N * N * tsz * chunk);
+            e5 = hipMalloc(&h->NY, per_dof_NY * chunk);
+            if (e5 == hipSuccess) e5 = hipMalloc(&h->FJ, per_dof_FJ * chunk);
         }
         if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess || e5 != hipSuccess)
             rc = fail(FQ_ERR_ALLOC, "hipMalloc scratch failed (reduce max_chunk)");
@@ -2305,7 +2407,30 @@ int fq_eval(fq_handle *h, const double *f, double *q, int32_t nupts, int64_t ldi
     AosoaPtrs ptr{};
     ptr.f[0] = f;
     ptr.q[0] = q;
-    return eval_impl(h, ptr, nupts, ldim, neles, first_dof, ndof, stream);
+    return eval_own(h, ptr, nupts, ldim, neles, first_dof, ndof, stream);
+}
+
+int fq_eval_noalloc(fq_handle *h, const double *f, double *q, int32_t nupts, int64_t ldim, int32_t neles,
+                    int64_t first_dof, int64_t ndof, void *stream) {
+    if (!h) return fail(FQ_ERR_INVALID, "null handle");
+    if (h->P.nsrc != 1 || h->P.ntgt != 1)
+        return fail(FQ_ERR_INVALID, "fq_eval_noalloc needs a single-species model");
+    if (!f || !q) return fail(FQ_ERR_INVALID, "null f or q");
+    if (!h->noalloc) return fail(FQ_ERR_HIP, "static scratch not resolved on this handle's device");
+    // the largest batch whose regions fit the static scratch
+    const size_t one = noalloc_bytes(h, 1);
+    if (one > size_t(FQ_NOALLOC_SCRATCH_BYTES))
+        return fail(FQ_ERR_UNSUPPORTED, "one DOF of this plan needs " + std::to_string(one) +
+                                            " bytes of scratch, more than the " +
+                                            std::to_string(int64_t(FQ_NOALLOC_SCRATCH_BYTES)) +
+                                            " bytes of fq_eval_noalloc's static scratch");
+    int64_t c = std::min<int64_t>(4096, int64_t(size_t(FQ_NOALLOC_SCRATCH_BYTES) / one));
+    while (c > 1 && noalloc_bytes(h, c) > size_t(FQ_NOALLOC_SCRATCH_BYTES)) --c;
+    AosoaPtrs ptr{};
+    ptr.f[0] = f;
+    ptr.q[0] = q;
+    return eval_impl(h, noalloc_carve(h, h->noalloc, c), c, false, ptr, nupts, ldim, neles, first_dof, ndof,
+                     stream);
 }
 
 int fq_eval_multi(fq_handle *h, const double *const *f, double *const *q, int32_t nupts, int64_t ldim,
@@ -2318,7 +2443,7 @@ int fq_eval_multi(fq_handle *h, const double *const *f, double *const *q, int32_
         ptr.f[s] = f[s];
         ptr.q[s] = q[s];
     }
-    return eval_impl(h, ptr, nupts, ldim, neles, first_dof, ndof, stream);
+    return eval_own(h, ptr, nupts, ldim, neles, first_dof, ndof, stream);
 }
 
 int fq_eval_bilinear(fq_handle *h, const double *f, const double *g, double *q, int32_t nupts, int64_t ldim,
@@ -2329,7 +2454,7 @@ int fq_eval_bilinear(fq_handle *h, const double *f, const double *g, double *q, 
     ptr.f[0] = f;
     ptr.f[1] = g;
     ptr.q[0] = q;
-    return eval_impl(h, ptr, nupts, ldim, neles, first_dof, ndof, stream);
+    return eval_own(h, ptr, nupts, ldim, neles, first_dof, ndof, stream);
 }
 
 }  // extern "C"

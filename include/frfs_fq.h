@@ -70,11 +70,17 @@ enum fq_flags {
     FQ_FLAG_TWO_LANE = 4,     /* FP32 on 24^3/32^3 grids: product kernel with two
                                  lanes per pencil instead of one (a schedule
                                  candidate of the autotuner) */
-    FQ_FLAG_NO_PACK = 8       /* FP32 one-lane product kernel: transform the two
+    FQ_FLAG_NO_PACK = 8,      /* FP32 one-lane product kernel: transform the two
                                  items of a product separately instead of as one
                                  packed complex volume with Nyquist-plane
                                  corrections (for comparison) */
+    FQ_FLAG_NO_SCRATCH = 16   /* fq_create allocates the tables only, no scratch:
+                                 the handle evaluates through fq_eval_noalloc */
 };
+
+/* Size of the static device scratch of fq_eval_noalloc (one copy per GPU the
+ * library runs on, held until the process ends). */
+#define FQ_NOALLOC_SCRATCH_BYTES (256ll << 20)
 
 /*
  * Model parameters.  The host builds the velocity mesh exactly as the
@@ -151,6 +157,24 @@ int fq_scratch_bytes(const fq_handle *h, int64_t *bytes);
 int fq_eval(fq_handle *h, const double *f, double *q,
             int32_t nupts, int64_t ldim, int32_t neles,
             int64_t first_dof, int64_t ndof, void *stream);
+
+/* fq_eval without any device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments and results
+ * as fq_eval (the same kernels, bit for bit), but the batch scratch lives in
+ * a static device array of FQ_NOALLOC_SCRATCH_BYTES per GPU instead of the
+ * handle's scratch: the DOF range is evaluated in batches of as many DOFs as
+ * fit there.  Returns FQ_ERR_UNSUPPORTED when one DOF of the plan needs
+ * more than FQ_NOALLOC_SCRATCH_BYTES (e.g. a 32^3 two-species FP64 plan is
+ * not served; 32^3 single-species mixed takes 9 DOFs per batch).  Works on
+ * any single-species handle, including one created with
+ * FQ_FLAG_NO_SCRATCH.  Every call on a GPU shares that GPU's array, which
+ * keeps the last call's contents (every kernel writes what it later reads,
+ * so nothing needs clearing): calls that overlap on one GPU, on any handle
+ * or stream, race. */
+int fq_eval_noalloc(fq_handle *h, const double *f, double *q,
+                    int32_t nupts, int64_t ldim, int32_t neles,
+                    int64_t first_dof, int64_t ndof, void *stream);
 
 /* Multi-species vss-gll: f[s], q[s] for s < nspcs, all with the same AoSoA
  * geometry (frfs/solvers/dgfsbi/scattering.py:259-330;
