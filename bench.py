@@ -9,7 +9,8 @@ accumulation and FP64 conservation projection) vs plain FP64, at the workload
 A step is one evaluation of Q for all DOFs (inputs resident in HBM).  Prints
 one JSON line.  ``--gpus N`` with N > 1 exits: this build runs on one GPU.
 Other workloads (``--workload``): vhs16 (plain FP64, 16^3, 64 DOFs), vss32x2
-(two species, four pairs), rhs24 (SSP-RK2 step with FR advection) and fno16
+(two species, four pairs), rhs24 (SSP-RK2 step with FR advection) and fno16 /
+fno32
 (the learned stand-in against the spectral operator on 16^3, 8192 DOFs).
 """
 import argparse
@@ -41,10 +42,11 @@ K_PROD_TRAFFIC = {
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--workload', default='vss32', choices=['vss32', 'vhs16', 'vss32x2', 'rhs24', 'fno16'],
+    ap.add_argument('--workload', default='vss32',
+                    choices=['vss32', 'vhs16', 'vss32x2', 'rhs24', 'fno16', 'fno32'],
                     help='vss32: the Q(f,f) headline; vhs16: plain FP64 16^3, 64 DOFs; '
                          'vss32x2: two species, four pairs; rhs24: full SSP-RK2 step (FR advection + Q); '
-                         'fno16: the learned stand-in on the 16^3 grid, 8192 DOFs')
+                         'fno16 / fno32: the learned stand-in on the 16^3 / 32^3 grid, 8192 DOFs')
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--ndof', type=int, default=4096)
@@ -62,15 +64,16 @@ def parse():
     ap.add_argument('--modes', default='fp64,mixed,mixedproj')
     ap.add_argument('--sm-opt', action='append', default=[], metavar='KEY=VALUE',
                     help='extra [scattering-model] key, e.g. pair-pack=0 (repeatable)')
-    ap.add_argument('--fno-weights', default='', help='fno16: trained weights (.npz); default: train first')
-    ap.add_argument('--fno-save', default='', help='fno16: write the trained weights here')
-    ap.add_argument('--fno-pairs', type=int, default=200000)
+    ap.add_argument('--fno-weights', default='', help='fno16/32: trained weights (.npz); default: train first')
+    ap.add_argument('--fno-save', default='', help='fno16/32: write the trained weights here')
+    ap.add_argument('--fno-pairs', type=int, default=0, help='training pairs; 0 = 200000 (fno16), 50000 (fno32)')
     ap.add_argument('--fno-epochs', type=int, default=20)
     ap.add_argument('--fno-train-seconds', type=float, default=150.0)
     ap.add_argument('--fno-tol', type=float, default=1e-4)
     ap.add_argument('--fno-train-acts', default='fp32', choices=['fp32', 'fp16', 'bf16'],
-                    help='fno16: activation storage while training')
+                    help='fno16/32: activation storage while training')
     ap.add_argument('--fno-dofs', type=int, default=8192)
+    ap.add_argument('--fno-batch', type=int, default=64, help='training batch (DOFs)')
     ap.add_argument('--calib-copy-mib', type=int, default=0,
                     help='run one device-to-device copy of this size first (counter calibration)')
     return ap.parse_args()
@@ -314,44 +317,50 @@ def bench_vhs16(a):
                          'sample': f'all {ndof} DOFs, NumPy complex128 oracle, one process'}}))
 
 
-# Network forward per DOF (16^3 grid, width 32, 8^3 modes, 4 layers): per layer
-# the truncated forward transform (0.33 MFLOP per channel), the per-mode
-# channel mixing (512 x 32 x 32 complex multiply-adds, 4.19 MFLOP), the
-# inverse transform (0.33 MFLOP per channel) and the pointwise 32 x 32 layer
-# (8.39 MFLOP); lift and projection 0.26 MFLOP each.
-FNO_FLOP_PER_DOF = 4 * (32 * 327680 + 4194304 + 32 * 327680 + 8388608) + 2 * 262144
+def fno_flop_per_dof(N):
+    """Network forward per DOF (N^3 grid, width 32, 8^3 modes, 4 layers): per
+    layer the truncated forward transform (32 N^3 + 512 N^2 + 4096 N flop per
+    channel: 8 modes of N-point lines along z, y, x; 0.33 MFLOP at 16^3, 1.70
+    MFLOP at 32^3), the per-mode channel mixing (512 x 32 x 32 complex
+    multiply-adds, 4.19 MFLOP), the inverse transform (as the forward) and the
+    pointwise 32 x 32 layer (2048 N^3); lift and projection 64 N^3 each."""
+    dft = 32 * N ** 3 + 512 * N ** 2 + 4096 * N
+    return 4 * (32 * dft + 4194304 + 32 * dft + 2048 * N ** 3) + 2 * 64 * N ** 3
 
 
-def bench_fno16(a):
-    """The learned stand-in (vhs-gll-fno) on the 16^3 grid: 4 Fourier layers
-    of width 32 with 8^3 modes, trained on the GPU (unless --fno-weights) on
-    --fno-pairs pairs of the FP64 vhs-gll operator (omega = 0.5, M = 6,
-    Nrho = 8), then timed on 8192 held-out DOFs of seeded sheared mixtures:
-    the whole plugin call (gather, network, conservation check and
-    projection, spectral fallback on the flagged DOFs) against the spectral
-    operator on the same DOFs in FP64 and mixed precision."""
+def bench_fno(a):
+    """The learned stand-in, 4 Fourier layers of width 32 with 8^3 modes,
+    trained on the GPU (unless --fno-weights) on --fno-pairs pairs of the
+    conservation-projected FP64 spectral operator, then timed on 8192
+    held-out DOFs of seeded sheared mixtures: the whole plugin call (gather,
+    network, conservation check and projection, spectral fallback on the
+    flagged DOFs) against the spectral operator on the same DOFs in plain
+    FP64 and in mixed precision.
+      fno16: vhs-gll-fno on the 16^3 grid (vhs-gll omega = 0.5, M = 6, Nrho = 8);
+      fno32: vss-gll-fno on the 32^3 grid (vss-gll omega = 0.74, alpha = 1.4,
+             M = 12, Nrho = 16, Mpre = 32, [-6, 6]^3)."""
     import torch
     from frfs_amd import train_surrogate as ts
     from frfs_amd.backend import HIPBackend
     from frfs_amd.inputs import from_aosoa, shear_mixture_device, shear_mixture_params, to_aosoa
-    from frfs_amd.scattering import DGFSVHSGLLScatteringModel
-    from frfs_amd.surrogate import DGFSVHSGLLFNOScatteringModel
-    from frfs_amd.velocitymesh import DGFSVelocityMesh
     be = HIPBackend()
+    grid = 'vss32' if a.workload == 'fno32' else 'vhs16'
     ndof = a.fno_dofs
     if a.fno_weights:
-        cfg = ts.config(tol=a.fno_tol, weights=a.fno_weights)
-        vm = DGFSVelocityMesh(be, cfg)
-        sm = DGFSVHSGLLFNOScatteringModel(be, cfg, vm)
+        cfg = ts.config(tol=a.fno_tol, weights=a.fno_weights, grid=grid)
+        sm = ts.model(be, cfg)
+        vm = sm.vm
         rep = {'weights': os.path.basename(a.fno_weights)}
     else:
-        sm, rep = ts.train(be, pairs=a.fno_pairs, epochs=a.fno_epochs, time_limit=a.fno_train_seconds,
-                           seed=a.seed, train_acts=a.fno_train_acts,
+        pairs = a.fno_pairs or (50000 if grid == 'vss32' else 200000)
+        sm, rep = ts.train(be, pairs=pairs, epochs=a.fno_epochs, time_limit=a.fno_train_seconds,
+                           seed=a.seed, train_acts=a.fno_train_acts, grid=grid, batch=a.fno_batch,
                            log=lambda m: print(m, file=sys.stderr, flush=True))
         sm.tol = a.fno_tol
         vm = sm.vm
         if a.fno_save:
             sm.net.save(a.fno_save)
+    NV = vm.Nv()
     n = vm.vsize()
     f = shear_mixture_device(vm.cv(), shear_mixture_params(ndof, a.seed + 1), device=be.device)
     fin = be.matrix((1, n, ndof))
@@ -369,13 +378,28 @@ def bench_fno16(a):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / a.steps
 
-    dt_fp64 = timed(lambda: sm.fs_spectral(fin, qref))
-    cfgm = ts.config()
-    cfgm.set('scattering-model', 'type', 'vhs-gll')
-    cfgm.set('scattering-model', 'precision', 'mixed')
-    smm = DGFSVHSGLLScatteringModel(be, cfgm, DGFSVelocityMesh(be, cfgm))
+    # the teacher (FP64 with the projection) gives the reference Q; the plain
+    # FP64 operator (no projection) gives the speed reference
+    sm.fs_spectral(fin, qref)
+
+    def spectral(precision, project):
+        c = ts.config(grid=grid, project=project)
+        c.set('scattering-model', 'type', 'vss-gll' if grid == 'vss32' else 'vhs-gll')
+        c.set('scattering-model', 'precision', precision)
+        from frfs_amd.scattering import DGFSBiVSSGLLScatteringModel, DGFSVHSGLLScatteringModel
+        from frfs_amd.velocitymesh import DGFSBiVelocityMesh, DGFSVelocityMesh
+        if grid == 'vss32':
+            return DGFSBiVSSGLLScatteringModel(be, c, DGFSBiVelocityMesh(be, c, 1))
+        return DGFSVHSGLLScatteringModel(be, c, DGFSVelocityMesh(be, c))
+    smp = spectral('double', 0)
+    qplain = be.matrix(fin.ioshape)
+    dt_fp64 = timed(lambda: smp.fs_batch(fin, qplain))
+    del smp
+    smm = spectral('mixed', 1)
     qm = be.matrix(fin.ioshape)
     dt_mixed = timed(lambda: smm.fs_batch(fin, qm))
+    del smm
+    torch.cuda.empty_cache()
     stats = {}
 
     def run():
@@ -395,17 +419,17 @@ def bench_fno16(a):
     g = torch.empty((ndof, n), dtype=torch.float32, device=be.device)
     out = torch.empty_like(g)
     from frfs_amd import _lib
-    from frfs_amd.surrogate import F_REF, NV
+    from frfs_amd.surrogate import F_REF
     lib, bst = _lib.load(), be.stream_handle()
     rho = torch.empty(ndof, dtype=torch.float64, device=be.device)
     _lib.check(lib.fq_sur_gather(NV, vm.cw(), F_REF, 1, fin._as_parameter_, 1, fin.leaddim, ndof, 0, ndof, None,
                                  rho.data_ptr(), g.data_ptr(), bst))
     st = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    sm.net.forward(g, out)
+    sm.net.predict(g, out=out)
     e0.record(st)
     for _ in range(a.steps):
-        sm.net.forward(g, out)
+        sm.net.predict(g, out=out)
     e1.record(st)
     torch.cuda.synchronize()
     fwd_ms = e0.elapsed_time(e1) / a.steps
@@ -413,10 +437,10 @@ def bench_fno16(a):
     acts16 = {}
     for kind in ('fp16', 'bf16'):
         out16 = torch.empty_like(g)
-        sm.net.forward(g, out16, acts=kind)
+        sm.net.predict(g, out=out16, acts=kind)
         e0.record(st)
         for _ in range(a.steps):
-            sm.net.forward(g, out16, acts=kind)
+            sm.net.predict(g, out=out16, acts=kind)
         e1.record(st)
         torch.cuda.synchronize()
         ms16 = e0.elapsed_time(e1) / a.steps
@@ -457,19 +481,25 @@ def bench_fno16(a):
         th = sm.net.theta.double().cpu().numpy()
         gs = g[:4].double().cpu().numpy()
         t0 = time.perf_counter()
-        osur.forward(th, gs, 16, 32, 8, sm.net.layers)
+        osur.forward(th, gs, NV, 32, 8, sm.net.layers)
         cs = time.perf_counter() - t0
         cpu = {'value': 4 / cs, 'unit': 'DOF-evals/s', 'cores': 1, 'kind': 'port',
                'sample': '4 DOFs of the same workload, network forward only, NumPy float64 oracle, one process'}
-    tflops = FNO_FLOP_PER_DOF * ndof / (fwd_ms * 1e-3) / 1e12
+    flop = fno_flop_per_dof(NV)
+    tflops = flop * ndof / (fwd_ms * 1e-3) / 1e12
+    rplain = ((from_aosoa(qs) - from_aosoa(qplain)).pow(2).sum(1)
+              / from_aosoa(qplain).pow(2).sum(1)).sqrt().cpu().numpy()
+    teacher = ('vss-gll omega=0.74 alpha=1.4, 32^3 grid on [-6,6]^3, M=12, Nrho=16, Mpre=32' if grid == 'vss32'
+               else 'vhs-gll omega=0.5, 16^3 grid on [-5,5]^3, M=6, Nrho=8')
     print(json.dumps({
         'metric': 'learned stand-in Q(f,f) DOF-evals/s (with conservation check and spectral fallback), '
-                  'vhs-gll 16^3, 8192 DOFs', 'value': ndof / dt,
+                  f'{sm.scattering_model} {NV}^3, {ndof} DOFs', 'value': ndof / dt,
         'unit': 'DOF-evals/s', 'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': dt * 1e3,
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
-        'config': {'workload': 'Fourier-layer stand-in (4 layers, width 32, 8^3 modes) for vhs-gll omega=0.5, '
-                               '16^3 grid on [-5,5]^3, M=6, Nrho=8; 8192 DOFs of seeded mixtures of up to 3 '
-                               'sheared Maxwellians; fallback tolerance ' + repr(sm.tol)},
+        'config': {'workload': 'Fourier-layer stand-in (4 layers, width 32, 8^3 modes) for ' + teacher +
+                               f'; {ndof} DOFs of seeded mixtures of up to 3 sheared Maxwellians; trained on '
+                               'the conservation-projected FP64 operator; fallback tolerance ' + repr(sm.tol)},
+        'rel_l2_vs_plain_fp64': {'median': float(np.median(rplain)), 'p90': float(np.quantile(rplain, 0.9))},
         'spectral_fp64_dof_evals_per_s': ndof / dt_fp64, 'spectral_mixed_dof_evals_per_s': ndof / dt_mixed,
         'speedup_vs_fp64': dt_fp64 / dt, 'network_forward_ms': fwd_ms,
         'network_only_speedup_vs_fp64': dt_fp64 / (fwd_ms * 1e-3),
@@ -497,7 +527,7 @@ def bench_fno16(a):
         'training': rep,
         'roofline': {'bound': 'fp32-valu', 'achieved': tflops, 'peak': PEAK['fp32'], 'unit': 'TFLOP/s',
                      'frac': tflops / PEAK['fp32'], 'traffic': None,
-                     'kernel': 'network forward (all its kernels), ' + str(FNO_FLOP_PER_DOF) + ' flop per DOF'},
+                     'kernel': 'network forward (all its kernels), ' + str(flop) + ' flop per DOF'},
         'cpu_baseline': cpu}))
 
 
@@ -577,8 +607,8 @@ def main():
     if a.workload == 'vss32x2':
         bench_multi(a)
         return
-    if a.workload == 'fno16':
-        bench_fno16(a)
+    if a.workload in ('fno16', 'fno32'):
+        bench_fno(a)
         return
     from frfs_amd.inputs import mixture_params
     fpar = mixture_params(a.ndof, a.seed)

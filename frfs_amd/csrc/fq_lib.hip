@@ -773,25 +773,20 @@ __global__ __launch_bounds__(ProdGeo<N>::THREADS, (sizeof(T) == 4 ? 4 : 2)) void
             static_assert(SN <= PG::THREADS, "one J and one I element per thread");
             const cpx<T> *ny = NY + (int64_t(slot) * nprods + pi) * 3 * G3::N2;
             const int tid = int(threadIdx.x);
-            cpx<T> rk[KPT], rj = {0, 0}, ri = {0, 0};
+            cpx<T> *nyk = lds + PG::S * PG::PLANE, *nyj = nyk + N * N, *nyi = nyj + SN;
+            // the corrections go to their own LDS area before the x,y
+            // transforms: held in registers across them they would spill
 #pragma unroll
             for (int r = 0; r < KPT; ++r) {
                 const int idx = tid + r * PG::THREADS;
-                rk[r] = (N * N % PG::THREADS == 0 || idx < N * N) ? ny[idx] : cpx<T>{0, 0};
+                if (N * N % PG::THREADS == 0 || idx < N * N) nyk[idx] = ny[idx];
             }
             if (tid < SN) {
                 const int kk = tid / N, w = tid % N, z = G + 4 * kk;
-                rj = ny[G3::N2 + z * N + w];
-                ri = ny[2 * G3::N2 + w * N + z];
+                nyj[tid] = ny[G3::N2 + z * N + w];
+                nyi[tid] = ny[2 * G3::N2 + w * N + z];
             }
             plane_inv_lds<N, T>(L1, ev, c, half, act);        // lane (c, half): vx = c, vy = half + 2e
-            cpx<T> *nyk = lds + PG::S * PG::PLANE, *nyj = nyk + N * N, *nyi = nyj + SN;
-#pragma unroll
-            for (int r = 0; r < KPT; ++r) {
-                const int idx = tid + r * PG::THREADS;
-                if (N * N % PG::THREADS == 0 || idx < N * N) nyk[idx] = rk[r];
-            }
-            if (tid < SN) { nyj[tid] = rj; nyi[tid] = ri; }
             __syncthreads();
             const T sz = (vz & 1) ? T(-1) : T(1);
             const T sc = (c & 1) ? T(-1) : T(1);
@@ -1287,17 +1282,18 @@ __global__ __launch_bounds__(256, accum_waves(N, sizeof(T))) void k_accum(const 
 
 // ---------------------------------------------------------------------------
 // k_accum_lds: the same accumulation on the 32^3 grid (a pencil row fills
-// one wave).  One workgroup = four consecutive DOF slots of one
-// (target, row I'), one wave each.  The weight row block W[c][.][I'][.]
+// one wave).  One workgroup = NS consecutive DOF slots of one
+// (target, row I'), one wave each (NS = 4, two workgroups per CU, or NS = 8,
+// one workgroup per CU and half the weight reads).  The weight row block W[c][.][I'][.]
 // (N x N FP64 values) is the same for the four slots, so the workgroup loads
 // it once per contribution into a double-buffered LDS area, one contribution
 // ahead of its use, and the waves read it from LDS: the weight latency is
-// hidden behind a whole contribution and weight reads from L2 drop fourfold.
+// hidden behind a whole contribution and weight reads from L2 drop NS-fold.
 // Waves whose slot lies past the batch take part in the copies and barriers
 // only.
 // ---------------------------------------------------------------------------
-template <int N, typename T>
-__global__ __launch_bounds__(256, 2) void k_accum_lds(
+template <int N, typename T, int NS = 4>
+__global__ __launch_bounds__(64 * NS, 8 / NS) void k_accum_lds(
     const cpx<T> *__restrict__ P, const cpx<double> *__restrict__ F, const cpx<double> *__restrict__ W,
     const int *__restrict__ c_group, const int *__restrict__ t_c0, const int *__restrict__ t_cn,
     const int *__restrict__ l_src, const int *__restrict__ t_l0, const int *__restrict__ t_ln, int ncontribs,
@@ -1306,15 +1302,16 @@ __global__ __launch_bounds__(256, 2) void k_accum_lds(
     static_assert(G::LPP == 64, "one pencil row per wave");
     constexpr int E = G::E;
     constexpr int WR = N * N;                     // weight row block (K', J)
-    constexpr int CP = (WR + 255) / 256;          // 16-byte copies per thread
+    constexpr int TH = 64 * NS;                   // NS waves, one DOF slot each
+    constexpr int CP = (WR + TH - 1) / TH;        // 16-byte copies per thread
     __shared__ cpx<double> wl[2][WR];
-    const int nsg = (D + 3) / 4;
+    const int nsg = (D + NS - 1) / NS;
     const int b = blockIdx.x;
     const int sg = b % nsg;
     const int t = (b / nsg) % ntgt;
     const int I = b / (nsg * ntgt);
     const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / 64);
-    const int slot = sg * 4 + wave;
+    const int slot = sg * NS + wave;
     const bool live = slot < D;                   // wave-uniform
     const int lt = threadIdx.x % 64;
     const int J = lt >> 1, half = lt & 1;
@@ -1375,8 +1372,8 @@ __global__ __launch_bounds__(256, 2) void k_accum_lds(
     if (cn > 0) {
         fq::sfor<0, CP>([&](auto kk) {
             constexpr int k = decltype(kk)::value;
-            const int idx = int(threadIdx.x) + 256 * k;
-            if (WR % 256 == 0 || idx < WR) {
+            const int idx = int(threadIdx.x) + TH * k;
+            if (WR % TH == 0 || idx < WR) {
                 const cpx<double> *q = wrow + int64_t(c0) * G::VOL + int64_t(idx / N) * G::N2 + idx % N;
                 wp[k] = {q->x, q->y};           // field copies: no aggregate memcpy
             }
@@ -1388,8 +1385,8 @@ __global__ __launch_bounds__(256, 2) void k_accum_lds(
         // reads of buf ended before the previous barrier
         fq::sfor<0, CP>([&](auto kk) {
             constexpr int k = decltype(kk)::value;
-            const int idx = int(threadIdx.x) + 256 * k;
-            if (WR % 256 == 0 || idx < WR) {
+            const int idx = int(threadIdx.x) + TH * k;
+            if (WR % TH == 0 || idx < WR) {
                 wl[buf][idx].x = wp[k].x;
                 wl[buf][idx].y = wp[k].y;
             }
@@ -1399,8 +1396,8 @@ __global__ __launch_bounds__(256, 2) void k_accum_lds(
         const int cn1 = ci + 1 < c0 + cn ? ci + 1 : ci;
         fq::sfor<0, CP>([&](auto kk) {
             constexpr int k = decltype(kk)::value;
-            const int idx = int(threadIdx.x) + 256 * k;
-            if (WR % 256 == 0 || idx < WR) {
+            const int idx = int(threadIdx.x) + TH * k;
+            if (WR % TH == 0 || idx < WR) {
                 const cpx<double> *q = wrow + int64_t(cn1) * G::VOL + int64_t(idx / N) * G::N2 + idx % N;
                 wp[k] = {q->x, q->y};
             }
@@ -1669,6 +1666,7 @@ struct fq_handle {
     // fq_eval_noalloc) and that array's device address on this handle's device
     size_t per_dof_F = 0, per_dof_Z = 0, per_dof_P = 0, per_dof_F32 = 0, per_dof_NY = 0, per_dof_FJ = 0;
     unsigned char *noalloc = nullptr;
+    int cus = 256;               // compute units of the device
     // optional per-kernel HIP-event timing
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -2046,18 +2044,24 @@ template <int N, typename T> int run_chunk(fq_handle *h, const Scratch &sc, cons
     }
     u = int64_t(D) * ntgt * N;
     pre();
-    if constexpr (N == 32)
-        hipLaunchKernelGGL((k_accum_lds<N, T>), dim3(unsigned(int64_t((D + 3) / 4) * ntgt * N)), dim3(256), 0, st,
+    if constexpr (N == 32) {
+        // FP32 pencils: eight DOF slots per workgroup (one per CU); FP64: four
+        // (three workgroups per CU, register-limited)
+        constexpr int NS = sizeof(T) == 4 ? 8 : 4;
+        hipLaunchKernelGGL((k_accum_lds<N, T, NS>), dim3(unsigned(int64_t((D + NS - 1) / NS) * ntgt * N)),
+                           dim3(64 * NS), 0, st,
                            Pg, sc.F, h->W, h->c_group, h->t_c0, h->t_cn, h->l_src, h->t_l0, h->t_ln,
                            int(P.contribs.size()), Y, ngroups, nsrc, ntgt, D);
-    else
+    } else {
         hipLaunchKernelGGL((k_accum<N, T>), dim3(blocks(u)), bt, 0, st, Pg, sc.F, h->W, h->c_group, h->t_c0,
                            h->t_cn, h->l_src, h->t_l0, h->t_ln, int(P.contribs.size()), Y, ngroups, nsrc, ntgt, D, u);
+    }
     post(4);
     pre();
     hipLaunchKernelGGL(k_out<N>, dim3(pblocks(u)), pt, 0, st, Y, ptr, dof, ntgt, u);
     post(5);
     if (h->flags & FQ_FLAG_PROJECT) {
+        // per batch, right after k_out: the batch's Q is still in L2
         pre();
         hipLaunchKernelGGL(k_project, dim3(unsigned(8 * ((D + 7) / 8))), dim3(256), 0, st, ptr, dof, ntgt, N, P.L,
                            h->ginv, h->pmass, int(D));
@@ -2093,9 +2097,22 @@ int eval_impl(fq_handle *h, const Scratch &sc, int64_t chunk, bool prof, const A
     if (dev != h->device) return fail(FQ_ERR_INVALID, "handle was created on another device");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     if (chunk <= 0) return fail(FQ_ERR_INVALID, "no scratch for this call");
-    // equal batches of at most chunk DOFs (no short tail batch)
-    const int64_t nbatch = (ndof + chunk - 1) / chunk;
-    const int64_t per = (ndof + nbatch - 1) / nbatch;
+    // equal batches of at most chunk DOFs (no short tail batch) ...
+    int64_t nbatch = (ndof + chunk - 1) / chunk;
+    int64_t per = (ndof + nbatch - 1) / nbatch;
+    // ... except on the 32^3 grid, where a batch is a multiple of the DOFs
+    // whose k_accum_lds workgroups (NS DOFs x N rows each) fill whole rounds
+    // of the device's workgroup slots when the scratch holds at least that
+    // many: the last batch is then the short one
+    if (h->P.N == 32 && h->fused && nbatch > 1) {
+        // FP32: 8 slots per workgroup, one per CU; FP64: 4 slots, three per CU
+        const int64_t align = int64_t(h->cus) * (h->precision == FQ_PREC_MIXED ? 8 : 12) / 32;
+        if (align > 0 && chunk >= align) {
+            const int64_t pmax = chunk / align * align;
+            nbatch = (ndof + pmax - 1) / pmax;
+            per = ((ndof + nbatch - 1) / nbatch + align - 1) / align * align;
+        }
+    }
     for (int64_t c0 = 0; c0 < ndof; c0 += per) {
         const int D = int(std::min<int64_t>(per, ndof - c0));
         Dof dof{first_dof + c0, nupts, 32, ldim, n};
@@ -2251,6 +2268,9 @@ int fq_create(const fq_params *prm, fq_handle **out) {
         delete h;
         return fail(FQ_ERR_HIP, "no HIP device");
     }
+    if (hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess ||
+        h->cus <= 0)
+        h->cus = 256;
     const fq::Plan &P = h->P;
     const int N = P.N;
     const int64_t n = int64_t(N) * N * N;
@@ -2301,7 +2321,7 @@ int fq_create(const fq_params *prm, fq_handle **out) {
         rc = fail(FQ_ERR_HIP, "hipGetSymbolAddress of the static scratch failed");
     int64_t chunk = prm->max_chunk;
     if (chunk <= 0) {
-        const size_t budget = size_t(32) << 30;   // 32 GiB default scratch budget (of 288 GB HBM)
+        const size_t budget = size_t(8) << 30;    // 8 GiB default scratch budget (of 288 GB HBM)
         chunk = std::max<int64_t>(1, int64_t(budget / per_dof));
         chunk = std::min<int64_t>(chunk, 4096);
     }

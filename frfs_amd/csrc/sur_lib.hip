@@ -1,11 +1,12 @@
 // Fourier-layer surrogate of the collision operator: forward, backward,
 // loss, Adam and the AoSoA gather / conservation-check / scatter kernels.
 //
-// The network (16^3 grid, width 32, 8^3 retained modes, L layers):
+// The network (N^3 grid with N = 16 or 32, width 32, 8^3 retained modes,
+// L layers):
 //   v_0      = P g + p0                                 (lift, per point)
-//   X_l(k)   = (1/n) sum_x v_l(x) e^{-2 pi i k.x/16}   (k_sur_dft, per channel)
+//   X_l(k)   = (1/n) sum_x v_l(x) e^{-2 pi i k.x/N}    (k_sur_dft, per channel)
 //   Y_l(o,k) = sum_i X_l(i,k) R_l(i,o,k)               (k_sur_mix_lds, per mode)
-//   s_l(x)   = Re sum_k Y_l(k) e^{+2 pi i k.x/16}       (k_sur_idft, per channel)
+//   s_l(x)   = Re sum_k Y_l(k) e^{+2 pi i k.x/N}        (k_sur_idft, per channel)
 //   z_l      = Wm_l v_l + b_l + s_l,  v_{l+1} = gelu(z_l) (last layer: z_l)
 //   out      = Qw . v_L + q0                            (projection)
 // Backward passes reuse the same transforms: the adjoint of Re(IDFT) is the
@@ -13,7 +14,7 @@
 // Weight gradients are per-workgroup sums over 512 grid points, added with
 // vector-memory atomics into a caller-zeroed gradient vector.
 //
-// Layouts: activations are [batch][channel][x] FP32 with x = (I*16+J)*16+K;
+// Layouts: activations are [batch][channel][x] FP32 with x = (I*N+J)*N+K;
 // mode arrays are [batch][channel][k] complex with k = (px*8+py)*8+pz and
 // wavenumber index kidx(p) per axis; theta holds P, p0, per layer
 // (R (i,o,k) complex interleaved, Wm (o,i), b), Qw, q0.
@@ -29,29 +30,35 @@ int fq_internal_fail(int code, const std::string &msg);   // fq_lib.hip
 
 namespace {
 
-constexpr int SN = 16, SN2 = SN * SN, SN3 = SN * SN * SN;
 constexpr int SW = 32, SM = 8, SM3 = SM * SM * SM;
 constexpr int64_t SR = 2LL * SW * SW * SM3;          // floats of one layer's R
 constexpr int64_t SLAYER = SR + SW * SW + SW;        // floats of one layer
 
-__device__ __forceinline__ int kidx(int p) { return p < SM / 2 ? p : p + SN - SM; }
+// Grid constants of the N^3 grid (N = 16 or 32).
+template <int N> struct SG {
+    static_assert(N == 16 || N == 32, "grids of 16^3 and 32^3 points");
+    static constexpr int N2 = N * N, N3 = N * N * N;
+    static constexpr int LOG3 = N == 16 ? 12 : 15;  // log2(N^3)
+};
 
-// cos(2 pi j / 16); sin(2 pi j / 16) = cos(2 pi (j - 4) / 16).  Called with
-// compile-time j (unrolled loops) the values fold into the instructions.
-__device__ __forceinline__ float cos16(int j) {
-    switch (j & 15) {
-    case 0: return 1.0f;
-    case 1: case 15: return 0.92387953251128676f;
-    case 2: case 14: return 0.70710678118654752f;
-    case 3: case 13: return 0.38268343236508977f;
-    case 4: case 12: return 0.0f;
-    case 5: case 11: return -0.38268343236508977f;
-    case 6: case 10: return -0.70710678118654752f;
-    case 7: case 9: return -0.92387953251128676f;
-    default: return -1.0f;
-    }
-}
-__device__ __forceinline__ float sin16(int j) { return cos16(j - 4); }
+template <int N> __device__ __forceinline__ int kidx(int p) { return p < SM / 2 ? p : p + N - SM; }
+
+// cos(2 pi j / 32), j = 0..31.
+__device__ constexpr float kCos32[32] = {
+    1.0f, 0.9807852804032304f, 0.9238795325112867f, 0.8314696123025452f,
+    0.7071067811865476f, 0.5555702330196023f, 0.38268343236508984f, 0.19509032201612833f,
+    0.0f, -0.1950903220161282f, -0.3826834323650897f, -0.555570233019602f,
+    -0.7071067811865475f, -0.8314696123025453f, -0.9238795325112867f, -0.9807852804032304f,
+    -1.0f, -0.9807852804032304f, -0.9238795325112868f, -0.8314696123025455f,
+    -0.7071067811865477f, -0.5555702330196022f, -0.38268343236509034f, -0.19509032201612866f,
+    0.0f, 0.1950903220161283f, 0.38268343236509f, 0.5555702330196018f,
+    0.7071067811865474f, 0.8314696123025452f, 0.9238795325112865f, 0.9807852804032303f};
+// cos(2 pi j / N) and sin(2 pi j / N) = cos(2 pi (j - N/4) / N), N = 16 or
+// 32: the table entry (j mod N) (32 / N).  Called with compile-time j
+// (unrolled loops) the values fold into the instructions; otherwise they are
+// uniform loads of the table.
+template <int N> __device__ __forceinline__ float cosN(int j) { return kCos32[(j * (32 / N)) & 31]; }
+template <int N> __device__ __forceinline__ float sinN(int j) { return cosN<N>(j - N / 4); }
 
 __device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
 // Activation storage: FP32, or FP16 / BF16 for inference (arithmetic stays
@@ -78,35 +85,35 @@ __device__ __forceinline__ float gelu_grad(float z) {
 }
 
 // X[slice] = scale * sum_x v[slice](x) e^{-i k.x} over the retained modes.
-// One workgroup per (batch, channel) slice; separable z, y, x stages in LDS,
+// 16^3 grid: one workgroup per (batch, channel) slice; separable z, y, x stages in LDS,
 // each thread reading its inputs once and producing all the modes of its
 // line with compile-time twiddles.
 template <typename TA>
-__global__ __launch_bounds__(256) void k_sur_dft(const TA *__restrict__ v, float scale, float2 *__restrict__ X) {
-    __shared__ float vin[SN2 * 17];
-    __shared__ float2 A[SN2 * 9];            // [I][J][pz]
-    __shared__ float2 Bs[SN * SM * SM];      // [I][py][pz]
+__global__ __launch_bounds__(256) void k_sur_dft16(const TA *__restrict__ v, float scale, float2 *__restrict__ X) {
+    __shared__ float vin[SG<16>::N2 * 17];
+    __shared__ float2 A[SG<16>::N2 * 9];            // [I][J][pz]
+    __shared__ float2 Bs[16 * SM * SM];      // [I][py][pz]
     const int t = threadIdx.x;
     const int64_t slice = blockIdx.x;
-    const TA *src = v + slice * SN3;
+    const TA *src = v + slice * SG<16>::N3;
 #pragma unroll
-    for (int r = 0; r < SN3 / 256; ++r) {
+    for (int r = 0; r < SG<16>::N3 / 256; ++r) {
         const int e = t + 256 * r;
         vin[(e >> 4) * 17 + (e & 15)] = ldA(src, e);
     }
     __syncthreads();
     {
-        float a[SN];
+        float a[16];
 #pragma unroll
-        for (int K = 0; K < SN; ++K) a[K] = vin[t * 17 + K];
+        for (int K = 0; K < 16; ++K) a[K] = vin[t * 17 + K];
 #pragma unroll
         for (int p = 0; p < SM; ++p) {
-            const int k = kidx(p);
+            const int k = kidx<16>(p);
             float re = 0.f, im = 0.f;
 #pragma unroll
-            for (int K = 0; K < SN; ++K) {
-                re += a[K] * cos16(k * K);
-                im -= a[K] * sin16(k * K);
+            for (int K = 0; K < 16; ++K) {
+                re += a[K] * cosN<16>(k * K);
+                im -= a[K] * sinN<16>(k * K);
             }
             A[t * 9 + p] = make_float2(re, im);
         }
@@ -114,18 +121,18 @@ __global__ __launch_bounds__(256) void k_sur_dft(const TA *__restrict__ v, float
     __syncthreads();
     // y: thread t < 128 owns (I, pz) = (t / 8, t % 8): its 16 values once, the
     // 8 y modes with compile-time twiddles
-    if (t < SN * SM) {
+    if (t < 16 * SM) {
         const int I = t >> 3, pz = t & 7;
-        float2 a[SN];
+        float2 a[16];
 #pragma unroll
-        for (int J = 0; J < SN; ++J) a[J] = A[(I * SN + J) * 9 + pz];
+        for (int J = 0; J < 16; ++J) a[J] = A[(I * 16 + J) * 9 + pz];
 #pragma unroll
         for (int py = 0; py < SM; ++py) {
-            const int k = kidx(py);
+            const int k = kidx<16>(py);
             float re = 0.f, im = 0.f;
 #pragma unroll
-            for (int J = 0; J < SN; ++J) {
-                const float c = cos16(k * J), sn = sin16(k * J);
+            for (int J = 0; J < 16; ++J) {
+                const float c = cosN<16>(k * J), sn = sinN<16>(k * J);
                 re += a[J].x * c + a[J].y * sn;
                 im += a[J].y * c - a[J].x * sn;
             }
@@ -135,16 +142,16 @@ __global__ __launch_bounds__(256) void k_sur_dft(const TA *__restrict__ v, float
     __syncthreads();
     // x: thread t < 64 owns (py, pz) = (t / 8, t % 8): 16 values, the 8 x modes
     if (t < SM * SM) {
-        float2 a[SN];
+        float2 a[16];
 #pragma unroll
-        for (int I = 0; I < SN; ++I) a[I] = Bs[I * SM * SM + t];
+        for (int I = 0; I < 16; ++I) a[I] = Bs[I * SM * SM + t];
 #pragma unroll
         for (int px = 0; px < SM; ++px) {
-            const int k = kidx(px);
+            const int k = kidx<16>(px);
             float re = 0.f, im = 0.f;
 #pragma unroll
-            for (int I = 0; I < SN; ++I) {
-                const float c = cos16(k * I), sn = sin16(k * I);
+            for (int I = 0; I < 16; ++I) {
+                const float c = cosN<16>(k * I), sn = sinN<16>(k * I);
                 re += a[I].x * c + a[I].y * sn;
                 im += a[I].y * c - a[I].x * sn;
             }
@@ -153,13 +160,14 @@ __global__ __launch_bounds__(256) void k_sur_dft(const TA *__restrict__ v, float
     }
 }
 
-// out[slice](x) = (accumulate ? out : 0) + scale * Re sum_k Y[slice](k) e^{+i k.x}.
+// out[slice](x) = (accumulate ? out : 0) + scale * Re sum_k Y[slice](k) e^{+i k.x}
+// (16^3 grid).
 template <typename TA>
-__global__ __launch_bounds__(256) void k_sur_idft(const float2 *__restrict__ Y, float scale, int accumulate,
+__global__ __launch_bounds__(256) void k_sur_idft16(const float2 *__restrict__ Y, float scale, int accumulate,
                                                   TA *__restrict__ out) {
     __shared__ float2 Ys[SM3];               // [px][py][pz]
-    __shared__ float2 T1[SN * SM * SM];      // [I][py][pz]
-    __shared__ float2 T2[SN2 * 9];           // [I][J][pz]
+    __shared__ float2 T1[16 * SM * SM];      // [I][py][pz]
+    __shared__ float2 T2[SG<16>::N2 * 9];           // [I][J][pz]
     const int t = threadIdx.x;
     const int64_t slice = blockIdx.x;
     Ys[t] = Y[slice * SM3 + t];
@@ -172,11 +180,11 @@ __global__ __launch_bounds__(256) void k_sur_idft(const float2 *__restrict__ Y, 
 #pragma unroll
         for (int px = 0; px < SM; ++px) a[px] = Ys[px * SM * SM + t];
 #pragma unroll
-        for (int I = 0; I < SN; ++I) {
+        for (int I = 0; I < 16; ++I) {
             float re = 0.f, im = 0.f;
 #pragma unroll
             for (int px = 0; px < SM; ++px) {
-                const float c = cos16(kidx(px) * I), sn = sin16(kidx(px) * I);
+                const float c = cosN<16>(kidx<16>(px) * I), sn = sinN<16>(kidx<16>(px) * I);
                 re += a[px].x * c - a[px].y * sn;
                 im += a[px].x * sn + a[px].y * c;
             }
@@ -185,21 +193,21 @@ __global__ __launch_bounds__(256) void k_sur_idft(const float2 *__restrict__ Y, 
     }
     __syncthreads();
     // y: thread t < 128 owns (I, pz) = (t / 8, t % 8): 8 values, the 16 rows J
-    if (t < SN * SM) {
+    if (t < 16 * SM) {
         const int I = t >> 3, pz = t & 7;
         float2 a[SM];
 #pragma unroll
         for (int py = 0; py < SM; ++py) a[py] = T1[(I * SM + py) * SM + pz];
 #pragma unroll
-        for (int J = 0; J < SN; ++J) {
+        for (int J = 0; J < 16; ++J) {
             float re = 0.f, im = 0.f;
 #pragma unroll
             for (int py = 0; py < SM; ++py) {
-                const float c = cos16(kidx(py) * J), sn = sin16(kidx(py) * J);
+                const float c = cosN<16>(kidx<16>(py) * J), sn = sinN<16>(kidx<16>(py) * J);
                 re += a[py].x * c - a[py].y * sn;
                 im += a[py].x * sn + a[py].y * c;
             }
-            T2[(I * SN + J) * 9 + pz] = make_float2(re, im);
+            T2[(I * 16 + J) * 9 + pz] = make_float2(re, im);
         }
     }
     __syncthreads();
@@ -208,18 +216,18 @@ __global__ __launch_bounds__(256) void k_sur_idft(const float2 *__restrict__ Y, 
     float2 a[SM];
 #pragma unroll
     for (int pz = 0; pz < SM; ++pz) a[pz] = T2[t * 9 + pz];
-    float o[SN];
+    float o[16];
 #pragma unroll
-    for (int K = 0; K < SN; ++K) {
+    for (int K = 0; K < 16; ++K) {
         float sum = 0.f;
 #pragma unroll
-        for (int pz = 0; pz < SM; ++pz) sum += a[pz].x * cos16(kidx(pz) * K) - a[pz].y * sin16(kidx(pz) * K);
+        for (int pz = 0; pz < SM; ++pz) sum += a[pz].x * cosN<16>(kidx<16>(pz) * K) - a[pz].y * sinN<16>(kidx<16>(pz) * K);
         o[K] = scale * sum;
     }
     if constexpr (std::is_same<TA, float>::value) {
-        float4 *dst = reinterpret_cast<float4 *>(out + slice * SN3 + t * SN);
+        float4 *dst = reinterpret_cast<float4 *>(out + slice * SG<16>::N3 + t * 16);
 #pragma unroll
-        for (int q = 0; q < SN / 4; ++q) {
+        for (int q = 0; q < 16 / 4; ++q) {
             float4 w = make_float4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
             if (accumulate) {
                 const float4 p0 = dst[q];
@@ -232,15 +240,230 @@ __global__ __launch_bounds__(256) void k_sur_idft(const float2 *__restrict__ Y, 
         }
     } else {                                  // 16-bit inference output (never accumulated)
         static_assert(sizeof(TA) == 2, "16-bit storage");
-        Pack8<TA> *dst = reinterpret_cast<Pack8<TA> *>(out + slice * SN3 + t * SN);
+        Pack8<TA> *dst = reinterpret_cast<Pack8<TA> *>(out + slice * SG<16>::N3 + t * 16);
 #pragma unroll
-        for (int q = 0; q < SN / 8; ++q) {
+        for (int q = 0; q < 16 / 8; ++q) {
             Pack8<TA> w;
 #pragma unroll
             for (int j = 0; j < 8; ++j) w.v[j] = TA(o[8 * q + j]);
             dst[q] = w;
         }
     }
+}
+
+// The same transform on the 32^3 grid.  One workgroup of 512 threads per
+// slice.  The z stage runs in two passes of 512 grid rows: each pass loads
+// its rows with coalesced 16-byte loads into LDS (vin), then thread t turns
+// row t into its 8 modes (A, 76 KB).  The y and x stages then work from LDS
+// as on the 16^3 grid (Bs reuses vin).  143 KB of LDS: one workgroup per CU.
+constexpr int kDft32Threads = 512;
+// LDS index of the 8 modes of grid row r = I*32 + J in the 32^3 transforms:
+// rows padded to 9 and the 32 rows of one I to 296 float2 values, so that
+// both the per-row accesses (z stage) and the per-(I, pz) accesses (y stage)
+// spread over the LDS banks.
+constexpr int kRowsI = 296;
+__device__ __forceinline__ int aidx32(int I, int J, int pz) { return I * kRowsI + J * 9 + pz; }
+template <typename TA>
+__global__ __launch_bounds__(kDft32Threads) void k_sur_dft32(const TA *__restrict__ v, float scale,
+                                                           float2 *__restrict__ X) {
+    constexpr int N = 32, N3 = SG<32>::N3, T = kDft32Threads, RS = N + 1;
+    __shared__ float2 A[N * kRowsI];          // [I][J][pz], aidx32
+    __shared__ float vin[T * RS];             // one pass of rows; then Bs [I][py][pz]
+    float2 *Bs = reinterpret_cast<float2 *>(vin);
+    const int t = threadIdx.x;
+    const int64_t slice = blockIdx.x;
+    const TA *src = v + slice * N3;
+#pragma unroll 1
+    for (int q = 0; q < N * N / T; ++q) {
+        const TA *rows = src + int64_t(q) * T * N;
+        if constexpr (std::is_same<TA, float>::value) {
+#pragma unroll
+            for (int w = 0; w < T * N / 4 / T; ++w) {
+                const int e4 = t + T * w;                 // float4 index in the pass
+                const float4 x = reinterpret_cast<const float4 *>(rows)[e4];
+                float *d = vin + (e4 >> 3) * RS + (e4 & 7) * 4;
+                d[0] = x.x; d[1] = x.y; d[2] = x.z; d[3] = x.w;
+            }
+        } else {
+#pragma unroll
+            for (int w = 0; w < T * N / 8 / T; ++w) {
+                const int e8 = t + T * w;                 // 8-value pack index in the pass
+                const Pack8<TA> x = reinterpret_cast<const Pack8<TA> *>(rows)[e8];
+                float *d = vin + (e8 >> 2) * RS + (e8 & 3) * 8;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) d[j] = float(x.v[j]);
+            }
+        }
+        __syncthreads();
+        float a[N];
+#pragma unroll
+        for (int K = 0; K < N; ++K) a[K] = vin[t * RS + K];
+        const int r = q * T + t;
+#pragma unroll
+        for (int pz = 0; pz < SM; ++pz) {
+            const int k = kidx<N>(pz);
+            float re = 0.f, im = 0.f;
+#pragma unroll
+            for (int K = 0; K < N; ++K) {
+                re += a[K] * cosN<N>(k * K);
+                im -= a[K] * sinN<N>(k * K);
+            }
+            A[aidx32(r >> 5, r & 31, pz)] = make_float2(re, im);
+        }
+        __syncthreads();
+    }
+    // y: thread t < 256 owns (I, pz) = (t / 8, t % 8): 32 values, the 8 y modes
+    if (t < N * SM) {
+        const int I = t >> 3, pz = t & 7;
+        float2 a[N];
+#pragma unroll
+        for (int J = 0; J < N; ++J) a[J] = A[aidx32(I, J, pz)];
+#pragma unroll
+        for (int py = 0; py < SM; ++py) {
+            const int k = kidx<N>(py);
+            float re = 0.f, im = 0.f;
+#pragma unroll
+            for (int J = 0; J < N; ++J) {
+                const float c = cosN<N>(k * J), sn = sinN<N>(k * J);
+                re += a[J].x * c + a[J].y * sn;
+                im += a[J].y * c - a[J].x * sn;
+            }
+            Bs[(I * SM + py) * SM + pz] = make_float2(re, im);
+        }
+    }
+    __syncthreads();
+    // x: thread t < 64 owns (py, pz): 32 values, the 8 x modes
+    if (t < SM * SM) {
+        float2 a[N];
+#pragma unroll
+        for (int I = 0; I < N; ++I) a[I] = Bs[I * SM * SM + t];
+#pragma unroll
+        for (int px = 0; px < SM; ++px) {
+            const int k = kidx<N>(px);
+            float re = 0.f, im = 0.f;
+#pragma unroll
+            for (int I = 0; I < N; ++I) {
+                const float c = cosN<N>(k * I), sn = sinN<N>(k * I);
+                re += a[I].x * c + a[I].y * sn;
+                im += a[I].y * c - a[I].x * sn;
+            }
+            X[slice * SM3 + px * SM * SM + t] = make_float2(scale * re, scale * im);
+        }
+    }
+}
+
+// The inverse transform on the 32^3 grid, 512 threads per slice: x and y
+// stages in LDS (the mode block Ys shares its space with the y-stage output
+// T2, and T1 with the output staging), then two passes of 512 grid rows:
+// thread t computes row t's 32 outputs into LDS (ob) and the workgroup writes
+// the pass with coalesced 16-byte stores.
+template <typename TA>
+__global__ __launch_bounds__(kDft32Threads) void k_sur_idft32(const float2 *__restrict__ Y, float scale,
+                                                            int accumulate, TA *__restrict__ out) {
+    constexpr int N = 32, N3 = SG<32>::N3, T = kDft32Threads, RS = N + 1;
+    __shared__ float2 T2[N * kRowsI];         // [I][J][pz] (aidx32); first the modes [px][py][pz]
+    __shared__ float ob[T * RS];              // T1 [I][py][pz], then one pass of output rows
+    float2 *Ys = T2;
+    float2 *T1 = reinterpret_cast<float2 *>(ob);
+    const int t = threadIdx.x;
+    const int64_t slice = blockIdx.x;
+    Ys[t] = Y[slice * SM3 + t];
+    __syncthreads();
+    // x: thread t < 64 owns (py, pz): 8 modes, the 32 grid values I
+    if (t < SM * SM) {
+        float2 a[SM];
+#pragma unroll
+        for (int px = 0; px < SM; ++px) a[px] = Ys[px * SM * SM + t];
+#pragma unroll
+        for (int I = 0; I < N; ++I) {
+            float re = 0.f, im = 0.f;
+#pragma unroll
+            for (int px = 0; px < SM; ++px) {
+                const float c = cosN<N>(kidx<N>(px) * I), sn = sinN<N>(kidx<N>(px) * I);
+                re += a[px].x * c - a[px].y * sn;
+                im += a[px].x * sn + a[px].y * c;
+            }
+            T1[I * SM * SM + t] = make_float2(re, im);
+        }
+    }
+    __syncthreads();
+    // y: thread t < 256 owns (I, pz) = (t / 8, t % 8): 8 values, the 32 rows J
+    if (t < N * SM) {
+        const int I = t >> 3, pz = t & 7;
+        float2 a[SM];
+#pragma unroll
+        for (int py = 0; py < SM; ++py) a[py] = T1[(I * SM + py) * SM + pz];
+#pragma unroll
+        for (int J = 0; J < N; ++J) {
+            float re = 0.f, im = 0.f;
+#pragma unroll
+            for (int py = 0; py < SM; ++py) {
+                const float c = cosN<N>(kidx<N>(py) * J), sn = sinN<N>(kidx<N>(py) * J);
+                re += a[py].x * c - a[py].y * sn;
+                im += a[py].x * sn + a[py].y * c;
+            }
+            T2[aidx32(I, J, pz)] = make_float2(re, im);
+        }
+    }
+    __syncthreads();
+    // z: two passes of 512 rows
+#pragma unroll 1
+    for (int q = 0; q < N * N / T; ++q) {
+        const int r = q * T + t;
+        float2 a[SM];
+#pragma unroll
+        for (int pz = 0; pz < SM; ++pz) a[pz] = T2[aidx32(r >> 5, r & 31, pz)];
+#pragma unroll
+        for (int K = 0; K < N; ++K) {
+            float sum = 0.f;
+#pragma unroll
+            for (int pz = 0; pz < SM; ++pz)
+                sum += a[pz].x * cosN<N>(kidx<N>(pz) * K) - a[pz].y * sinN<N>(kidx<N>(pz) * K);
+            ob[t * RS + K] = scale * sum;
+        }
+        __syncthreads();
+        TA *rows = out + slice * N3 + int64_t(q) * T * N;
+        if constexpr (std::is_same<TA, float>::value) {
+#pragma unroll
+            for (int w = 0; w < T * N / 4 / T; ++w) {
+                const int e4 = t + T * w;
+                const float *sp = ob + (e4 >> 3) * RS + (e4 & 7) * 4;
+                float4 x = make_float4(sp[0], sp[1], sp[2], sp[3]);
+                float4 *dp = reinterpret_cast<float4 *>(rows) + e4;
+                if (accumulate) {
+                    const float4 p0 = *dp;
+                    x.x += p0.x;
+                    x.y += p0.y;
+                    x.z += p0.z;
+                    x.w += p0.w;
+                }
+                *dp = x;
+            }
+        } else {                              // 16-bit inference output (never accumulated)
+#pragma unroll
+            for (int w = 0; w < T * N / 8 / T; ++w) {
+                const int e8 = t + T * w;
+                const float *sp = ob + (e8 >> 2) * RS + (e8 & 3) * 8;
+                Pack8<TA> x;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) x.v[j] = TA(sp[j]);
+                reinterpret_cast<Pack8<TA> *>(rows)[e8] = x;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Launchers of the per-slice transforms on the N^3 grid.
+template <int N, typename TA>
+void launch_dft(const TA *v, float scale, float2 *X, unsigned slices, hipStream_t st) {
+    if constexpr (N == 16) hipLaunchKernelGGL(k_sur_dft16<TA>, dim3(slices), dim3(256), 0, st, v, scale, X);
+    else hipLaunchKernelGGL(k_sur_dft32<TA>, dim3(slices), dim3(kDft32Threads), 0, st, v, scale, X);
+}
+template <int N, typename TA>
+void launch_idft(const float2 *Y, float scale, int accumulate, TA *out, unsigned slices, hipStream_t st) {
+    if constexpr (N == 16) hipLaunchKernelGGL(k_sur_idft16<TA>, dim3(slices), dim3(256), 0, st, Y, scale, accumulate, out);
+    else hipLaunchKernelGGL(k_sur_idft32<TA>, dim3(slices), dim3(kDft32Threads), 0, st, Y, scale, accumulate, out);
 }
 
 // Weight gradient of the mixing: gR(i,o,k) += sum_b conj X(b,i,k) gY(b,o,k).
@@ -311,21 +534,21 @@ __global__ __launch_bounds__(256) void k_sur_mix_lds(const float2 *__restrict__ 
 }
 
 // v_0(b, c, x) = P_c g(b, x) + p0_c
-template <typename TA>
+template <int N, typename TA>
 __global__ __launch_bounds__(256) void k_sur_lift(const float *__restrict__ g, const float *__restrict__ th,
                                                   TA *__restrict__ v, int64_t total) {
     const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (idx >= total) return;
-    const int64_t b = idx >> 12;
-    const int x = int(idx & (SN3 - 1));
+    const int64_t b = idx >> SG<N>::LOG3;
+    const int x = int(idx & (SG<N>::N3 - 1));
     const float gv = g[idx];
 #pragma unroll 8
-    for (int c = 0; c < SW; ++c) stA(v, (b * SW + c) * SN3 + x, th[c] * gv + th[SW + c]);
+    for (int c = 0; c < SW; ++c) stA(v, (b * SW + c) * SG<N>::N3 + x, th[c] * gv + th[SW + c]);
 }
 
 // z = Wm v + b + s; vout = act ? gelu(z) : z (z stored when zout is set).
 // One thread per two neighbouring grid points (total = points, even).
-template <typename TA>
+template <int N, typename TA>
 __global__ __launch_bounds__(256) void k_sur_pw(const TA *__restrict__ v, const TA *__restrict__ s,
                                                 const float *__restrict__ lw, int act, TA *__restrict__ zout,
                                                 TA *__restrict__ vout, int64_t total) {
@@ -333,14 +556,14 @@ __global__ __launch_bounds__(256) void k_sur_pw(const TA *__restrict__ v, const 
     // arrive through scalar loads instead of LDS
     const int64_t idx = 2 * (int64_t(blockIdx.x) * 256 + threadIdx.x);
     if (idx >= total) return;
-    const int64_t b = idx >> 12;
-    const int x = int(idx & (SN3 - 1));
+    const int64_t b = idx >> SG<N>::LOG3;
+    const int x = int(idx & (SG<N>::N3 - 1));
     float2 vi[SW];
 #pragma unroll
-    for (int i = 0; i < SW; ++i) vi[i] = ld2(v, (b * SW + i) * SN3 + x);
+    for (int i = 0; i < SW; ++i) vi[i] = ld2(v, (b * SW + i) * SG<N>::N3 + x);
 #pragma unroll 4
     for (int o = 0; o < SW; ++o) {
-        const int64_t at = (b * SW + o) * SN3 + x;
+        const int64_t at = (b * SW + o) * SG<N>::N3 + x;
         const float2 sv = ld2(s, at);
         float z0 = lw[SW * SW + o] + sv.x, z1 = lw[SW * SW + o] + sv.y;
 #pragma unroll
@@ -354,34 +577,35 @@ __global__ __launch_bounds__(256) void k_sur_pw(const TA *__restrict__ v, const 
 }
 
 // out(b, x) = Qw . v_L(b, :, x) + q0
-template <typename TA>
+template <int N, typename TA>
 __global__ __launch_bounds__(256) void k_sur_proj(const TA *__restrict__ v, const float *__restrict__ qw,
                                                   float *__restrict__ out, int64_t total) {
     const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (idx >= total) return;
-    const int64_t b = idx >> 12;
-    const int x = int(idx & (SN3 - 1));
+    const int64_t b = idx >> SG<N>::LOG3;
+    const int x = int(idx & (SG<N>::N3 - 1));
     float s = qw[SW];
 #pragma unroll 8
-    for (int c = 0; c < SW; ++c) s += qw[c] * ldA(v, (b * SW + c) * SN3 + x);
+    for (int c = 0; c < SW; ++c) s += qw[c] * ldA(v, (b * SW + c) * SG<N>::N3 + x);
     out[idx] = s;
 }
 
 // gv(b, c, x) = Qw_c gout(b, x)
+template <int N>
 __global__ __launch_bounds__(256) void k_sur_proj_bwd(const float *__restrict__ gout, const float *__restrict__ qw,
                                                       float *__restrict__ gv, int64_t total) {
     const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (idx >= total) return;
-    const int64_t b = idx >> 12;
-    const int x = int(idx & (SN3 - 1));
+    const int64_t b = idx >> SG<N>::LOG3;
+    const int x = int(idx & (SG<N>::N3 - 1));
     const float g = gout[idx];
 #pragma unroll 8
-    for (int c = 0; c < SW; ++c) gv[(b * SW + c) * SN3 + x] = qw[c] * g;
+    for (int c = 0; c < SW; ++c) gv[(b * SW + c) * SG<N>::N3 + x] = qw[c] * g;
 }
 
 // gz = gv * gelu'(z) (or gv for the last layer), written over gv;
 // gvin = Wm^T gz.
-template <typename TA>
+template <int N, typename TA>
 __global__ __launch_bounds__(256) void k_sur_pw_bwd(float *__restrict__ gv, const TA *__restrict__ z,
                                                     const float *__restrict__ lw, int act, float *__restrict__ gvin,
                                                     int64_t total) {
@@ -389,14 +613,14 @@ __global__ __launch_bounds__(256) void k_sur_pw_bwd(float *__restrict__ gv, cons
     // by gz_o and added into the 32 input-channel sums
     const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (idx >= total) return;
-    const int64_t b = idx >> 12;
-    const int x = int(idx & (SN3 - 1));
+    const int64_t b = idx >> SG<N>::LOG3;
+    const int x = int(idx & (SG<N>::N3 - 1));
     float a[SW];
 #pragma unroll
     for (int i = 0; i < SW; ++i) a[i] = 0.f;
 #pragma unroll 4
     for (int o = 0; o < SW; ++o) {
-        const int64_t at = (b * SW + o) * SN3 + x;
+        const int64_t at = (b * SW + o) * SG<N>::N3 + x;
         const float g = gv[at];
         const float gz = act ? g * gelu_grad(ldA(z, at)) : g;
         gv[at] = gz;
@@ -404,14 +628,13 @@ __global__ __launch_bounds__(256) void k_sur_pw_bwd(float *__restrict__ gv, cons
         for (int i = 0; i < SW; ++i) a[i] += lw[o * SW + i] * gz;
     }
 #pragma unroll
-    for (int i = 0; i < SW; ++i) gvin[(b * SW + i) * SN3 + x] = a[i];
+    for (int i = 0; i < SW; ++i) gvin[(b * SW + i) * SG<N>::N3 + x] = a[i];
 }
 
 // G(o, i) += sum_{b, x} A(b, o, x) V(b, i, x);  gb(o) += sum_{b, x} A(b, o, x).
-// Grid 8 * batch: workgroup w sums the 512 points from (w % 8) * 512 of DOF
-// w / 8 in LDS tiles of 64.
-static_assert(SN3 / 512 == 8, "k_sur_wgrad splits a DOF into eight 512-point tiles");
-template <int COUT, int CIN, typename TV>
+// Grid T * batch with T = N^3 / 512 tiles per DOF: workgroup w sums the 512
+// points from (w % T) * 512 of DOF w / T in LDS tiles of 64.
+template <int N, int COUT, int CIN, typename TV>
 __global__ __launch_bounds__(256) void k_sur_wgrad(const float *__restrict__ A, const TV *__restrict__ V,
                                                    float *__restrict__ G, float *__restrict__ gb) {
     constexpr int NO = COUT * CIN;
@@ -419,15 +642,16 @@ __global__ __launch_bounds__(256) void k_sur_wgrad(const float *__restrict__ A, 
     __shared__ float As[COUT][65];
     __shared__ float Vs[CIN][65];
     const int t = threadIdx.x;
-    const int64_t b = blockIdx.x >> 3;
-    const int x0 = int(blockIdx.x & 7) * 512;
+    constexpr int TILES = SG<N>::N3 / 512;
+    const int64_t b = blockIdx.x / TILES;
+    const int x0 = int(blockIdx.x % TILES) * 512;
     float acc[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) acc[q] = 0.f;
     float accb = 0.f;
     for (int p0 = x0; p0 < x0 + 512; p0 += 64) {
-        for (int e = t; e < COUT * 64; e += 256) As[e >> 6][e & 63] = A[(b * COUT + (e >> 6)) * SN3 + p0 + (e & 63)];
-        for (int e = t; e < CIN * 64; e += 256) Vs[e >> 6][e & 63] = ldA(V, (b * CIN + (e >> 6)) * SN3 + p0 + (e & 63));
+        for (int e = t; e < COUT * 64; e += 256) As[e >> 6][e & 63] = A[(b * COUT + (e >> 6)) * SG<N>::N3 + p0 + (e & 63)];
+        for (int e = t; e < CIN * 64; e += 256) Vs[e >> 6][e & 63] = ldA(V, (b * CIN + (e >> 6)) * SG<N>::N3 + p0 + (e & 63));
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
@@ -698,18 +922,18 @@ __global__ __launch_bounds__(256) void k_sur_mix_first(const float2 *__restrict_
 }
 
 // v_1(o) = gelu(A_o g + a0_o + s_0(o)) per point
-template <typename TA>
+template <int N, typename TA>
 __global__ __launch_bounds__(256) void k_sur_pw_first(const float *__restrict__ g, const TA *__restrict__ s,
                                                       const float *__restrict__ fw, TA *__restrict__ vout,
                                                       int64_t total) {
     const int64_t idx = 2 * (int64_t(blockIdx.x) * 256 + threadIdx.x);   // two points per thread
     if (idx >= total) return;
-    const int64_t b = idx >> 12;
-    const int x = int(idx & (SN3 - 1));
+    const int64_t b = idx >> SG<N>::LOG3;
+    const int x = int(idx & (SG<N>::N3 - 1));
     const float2 gv = ld2(g, idx);
 #pragma unroll 8
     for (int o = 0; o < SW; ++o) {
-        const int64_t at = (b * SW + o) * SN3 + x;
+        const int64_t at = (b * SW + o) * SG<N>::N3 + x;
         const float2 sv = ld2(s, at);
         st2(vout, at, make_float2(gelu(fw[FA + o] * gv.x + fw[FA0 + o] + sv.x),
                                   gelu(fw[FA + o] * gv.y + fw[FA0 + o] + sv.y)));
@@ -735,17 +959,17 @@ __global__ __launch_bounds__(256) void k_sur_mix_last(const float2 *__restrict__
 }
 
 // out(b,x) = QW . v(b,:,x) + qb + s1(b,x)
-template <typename TA>
+template <int N, typename TA>
 __global__ __launch_bounds__(256) void k_sur_pw_last(const TA *__restrict__ v, const TA *__restrict__ s1,
                                                      const float *__restrict__ fw, float *__restrict__ out,
                                                      int64_t total) {
     const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (idx >= total) return;
-    const int64_t b = idx >> 12;
-    const int x = int(idx & (SN3 - 1));
+    const int64_t b = idx >> SG<N>::LOG3;
+    const int x = int(idx & (SG<N>::N3 - 1));
     float a = fw[FQB] + ldA(s1, idx);
 #pragma unroll 8
-    for (int i = 0; i < SW; ++i) a += fw[FQW + i] * ldA(v, (b * SW + i) * SN3 + x);
+    for (int i = 0; i < SW; ++i) a += fw[FQW + i] * ldA(v, (b * SW + i) * SG<N>::N3 + x);
     out[idx] = a;
 }
 
@@ -755,9 +979,10 @@ int check_shape(const fq_sur_shape *s, const char *who) {
     if (!s) return fq_internal_fail(FQ_ERR_INVALID, std::string(who) + ": null shape");
     if (s->abi_version != FQ_SUR_ABI_VERSION)
         return fq_internal_fail(FQ_ERR_INVALID, std::string(who) + ": ABI version mismatch");
-    if (s->Nv != SN || s->width != SW || s->modes != SM || s->layers < 1 || s->layers > FQ_SUR_MAX_LAYERS)
+    if ((s->Nv != 16 && s->Nv != 32) || s->width != SW || s->modes != SM || s->layers < 1 ||
+        s->layers > FQ_SUR_MAX_LAYERS)
         return fq_internal_fail(FQ_ERR_UNSUPPORTED, std::string(who) +
-                                ": supported shape is Nv 16, width 32, modes 8, 1..8 layers");
+                                ": supported shapes are Nv 16 or 32, width 32, modes 8, 1..8 layers");
     return FQ_SUCCESS;
 }
 
@@ -777,13 +1002,13 @@ struct ActPtrs {
 // Activation layout (floats): training keeps v_0..v_L, z_0..z_{L-1} and
 // X_0..X_{L-1}; inference keeps two v buffers and one X.  Both end with the
 // s and Y scratch.
-int64_t act_floats(int L, int64_t B, bool training) {
-    const int64_t vol = B * SW * SN3, md = 2 * B * SW * SM3;
+int64_t act_floats(int N3, int L, int64_t B, bool training) {
+    const int64_t vol = B * SW * N3, md = 2 * B * SW * SM3;
     return training ? (L + 1) * vol + L * vol + L * md + vol + md : 2 * vol + md + vol + md;
 }
 
-ActPtrs act_ptrs(float *acts, int L, int64_t B, bool training) {
-    const int64_t vol = B * SW * SN3, md = 2 * B * SW * SM3;
+ActPtrs act_ptrs(int N3, float *acts, int L, int64_t B, bool training) {
+    const int64_t vol = B * SW * N3, md = 2 * B * SW * SM3;
     ActPtrs p{};
     float *q = acts;
     if (training) {
@@ -806,98 +1031,121 @@ ActPtrs act_ptrs(float *acts, int L, int64_t B, bool training) {
 
 // The folded inference pass with activations stored as A (FP32 or FP16) in
 // the FP32-sized regions of the inference activation layout.
-template <typename A>
+template <int N, typename A>
 void forward_fused(int L, const float *theta, const float *fused, const float *g, int64_t batch, float *acts,
                    float *out, hipStream_t st) {
-    const ActPtrs p = act_ptrs(acts, L, batch, false);
+    const ActPtrs p = act_ptrs(SG<N>::N3, acts, L, batch, false);
     A *v[2] = {reinterpret_cast<A *>(p.v[0]), reinterpret_cast<A *>(p.v[1])};
     A *sa = reinterpret_cast<A *>(p.s);
-    const int64_t pts = batch * SN3;
+    const int64_t pts = batch * SG<N>::N3;
     const unsigned slices = unsigned(batch * SW);
     // first layer from the one-channel transform of g
-    hipLaunchKernelGGL(k_sur_dft<float>, dim3(unsigned(batch)), dim3(256), 0, st, g, 1.0f / SN3, p.X[0]);
+    launch_dft<N, float>(g, 1.0f / SG<N>::N3, p.X[0], unsigned(batch), st);
     hipLaunchKernelGGL(k_sur_mix_first, dim3(nblk(batch * SW * SM3)), dim3(256), 0, st, p.X[0], fused, p.Y,
                        batch * SW * SM3);
-    hipLaunchKernelGGL(k_sur_idft<A>, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, sa);
-    hipLaunchKernelGGL(k_sur_pw_first<A>, dim3(nblk(pts / 2)), dim3(256), 0, st, g, sa, fused, v[1], pts);
+    launch_idft<N, A>(p.Y, 1.0f, 0, sa, slices, st);
+    hipLaunchKernelGGL((k_sur_pw_first<N, A>), dim3(nblk(pts / 2)), dim3(256), 0, st, g, sa, fused, v[1], pts);
     for (int l = 1; l < L - 1; ++l) {
         const float *lay = theta + 2 * SW + l * SLAYER;
         const float2 *R = reinterpret_cast<const float2 *>(lay);
-        hipLaunchKernelGGL(k_sur_dft<A>, dim3(slices), dim3(256), 0, st, v[l & 1], 1.0f / SN3, p.X[l]);
+        launch_dft<N, A>(v[l & 1], 1.0f / SG<N>::N3, p.X[l], slices, st);
         hipLaunchKernelGGL(k_sur_mix_lds<0>, dim3(SM3 / 8, unsigned((batch + 31) / 32)), dim3(256), 0, st, p.X[l],
                            R, p.Y, batch);
-        hipLaunchKernelGGL(k_sur_idft<A>, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, sa);
-        hipLaunchKernelGGL(k_sur_pw<A>, dim3(nblk(pts / 2)), dim3(256), 0, st, v[l & 1], sa, lay + SR, 1, nullptr,
+        launch_idft<N, A>(p.Y, 1.0f, 0, sa, slices, st);
+        hipLaunchKernelGGL((k_sur_pw<N, A>), dim3(nblk(pts / 2)), dim3(256), 0, st, v[l & 1], sa, lay + SR, 1, nullptr,
                            v[(l + 1) & 1], pts);
     }
     // last layer and projection as one output channel
-    hipLaunchKernelGGL(k_sur_dft<A>, dim3(slices), dim3(256), 0, st, v[(L - 1) & 1], 1.0f / SN3, p.X[L - 1]);
+    launch_dft<N, A>(v[(L - 1) & 1], 1.0f / SG<N>::N3, p.X[L - 1], slices, st);
     hipLaunchKernelGGL(k_sur_mix_last, dim3(nblk(batch * SM3)), dim3(256), 0, st, p.X[L - 1], fused, p.Y,
                        batch * SM3);
-    hipLaunchKernelGGL(k_sur_idft<A>, dim3(unsigned(batch)), dim3(256), 0, st, p.Y, 1.0f, 0, sa);
-    hipLaunchKernelGGL(k_sur_pw_last<A>, dim3(nblk(pts)), dim3(256), 0, st, v[(L - 1) & 1], sa, fused, out, pts);
+    launch_idft<N, A>(p.Y, 1.0f, 0, sa, unsigned(batch), st);
+    hipLaunchKernelGGL((k_sur_pw_last<N, A>), dim3(nblk(pts)), dim3(256), 0, st, v[(L - 1) & 1], sa, fused, out, pts);
 }
 
 // The unfolded pass (training keeps v_l, z_l and X_l) with activations
 // stored as A in the FP32-sized regions of the activation layout.
-template <typename A>
+template <int N, typename A>
 void forward_impl(int L, const float *theta, const float *g, int64_t batch, bool tr, float *acts, float *out,
                   hipStream_t st) {
-    const ActPtrs p = act_ptrs(acts, L, batch, tr);
-    const int64_t pts = batch * SN3;
+    const ActPtrs p = act_ptrs(SG<N>::N3, acts, L, batch, tr);
+    const int64_t pts = batch * SG<N>::N3;
     const unsigned slices = unsigned(batch * SW);
     A *sa = reinterpret_cast<A *>(p.s);
-    hipLaunchKernelGGL(k_sur_lift<A>, dim3(nblk(pts)), dim3(256), 0, st, g, theta, reinterpret_cast<A *>(p.v[0]), pts);
+    hipLaunchKernelGGL((k_sur_lift<N, A>), dim3(nblk(pts)), dim3(256), 0, st, g, theta, reinterpret_cast<A *>(p.v[0]), pts);
     for (int l = 0; l < L; ++l) {
         const float *lay = theta + 2 * SW + l * SLAYER;
         const float2 *R = reinterpret_cast<const float2 *>(lay);
         A *vl = reinterpret_cast<A *>(p.v[l]), *vn = reinterpret_cast<A *>(p.v[l + 1]);
-        hipLaunchKernelGGL(k_sur_dft<A>, dim3(slices), dim3(256), 0, st, vl, 1.0f / SN3, p.X[l]);
+        launch_dft<N, A>(vl, 1.0f / SG<N>::N3, p.X[l], slices, st);
         hipLaunchKernelGGL(k_sur_mix_lds<0>, dim3(SM3 / 8, unsigned((batch + 31) / 32)), dim3(256), 0, st, p.X[l],
                            R, p.Y, batch);
-        hipLaunchKernelGGL(k_sur_idft<A>, dim3(slices), dim3(256), 0, st, p.Y, 1.0f, 0, sa);
-        hipLaunchKernelGGL(k_sur_pw<A>, dim3(nblk(pts / 2)), dim3(256), 0, st, vl, sa, lay + SR, int(l < L - 1),
+        launch_idft<N, A>(p.Y, 1.0f, 0, sa, slices, st);
+        hipLaunchKernelGGL((k_sur_pw<N, A>), dim3(nblk(pts / 2)), dim3(256), 0, st, vl, sa, lay + SR, int(l < L - 1),
                            tr ? reinterpret_cast<A *>(p.z[l]) : nullptr, vn, pts);
     }
-    hipLaunchKernelGGL(k_sur_proj<A>, dim3(nblk(pts)), dim3(256), 0, st, reinterpret_cast<const A *>(p.v[L]),
+    hipLaunchKernelGGL((k_sur_proj<N, A>), dim3(nblk(pts)), dim3(256), 0, st, reinterpret_cast<const A *>(p.v[L]),
                        theta + 2 * SW + L * SLAYER, out, pts);
 }
 
 // Gradients from the activations of a training forward_impl<A>; the
 // gradients themselves (gv, gz, the mode arrays) stay FP32.
-template <typename A>
+template <int N, typename A>
 void backward_impl(int L, const float *theta, const float *g, int64_t batch, float *acts, const float *gout,
                    float *grad, float *work, hipStream_t st) {
-    const ActPtrs p = act_ptrs(acts, L, batch, true);
-    const int64_t pts = batch * SN3, vol = batch * SW * SN3;
+    const ActPtrs p = act_ptrs(SG<N>::N3, acts, L, batch, true);
+    const int64_t pts = batch * SG<N>::N3, vol = batch * SW * SG<N>::N3;
     const unsigned slices = unsigned(batch * SW);
     float *gv = work, *gvin = work + vol;
     float2 *gY = reinterpret_cast<float2 *>(work + 2 * vol);
     float2 *gX = gY + batch * SW * SM3;
-    const dim3 wg(unsigned(batch * (SN3 / 512)));
+    const dim3 wg(unsigned(batch * (SG<N>::N3 / 512)));
     const int64_t qoff = 2 * SW + L * SLAYER;
-    hipLaunchKernelGGL((k_sur_wgrad<1, SW, A>), wg, dim3(256), 0, st, gout, reinterpret_cast<const A *>(p.v[L]),
+    hipLaunchKernelGGL((k_sur_wgrad<N, 1, SW, A>), wg, dim3(256), 0, st, gout, reinterpret_cast<const A *>(p.v[L]),
                        grad + qoff, grad + qoff + SW);
-    hipLaunchKernelGGL(k_sur_proj_bwd, dim3(nblk(pts)), dim3(256), 0, st, gout, theta + qoff, gv, pts);
+    hipLaunchKernelGGL(k_sur_proj_bwd<N>, dim3(nblk(pts)), dim3(256), 0, st, gout, theta + qoff, gv, pts);
     for (int l = L - 1; l >= 0; --l) {
         const int64_t lo = 2 * SW + l * SLAYER;
         const float2 *R = reinterpret_cast<const float2 *>(theta + lo);
         float2 *gR = reinterpret_cast<float2 *>(grad + lo);
-        hipLaunchKernelGGL(k_sur_pw_bwd<A>, dim3(nblk(pts)), dim3(256), 0, st, gv,
+        hipLaunchKernelGGL((k_sur_pw_bwd<N, A>), dim3(nblk(pts)), dim3(256), 0, st, gv,
                            reinterpret_cast<const A *>(p.z[l]), theta + lo + SR, int(l < L - 1), gvin, pts);
-        hipLaunchKernelGGL((k_sur_wgrad<SW, SW, A>), wg, dim3(256), 0, st, gv, reinterpret_cast<const A *>(p.v[l]),
+        hipLaunchKernelGGL((k_sur_wgrad<N, SW, SW, A>), wg, dim3(256), 0, st, gv, reinterpret_cast<const A *>(p.v[l]),
                            grad + lo + SR, grad + lo + SR + SW * SW);
-        hipLaunchKernelGGL(k_sur_dft<float>, dim3(slices), dim3(256), 0, st, gv, 1.0f, gY);
+        launch_dft<N, float>(gv, 1.0f, gY, slices, st);
         hipLaunchKernelGGL(k_sur_mix_grad, dim3(nblk(int64_t(SW) * SW * SM3)), dim3(256), 0, st, p.X[l], gY, gR,
                            batch);
         hipLaunchKernelGGL(k_sur_mix_lds<1>, dim3(SM3 / 8, unsigned((batch + 31) / 32)), dim3(256), 0, st, gY, R,
                            gX, batch);
-        hipLaunchKernelGGL(k_sur_idft<float>, dim3(slices), dim3(256), 0, st, gX, 1.0f / SN3, 1, gvin);
+        launch_idft<N, float>(gX, 1.0f / SG<N>::N3, 1, gvin, slices, st);
         float *tmp = gv;
         gv = gvin;
         gvin = tmp;
     }
-    hipLaunchKernelGGL((k_sur_wgrad<SW, 1, float>), wg, dim3(256), 0, st, gv, g, grad, grad + SW);
+    hipLaunchKernelGGL((k_sur_wgrad<N, SW, 1, float>), wg, dim3(256), 0, st, gv, g, grad, grad + SW);
+}
+
+// Dispatch on the activation storage type.
+template <int N>
+void forward_acts(int acts_type, int L, const float *theta, const float *g, int64_t batch, bool tr, float *acts,
+                  float *out, hipStream_t st) {
+    if (acts_type == FQ_SUR_ACTS_FP16) forward_impl<N, _Float16>(L, theta, g, batch, tr, acts, out, st);
+    else if (acts_type == FQ_SUR_ACTS_BF16) forward_impl<N, __bf16>(L, theta, g, batch, tr, acts, out, st);
+    else forward_impl<N, float>(L, theta, g, batch, tr, acts, out, st);
+}
+template <int N>
+void fused_acts(int acts_type, int L, const float *theta, const float *fused, const float *g, int64_t batch,
+                float *acts, float *out, hipStream_t st) {
+    if (acts_type == FQ_SUR_ACTS_FP16) forward_fused<N, _Float16>(L, theta, fused, g, batch, acts, out, st);
+    else if (acts_type == FQ_SUR_ACTS_BF16) forward_fused<N, __bf16>(L, theta, fused, g, batch, acts, out, st);
+    else forward_fused<N, float>(L, theta, fused, g, batch, acts, out, st);
+}
+template <int N>
+void backward_acts(int acts_type, int L, const float *theta, const float *g, int64_t batch, float *acts,
+                   const float *gout, float *grad, float *work, hipStream_t st) {
+    if (acts_type == FQ_SUR_ACTS_FP16) backward_impl<N, _Float16>(L, theta, g, batch, acts, gout, grad, work, st);
+    else if (acts_type == FQ_SUR_ACTS_BF16) backward_impl<N, __bf16>(L, theta, g, batch, acts, gout, grad, work, st);
+    else backward_impl<N, float>(L, theta, g, batch, acts, gout, grad, work, st);
 }
 
 }  // namespace
@@ -910,8 +1158,9 @@ int fq_sur_sizes(const fq_sur_shape *shape, int64_t batch, int32_t training, int
     if (batch < 0) return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_sizes: negative batch");
     const int L = shape->layers;
     if (nparams) *nparams = 2 * SW + L * SLAYER + SW + 1;
-    if (act_size) *act_size = act_floats(L, batch, training != 0);
-    if (work_size) *work_size = 2 * batch * SW * SN3 + 4 * batch * SW * SM3;
+    const int N3 = shape->Nv * shape->Nv * shape->Nv;
+    if (act_size) *act_size = act_floats(N3, L, batch, training != 0);
+    if (work_size) *work_size = 2 * batch * SW * N3 + 4 * batch * SW * SM3;
     return FQ_SUCCESS;
 }
 
@@ -924,9 +1173,8 @@ int fq_sur_forward(const fq_sur_shape *shape, const float *theta, const float *g
     if (batch == 0) return FQ_SUCCESS;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const bool tr = training != 0;
-    if (acts_type == FQ_SUR_ACTS_FP16) forward_impl<_Float16>(shape->layers, theta, g, batch, tr, acts, out, st);
-    else if (acts_type == FQ_SUR_ACTS_BF16) forward_impl<__bf16>(shape->layers, theta, g, batch, tr, acts, out, st);
-    else forward_impl<float>(shape->layers, theta, g, batch, tr, acts, out, st);
+    if (shape->Nv == 16) forward_acts<16>(acts_type, shape->layers, theta, g, batch, tr, acts, out, st);
+    else forward_acts<32>(acts_type, shape->layers, theta, g, batch, tr, acts, out, st);
     return hip_status("fq_sur_forward");
 }
 
@@ -949,9 +1197,8 @@ int fq_sur_forward_fused(const fq_sur_shape *shape, const float *theta, const fl
         return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_forward_fused: invalid arguments");
     if (batch == 0) return FQ_SUCCESS;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    if (acts_type == FQ_SUR_ACTS_FP16) forward_fused<_Float16>(shape->layers, theta, fused, g, batch, acts, out, st);
-    else if (acts_type == FQ_SUR_ACTS_BF16) forward_fused<__bf16>(shape->layers, theta, fused, g, batch, acts, out, st);
-    else forward_fused<float>(shape->layers, theta, fused, g, batch, acts, out, st);
+    if (shape->Nv == 16) fused_acts<16>(acts_type, shape->layers, theta, fused, g, batch, acts, out, st);
+    else fused_acts<32>(acts_type, shape->layers, theta, fused, g, batch, acts, out, st);
     return hip_status("fq_sur_forward_fused");
 }
 
@@ -963,9 +1210,8 @@ int fq_sur_backward(const fq_sur_shape *shape, const float *theta, const float *
         return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_backward: invalid arguments");
     if (batch == 0) return FQ_SUCCESS;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    if (acts_type == FQ_SUR_ACTS_FP16) backward_impl<_Float16>(shape->layers, theta, g, batch, acts, gout, grad, work, st);
-    else if (acts_type == FQ_SUR_ACTS_BF16) backward_impl<__bf16>(shape->layers, theta, g, batch, acts, gout, grad, work, st);
-    else backward_impl<float>(shape->layers, theta, g, batch, acts, gout, grad, work, st);
+    if (shape->Nv == 16) backward_acts<16>(acts_type, shape->layers, theta, g, batch, acts, gout, grad, work, st);
+    else backward_acts<32>(acts_type, shape->layers, theta, g, batch, acts, gout, grad, work, st);
     return hip_status("fq_sur_backward");
 }
 

@@ -4,8 +4,10 @@ include/frfs_sur.h.
 
 The reference has no learned operator.  The stand-in takes the plugin place
 of ``vhs-gll`` (frfs/solvers/dgfs/scattering.py:227-286, type key
-``vhs-gll-fno``) on the 16^3 grid and keeps the spectral operator both as
-its teacher (training pairs are (f, Q(f)) from ``fs_batch``) and as its
+``vhs-gll-fno``) or of single-species ``vss-gll``
+(frfs/solvers/dgfsbi/scattering.py:259-330, type key ``vss-gll-fno``) on the
+16^3 or 32^3 grid and keeps the spectral operator both as its teacher
+(training pairs are (f, Q(f)) from the spectral ``fs_batch``) and as its
 fallback: ``fs_batch`` runs the network on every DOF, measures each output's
 conservation defect (the largest cosine between Q and a collision invariant
 1, c, |c|^2), writes the conservation-projected network output where the
@@ -38,10 +40,11 @@ import time
 import numpy as np
 
 from . import _lib
-from .scattering import DGFSVHSGLLScatteringModel, _stream
+from .scattering import DGFSBiVSSGLLScatteringModel, DGFSVHSGLLScatteringModel, _stream
 
 F_REF = math.pi ** -1.5
-NV, WIDTH, MODES = 16, 32, 8
+WIDTH, MODES = 32, 8
+GRIDS = (16, 32)                               # grids the kernels are built for
 ACTS = {'fp32': 0, 'fp16': 1, 'bf16': 2}      # FQ_SUR_ACTS_*
 
 
@@ -73,12 +76,15 @@ def init_params(layers, seed, width=WIDTH, modes=MODES):
 class FourierSurrogate:
     """The network, its parameters and optimiser state on the current device."""
 
-    def __init__(self, backend, layers=4, seed=0, theta=None, q_ref=1.0):
+    def __init__(self, backend, layers=4, seed=0, theta=None, q_ref=1.0, nv=16):
         import torch
+        if nv not in GRIDS:
+            raise ValueError(f'the learned stand-in is built for the grids {GRIDS}, not {nv}^3')
         self.torch = torch
         self.backend = backend
         self.dev = backend.device
-        self.shape = _lib.fq_sur_shape(_lib.SUR_ABI_VERSION, NV, WIDTH, MODES, layers)
+        self.nv = nv
+        self.shape = _lib.fq_sur_shape(_lib.SUR_ABI_VERSION, nv, WIDTH, MODES, layers)
         self.layers = layers
         self.nparams = self._sizes(0, False)[0]
         th = init_params(layers, seed) if theta is None else np.asarray(theta, dtype=np.float64)
@@ -118,7 +124,7 @@ class FourierSurrogate:
 
     # -- network -----------------------------------------------------------
     def forward(self, g, out, training=False, fused=True, acts=None):
-        """out[b] = network(g[b]) for (batch, 16^3) float32 device tensors.
+        """out[b] = network(g[b]) for (batch, N^3) float32 device tensors.
         Inference (training False) with two or more layers runs the folded
         first and last layers (fq_sur_forward_fused) unless fused is False;
         acts is the activation storage, 'fp32', 'fp16' or 'bf16' (default:
@@ -208,28 +214,35 @@ class FourierSurrogate:
                 break
         return hist
 
-    def predict(self, g, chunk=8192, fused=True, acts=None):
-        """Network output for (batch, 16^3) float32 rows, in chunks."""
-        out = self.torch.empty_like(g)
+    def default_chunk(self):
+        """DOFs per forward call: 8192 on the 16^3 grid, 1024 on the 32^3 grid
+        (inference activations of 3 x 32 channels x N^3 floats per DOF)."""
+        return 8192 if self.nv == 16 else 1024
+
+    def predict(self, g, chunk=None, fused=True, acts=None, out=None):
+        """Network output for (batch, N^3) float32 rows, in chunks."""
+        chunk = chunk or self.default_chunk()
+        out = self.torch.empty_like(g) if out is None else out
         for b0 in range(0, g.shape[0], chunk):
             self.forward(g[b0:b0 + chunk], out[b0:b0 + chunk], fused=fused, acts=acts)
         return out
 
     # -- persistence -------------------------------------------------------
     def save(self, path):
-        np.savez(path, theta=self.theta.cpu().numpy(), q_ref=self.q_ref, layers=self.layers)
+        np.savez(path, theta=self.theta.cpu().numpy(), q_ref=self.q_ref, layers=self.layers, nv=self.nv)
 
     @classmethod
     def load(cls, backend, path):
         with np.load(path, allow_pickle=False) as z:
+            nv = int(z['nv']) if 'nv' in z.files else 16
             return cls(backend, layers=int(z['layers']), theta=z['theta'].astype(np.float64),
-                       q_ref=float(z['q_ref']))
+                       q_ref=float(z['q_ref']), nv=nv)
 
 
 def training_pairs(model, f, chunk=8192):
     """(g, target, rho) float32/float64 device tensors for the rows of f
-    (ndof, 16^3): g = f / (rho f_ref), target = Q(f) / (rho^2 q) with q = 1
-    (the caller rescales by q_ref)."""
+    (ndof, N^3): g = f / (rho f_ref), target = Q(f) / (rho^2 q) with q = 1
+    (the caller rescales by q_ref), Q from the model's spectral operator."""
     import torch
     from .inputs import to_aosoa
     be, vm = model.backend, model.vm
@@ -244,32 +257,36 @@ def training_pairs(model, f, chunk=8192):
         fin = be.matrix((1, n, m))
         q = be.matrix((1, n, m))
         to_aosoa(f[d0:d0 + m], fin)
-        DGFSVHSGLLScatteringModel.fs_batch(model, fin, q)
+        model.fs_spectral(fin, q)
         st = _stream(be)
-        _lib.check(lib.fq_sur_gather(NV, vm.cw(), F_REF, 1, fin._as_parameter_, 1, fin.leaddim, m, 0, m, None,
+        nv = vm.Nv()
+        _lib.check(lib.fq_sur_gather(nv, vm.cw(), F_REF, 1, fin._as_parameter_, 1, fin.leaddim, m, 0, m, None,
                                      rho[d0:].data_ptr(), g[d0:].data_ptr(), st))
-        _lib.check(lib.fq_sur_gather(NV, vm.cw(), 1.0, 2, q._as_parameter_, 1, q.leaddim, m, 0, m,
+        _lib.check(lib.fq_sur_gather(nv, vm.cw(), 1.0, 2, q._as_parameter_, 1, q.leaddim, m, 0, m,
                                      rho[d0:].data_ptr(), None, t[d0:].data_ptr(), st))
     return g, t, rho
 
 
-class DGFSVHSGLLFNOScatteringModel(DGFSVHSGLLScatteringModel):
-    """vhs-gll with the learned stand-in in front of the spectral operator."""
-    scattering_model = 'vhs-gll-fno'
+class _FNOStandIn:
+    """The stand-in's logic, shared by its plugins: network in front, the
+    plugin's spectral operator (``fs_spectral``) as fallback and judge."""
     # fs_batch reads fallback flags on the host and branches on them
     capture_safe = False
 
-    def perform_precomputation(self):
-        super().perform_precomputation()
-        if self.vm.Nv() != NV:
-            raise ValueError(f'the learned stand-in is built for the {NV}^3 grid')
+    def _setup_standin(self):
+        nv = self.vm.Nv()
+        if nv not in GRIDS:
+            raise ValueError(f'the learned stand-in is built for the grids {GRIDS}, not {nv}^3')
         path = self.cfg.get('scattering-model', 'surrogate-weights', '')
         layers = self.cfg.getint('scattering-model', 'surrogate-layers', 4)
         self.tol = self.cfg.getfloat('scattering-model', 'surrogate-tol', 1e-4)
         self.check_every = self.cfg.getint('scattering-model', 'surrogate-check-every', 64)
         self.check_tol = self.cfg.getfloat('scattering-model', 'surrogate-check-tol', 3e-2)
         self.inplace_runs = 4        # more runs of flagged DOFs than this: one compact batch
-        self.net = FourierSurrogate.load(self.backend, path) if path else FourierSurrogate(self.backend, layers)
+        self.net = (FourierSurrogate.load(self.backend, path) if path
+                    else FourierSurrogate(self.backend, layers, nv=nv))
+        if self.net.nv != nv:
+            raise ValueError(f'surrogate-weights are for the {self.net.nv}^3 grid, the mesh is {nv}^3')
         acts = self.cfg.get('scattering-model', 'surrogate-activations', 'fp32')
         if acts not in ACTS:
             raise ValueError("[scattering-model] surrogate-activations must be 'fp32', 'fp16' or 'bf16'")
@@ -280,27 +297,21 @@ class DGFSVHSGLLFNOScatteringModel(DGFSVHSGLLScatteringModel):
         self._gdiag = torch.tensor(gd, dtype=torch.float64, device=self.backend.device)
         self.last_stats = None
 
-    def fs(self, d_arr_in, d_arr_out, elem, upt):
-        nrow = d_arr_in.traits[0]
-        self.fs_batch(d_arr_in, d_arr_out, elem * nrow + upt, 1)
-
-    def fs_spectral(self, d_arr_in, d_arr_out, first_dof=0, ndof=None):
-        """The spectral operator alone (the teacher and the fallback)."""
-        DGFSVHSGLLScatteringModel.fs_batch(self, d_arr_in, d_arr_out, first_dof, ndof)
-
-    def fs_batch(self, d_arr_in, d_arr_out, first_dof=0, ndof=None, chunk=8192):
+    def _standin_batch(self, d_arr_in, d_arr_out, first_dof=0, ndof=None, chunk=None):
         if d_arr_in._as_parameter_ == d_arr_out._as_parameter_:
             # the fallback and the sampled check read f after the network's Q
             # has been written, so Q may not alias f here
-            raise ValueError('vhs-gll-fno: d_arr_out must not alias d_arr_in (in-place evaluation '
-                             'is not supported by the learned stand-in)')
+            raise ValueError(f'{self.scattering_model}: d_arr_out must not alias d_arr_in (in-place '
+                             'evaluation is not supported by the learned stand-in)')
         torch = self.net.torch
+        chunk = chunk or self.net.default_chunk()
         nrow, ldim, _ = d_arr_in.traits
         neles = d_arr_in.ioshape[2]
         ndof = nrow * neles - first_dof if ndof is None else ndof
         lib = _lib.load()
         st = _stream(self.backend)
         n = self.vm.vsize()
+        nv = self.vm.Nv()
         cw, L = self.vm.cw(), self.vm.L()
         flags_all, defects, checks = [], [], []
         for d0 in range(first_dof, first_dof + ndof, chunk):
@@ -310,10 +321,10 @@ class DGFSVHSGLLFNOScatteringModel(DGFSVHSGLLScatteringModel):
             rho = self.net._buf('rho', m, torch.float64)
             dfc = self.net._buf('defect', m, torch.float64)
             flag = self.net._buf('flag', m, torch.int32)
-            _lib.check(lib.fq_sur_gather(NV, cw, F_REF, 1, d_arr_in._as_parameter_, nrow, ldim, neles, d0, m,
+            _lib.check(lib.fq_sur_gather(nv, cw, F_REF, 1, d_arr_in._as_parameter_, nrow, ldim, neles, d0, m,
                                          None, rho.data_ptr(), g.data_ptr(), st))
             self.net.forward(g, q)
-            _lib.check(lib.fq_sur_finish(NV, L, cw, self.net.q_ref, self.tol, q.data_ptr(), rho.data_ptr(),
+            _lib.check(lib.fq_sur_finish(nv, L, cw, self.net.q_ref, self.tol, q.data_ptr(), rho.data_ptr(),
                                          self._ginv.data_ptr(), self._gdiag.data_ptr(), d_arr_out._as_parameter_,
                                          nrow, ldim, neles, d0, m, dfc.data_ptr(), flag.data_ptr(), st))
             fl = flag[:m].cpu().numpy().copy()
@@ -375,3 +386,49 @@ class DGFSVHSGLLFNOScatteringModel(DGFSVHSGLLScatteringModel):
         qn = d_arr_out.data[idx]
         rel = ((qn - qs).pow(2).sum(1) / qs.pow(2).sum(1).clamp_min(1e-300)).sqrt()
         return float(rel.median())
+
+
+class DGFSVHSGLLFNOScatteringModel(_FNOStandIn, DGFSVHSGLLScatteringModel):
+    """vhs-gll with the learned stand-in in front of the spectral operator."""
+    scattering_model = 'vhs-gll-fno'
+
+    def perform_precomputation(self):
+        DGFSVHSGLLScatteringModel.perform_precomputation(self)
+        self._setup_standin()
+
+    def fs(self, d_arr_in, d_arr_out, elem, upt):
+        nrow = d_arr_in.traits[0]
+        self.fs_batch(d_arr_in, d_arr_out, elem * nrow + upt, 1)
+
+    def fs_spectral(self, d_arr_in, d_arr_out, first_dof=0, ndof=None):
+        """The spectral operator alone (the teacher and the fallback)."""
+        DGFSVHSGLLScatteringModel.fs_batch(self, d_arr_in, d_arr_out, first_dof, ndof)
+
+    def fs_batch(self, d_arr_in, d_arr_out, first_dof=0, ndof=None, chunk=None):
+        return self._standin_batch(d_arr_in, d_arr_out, first_dof, ndof, chunk)
+
+
+class DGFSBiVSSGLLFNOScatteringModel(_FNOStandIn, DGFSBiVSSGLLScatteringModel):
+    """Single-species vss-gll with the learned stand-in in front of the
+    spectral operator; the plugin signatures of vss-gll
+    (frfs/solvers/dgfsbi/scattering.py:323-330) with args = [f, Q]."""
+    scattering_model = 'vss-gll-fno'
+
+    def perform_precomputation(self):
+        if self._nspcs != 1:
+            raise ValueError('vss-gll-fno: the learned stand-in serves one species')
+        DGFSBiVSSGLLScatteringModel.perform_precomputation(self)
+        self._setup_standin()
+
+    def fs(self, elem, upt, *args):
+        nrow = args[0].traits[0]
+        self._standin_batch(args[0], args[1], elem * nrow + upt, 1)
+
+    def fs_spectral(self, d_arr_in, d_arr_out, first_dof=0, ndof=None):
+        """The spectral operator alone (the teacher and the fallback)."""
+        DGFSBiVSSGLLScatteringModel.fs_batch(self, d_arr_in, d_arr_out, first_dof=first_dof, ndof=ndof)
+
+    def fs_batch(self, *args, first_dof=0, ndof=None, chunk=None):
+        if len(args) != 2:
+            raise ValueError('vss-gll-fno: fs_batch(f, Q)')
+        return self._standin_batch(args[0], args[1], first_dof, ndof, chunk)

@@ -1,13 +1,20 @@
 """Train the learned stand-in on the GPU with the library's own kernels.
 
-    python -m frfs_amd.train_surrogate --pairs 200000 --epochs 20 --out fno16.npz
+    python -m frfs_amd.train_surrogate --grid vss32 --pairs 50000 --epochs 20 --out fno32.npz
 
 Training pairs come from the spectral operator itself: seeded mixtures of up
-to three sheared Maxwellians (``inputs.shear_mixture_params``) on the 16^3
-grid, Q(f) from ``vhs-gll`` in FP64 (omega = 0.5, M = 6, Nrho = 8,
-[-5, 5]^3: the operator of frfs/solvers/dgfs/scattering.py:227-286 at the
-reference's hard-sphere settings).  Forward, backward, loss and Adam run in
-the HIP kernels of include/frfs_sur.h; PyTorch only holds the device arrays.
+to three sheared Maxwellians (``inputs.shear_mixture_params``), Q(f) from the
+FP64 operator with the conservation projection (so the targets conserve
+mass, momentum and energy and the stand-in's defect check compares like with
+like).  Two teachers:
+  * ``vhs16``: ``vhs-gll`` on the 16^3 grid (omega = 0.5, M = 6, Nrho = 8,
+    [-5, 5]^3: the operator of frfs/solvers/dgfs/scattering.py:227-286 at the
+    reference's hard-sphere settings);
+  * ``vss32``: single-species ``vss-gll`` on the 32^3 grid (omega = 0.74,
+    alpha = 1.4, M = 12, Nrho = 16, Mpre = 32, [-6, 6]^3: the operator of
+    frfs/solvers/dgfsbi/scattering.py:259-330 at the 32^3 workload).
+Forward, backward, loss and Adam run in the HIP kernels of
+include/frfs_sur.h; PyTorch only holds the device arrays.
 """
 from __future__ import annotations
 
@@ -20,16 +27,47 @@ from .inifile import Inifile
 ARGON = dict(T0=273.0, H0=1e-3, rho0=0.00011140244750324529, molarMass0=39.96e-3)
 
 
-def config(tol=1e-4, weights='', layers=4):
-    """vhs-gll-fno on the 16^3 grid (hard spheres, M = 6, Nrho = 8)."""
-    return Inifile.from_dict({
-        'non-dim': ARGON,
-        'constants': {'Nv': 16, 'NvBatchSize': 16 ** 3, 'Nrho': 8},
-        'velocity-mesh': {'dev': 5.0, 'cmax': 0, 'Tmax': 273},
-        'spherical-design-rule': {'ssrule': 'womersley', 'M': 6},
-        'scattering-model': {'type': 'vhs-gll-fno', 'omega': 0.5, 'dRef': 4.17e-10, 'Tref': 273,
-                             'precision': 'double', 'surrogate-tol': tol, 'surrogate-weights': weights,
-                             'surrogate-layers': layers}})
+GRIDS = ('vhs16', 'vss32')
+
+
+def config(tol=1e-4, weights='', layers=4, grid='vhs16', project=1, nrho=None):
+    """The stand-in plugin's configuration: vhs-gll-fno on the 16^3 grid
+    (hard spheres, M = 6, Nrho = 8) or vss-gll-fno on the 32^3 grid (VSS
+    omega = 0.74, alpha = 1.4, M = 12, Nrho = 16, Mpre = 32); the spectral
+    operator in FP64, with the conservation projection unless project = 0."""
+    if grid == 'vhs16':
+        return Inifile.from_dict({
+            'non-dim': ARGON,
+            'constants': {'Nv': 16, 'NvBatchSize': 16 ** 3, 'Nrho': nrho or 8},
+            'velocity-mesh': {'dev': 5.0, 'cmax': 0, 'Tmax': 273},
+            'spherical-design-rule': {'ssrule': 'womersley', 'M': 6},
+            'scattering-model': {'type': 'vhs-gll-fno', 'omega': 0.5, 'dRef': 4.17e-10, 'Tref': 273,
+                                 'precision': 'double', 'conservation-projection': project,
+                                 'surrogate-tol': tol, 'surrogate-weights': weights,
+                                 'surrogate-layers': layers}})
+    if grid == 'vss32':
+        return Inifile.from_dict({
+            'non-dim': ARGON,
+            'constants': {'Nv': 32, 'NvBatchSize': 32 ** 3, 'Nrho': nrho or 16},
+            'velocity-mesh': {'dev': 6.0, 'cmax': 0, 'Tmax': 273, 'molarMass1': ARGON['molarMass0']},
+            'spherical-design-rule': {'ssrule': 'womersley', 'M': 12},
+            'scattering-model': {'type': 'vss-gll-fno', 'ssrulepre': 'womersley', 'Mpre': 32,
+                                 'alpha00': 1.4, 'omega00': 0.74, 'dRef00': 4.17e-10, 'Tref00': 273,
+                                 'precision': 'double', 'conservation-projection': project,
+                                 'surrogate-tol': tol, 'surrogate-weights': weights,
+                                 'surrogate-layers': layers}})
+    raise ValueError(f'grid must be one of {GRIDS}')
+
+
+def model(backend, cfg):
+    """The stand-in plugin of a configuration from :func:`config`."""
+    from .surrogate import DGFSBiVSSGLLFNOScatteringModel, DGFSVHSGLLFNOScatteringModel
+    from .velocitymesh import DGFSBiVelocityMesh, DGFSVelocityMesh
+    if cfg.get('scattering-model', 'type') == 'vss-gll-fno':
+        vm = DGFSBiVelocityMesh(backend, cfg, 1)
+        return DGFSBiVSSGLLFNOScatteringModel(backend, cfg, vm)
+    vm = DGFSVelocityMesh(backend, cfg)
+    return DGFSVHSGLLFNOScatteringModel(backend, cfg, vm)
 
 
 def make_pairs(sm, ndof, seed, chunk=8192):
@@ -54,16 +92,13 @@ def rel_l2_rows(p, t):
 
 
 def train(backend, *, pairs=200000, heldout=2048, epochs=20, batch=64, lr=2e-3, seed=0, time_limit=None,
-          layers=4, train_acts='fp32', log=print):
+          layers=4, train_acts='fp32', log=print, grid='vhs16', project=1, nrho=None):
     """Returns (model, report).  The model's network is trained in place;
     train_acts is the activation storage of training ('fp32' or 'bf16':
     BF16 storage with FP32 arithmetic, gradients and weights)."""
     import torch
-    from .surrogate import DGFSVHSGLLFNOScatteringModel
-    from .velocitymesh import DGFSVelocityMesh
-    cfg = config(layers=layers)
-    vm = DGFSVelocityMesh(backend, cfg)
-    sm = DGFSVHSGLLFNOScatteringModel(backend, cfg, vm)
+    cfg = config(layers=layers, grid=grid, project=project, nrho=nrho)
+    sm = model(backend, cfg)
     sm.net.train_acts = train_acts
     t0 = time.time()
     g, t, _ = make_pairs(sm, pairs + heldout, seed)
@@ -79,7 +114,8 @@ def train(backend, *, pairs=200000, heldout=2048, epochs=20, batch=64, lr=2e-3, 
     torch.cuda.synchronize()
     t_fit = time.time() - t0
     r = rel_l2_rows(sm.net.predict(g[pairs:]), t[pairs:])
-    rep = {'pairs': pairs, 'heldout': heldout, 'epochs_run': len(hist), 'batch': batch, 'lr': lr,
+    rep = {'grid': grid, 'targets': 'FP64 operator' + (' with conservation projection' if project else ''),
+           'pairs': pairs, 'heldout': heldout, 'epochs_run': len(hist), 'batch': batch, 'lr': lr,
            'layers': layers, 'train_acts': train_acts, 'generate_s': round(t_gen, 2),
            'train_s': round(t_fit, 2),
            'train_rel_sq_err': hist[-1] if hist else None,
@@ -92,6 +128,8 @@ def train(backend, *, pairs=200000, heldout=2048, epochs=20, batch=64, lr=2e-3, 
 
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument('--grid', default='vhs16', choices=GRIDS)
+    ap.add_argument('--no-projection', action='store_true', help='train on unprojected FP64 targets')
     ap.add_argument('--pairs', type=int, default=200000)
     ap.add_argument('--heldout', type=int, default=2048)
     ap.add_argument('--epochs', type=int, default=20)
@@ -107,7 +145,7 @@ def main():
     be = HIPBackend()
     sm, rep = train(be, pairs=a.pairs, heldout=a.heldout, epochs=a.epochs, batch=a.batch, lr=a.lr,
                     seed=a.seed, time_limit=a.time_limit, layers=a.layers, train_acts=a.train_acts,
-                    log=lambda s: print(s, flush=True))
+                    log=lambda s: print(s, flush=True), grid=a.grid, project=0 if a.no_projection else 1)
     sm.net.save(a.out)
     print(json.dumps(rep), flush=True)
 

@@ -6,18 +6,21 @@
  * The reference has no learned operator; the stand-in takes the place of
  * the same plugin call as the spectral operator,
  *   DGFSVHSGLLScatteringModel.fs (frfs/solvers/dgfs/scattering.py:227-286)
+ *   DGFSBiVSSGLLScatteringModel.fs, one species
+ *       (frfs/solvers/dgfsbi/scattering.py:323-330)
  * batched like fq_eval, and reads and writes the same AoSoA matrices
  * (frfs/backends/base/types.py:26-47; dof = elem*nupts + upt).  DOFs whose
  * conservation defect exceeds the tolerance are flagged by fq_sur_finish and
  * left for the spectral operator (fq_eval) to fill.
  *
- * Network (16^3 grid, width 32, 8^3 retained modes, 1..8 layers):
+ * Network (N^3 grid with N = 16 or 32, width 32, 8^3 retained modes,
+ * 1..8 layers):
  *   v_0 = P g + p0;  z_l = Wm_l v_l + b_l + Re IDFT(R_l . DFT(v_l)/n);
  *   v_{l+1} = gelu(z_l) (last layer: z_l);  out = Qw . v_L + q0,
  * with g = f / (rho f_ref) and out ~ Q / (rho^2 q_ref) (Q is quadratic in f).
  * theta layout (floats): P (W), p0 (W), per layer [R (W*W*m^3 complex,
  * (i, o, k) order, interleaved), Wm (W*W, (o, i)), b (W)], Qw (W), q0.
- * Activation arrays are [batch][channel][x] FP32, x = (I*16 + J)*16 + K.
+ * Activation arrays are [batch][channel][x] FP32, x = (I*N + J)*N + K.
  *
  * All buffers are caller-owned device memory; nothing here allocates.
  * Errors: FQ_SUCCESS or a negative FQ_ERR_* code with fq_last_error().
@@ -43,7 +46,7 @@ extern "C" {
 
 typedef struct fq_sur_shape {
     int32_t abi_version;   /* FQ_SUR_ABI_VERSION */
-    int32_t Nv;            /* grid points per axis: 16 */
+    int32_t Nv;            /* grid points per axis: 16 or 32 */
     int32_t width;         /* channels: 32 */
     int32_t modes;         /* retained modes per axis: 8 */
     int32_t layers;        /* Fourier layers: 1..FQ_SUR_MAX_LAYERS */
@@ -55,7 +58,7 @@ typedef struct fq_sur_shape {
 int fq_sur_sizes(const fq_sur_shape *shape, int64_t batch, int32_t training, int64_t *nparams,
                  int64_t *act_size, int64_t *work_size);
 
-/* out[batch][16^3] = network(g[batch][16^3]).  acts: fq_sur_sizes floats.
+/* out[batch][N^3] = network(g[batch][N^3]).  acts: fq_sur_sizes floats.
  * acts_type (FQ_SUR_ACTS_*) is the storage of the activations; arithmetic
  * and the gradients stay FP32 (BF16 storage with FP32 accumulation for
  * training, for example).  A training call and the fq_sur_backward call that

@@ -142,8 +142,16 @@ def test_sur_abi_rejects_bad_shapes_and_arguments():
     vol, md = 2 * 32 * 4096, 2 * 2 * 32 * 512
     assert act.value == 5 * vol + 4 * vol + 4 * md + vol + md
     assert work.value == 2 * vol + 2 * md
+    # the 32^3 grid: the same parameters, activations sized by its 32768 points
+    good32 = _lib.fq_sur_shape(_lib.SUR_ABI_VERSION, 32, 32, 8, 4)
+    n32 = C.c_int64()
+    assert lib.fq_sur_sizes(C.byref(good32), 2, 1, C.byref(n32), C.byref(act), C.byref(work)) == _lib.SUCCESS
+    vol32 = 2 * 32 * 32768
+    assert n32.value == n.value
+    assert act.value == 5 * vol32 + 4 * vol32 + 4 * md + vol32 + md
+    assert work.value == 2 * vol32 + 2 * md
     for bad, code in ((_lib.fq_sur_shape(99, 16, 32, 8, 4), _lib.ERR_INVALID),
-                      (_lib.fq_sur_shape(_lib.SUR_ABI_VERSION, 32, 32, 8, 4), _lib.ERR_UNSUPPORTED),
+                      (_lib.fq_sur_shape(_lib.SUR_ABI_VERSION, 24, 32, 8, 4), _lib.ERR_UNSUPPORTED),
                       (_lib.fq_sur_shape(_lib.SUR_ABI_VERSION, 16, 16, 8, 4), _lib.ERR_UNSUPPORTED),
                       (_lib.fq_sur_shape(_lib.SUR_ABI_VERSION, 16, 32, 8, 9), _lib.ERR_UNSUPPORTED)):
         assert lib.fq_sur_sizes(C.byref(bad), 1, 0, C.byref(n), None, None) == code
