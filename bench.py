@@ -454,11 +454,12 @@ def bench_fno(a):
     rm = from_aosoa(qm)
     rmix = float(((rm - qb).pow(2).sum(1) / qb.pow(2).sum(1)).sqrt().median())
     # untimed: the network (conservation-projected) on every DOF, no fallback
-    tol = sm.tol
-    sm.tol = 2.0
+    # and no sampled check
+    tol, every = sm.tol, sm.check_every
+    sm.tol, sm.check_every = 2.0, 0
     qn = be.matrix(fin.ioshape)
     sm.fs_batch(fin, qn)
-    sm.tol = tol
+    sm.tol, sm.check_every = tol, every
     rn = ((from_aosoa(qn) - qb).pow(2).sum(1) / qb.pow(2).sum(1)).sqrt().cpu().numpy()
     dfc = stats['defect']
     # the teacher's own conservation defect, and its projection, through the

@@ -28,6 +28,7 @@ FLAG_UNFUSED = 2
 FLAG_TWO_LANE = 4
 FLAG_NO_PACK = 8
 FLAG_NO_SCRATCH = 16
+FLAG_ACCUM_QUAD = 32
 NOALLOC_SCRATCH_BYTES = 256 << 20
 
 _dp = C.POINTER(C.c_double)

@@ -1,11 +1,13 @@
 """Schedule autotuner for the collision operator.
 
-A schedule is the triple (product stage, pencil layout, DOFs per internal
-batch): the fused product kernel or the staged path that writes the inverse z
-transforms to HBM; for FP32 transforms on the 24^3 and 32^3 grids, one lane
-per pencil (in-register radix-4/8 transforms) or two (radix-2 exchange
-between lanes); and the largest batch the scratch budget allows or a smaller
-one.
+A schedule is (product stage, pencil layout, pair packing, accumulation tile,
+DOFs per internal batch): the fused product kernel or the staged path that
+writes the inverse z transforms to HBM; for FP32 transforms on the 24^3 and
+32^3 grids, one lane per pencil (in-register radix-4/8 transforms) or two
+(radix-2 exchange between lanes), and the two items of a product transformed
+as one packed volume or separately; on the 32^3 grid, eight or four DOFs per
+accumulation workgroup sharing each weight row; and the batch the scratch
+budget allows, a larger or a smaller one.
 ``tune`` builds one handle per candidate on the current device, times each on
 the caller's own inputs (one untimed call, then ``reps`` timed calls between
 device synchronisations) and releases it before the next is built, so at most
@@ -21,7 +23,10 @@ from . import _lib
 
 CANDIDATES = (
     {'fused': 1, 'max_chunk': 0, 'two_lane': 0},
-    {'fused': 1, 'max_chunk': 512, 'two_lane': 0},
+    {'fused': 1, 'max_chunk': 1024, 'two_lane': 0},
+    {'fused': 1, 'max_chunk': 128, 'two_lane': 0},
+    {'fused': 1, 'max_chunk': 0, 'two_lane': 0, 'accum_quad': 1},
+    {'fused': 1, 'max_chunk': 0, 'two_lane': 0, 'no_pack': 1},
     {'fused': 1, 'max_chunk': 0, 'two_lane': 1},
     {'fused': 0, 'max_chunk': 0, 'two_lane': 0},
 )
@@ -44,11 +49,15 @@ def _bounded(cand, limit):
 
 def _apply(params, cand):
     c = params.c
-    fl = c.flags & ~(_lib.FLAG_UNFUSED | _lib.FLAG_TWO_LANE)
+    fl = c.flags & ~(_lib.FLAG_UNFUSED | _lib.FLAG_TWO_LANE | _lib.FLAG_NO_PACK | _lib.FLAG_ACCUM_QUAD)
     if not cand['fused']:
         fl |= _lib.FLAG_UNFUSED
     if cand.get('two_lane'):
         fl |= _lib.FLAG_TWO_LANE
+    if cand.get('no_pack'):
+        fl |= _lib.FLAG_NO_PACK
+    if cand.get('accum_quad'):
+        fl |= _lib.FLAG_ACCUM_QUAD
     c.flags = fl
     c.max_chunk = cand['max_chunk']
 

@@ -2045,11 +2045,13 @@ template <int N, typename T> int run_chunk(fq_handle *h, const Scratch &sc, cons
     u = int64_t(D) * ntgt * N;
     pre();
     if constexpr (N == 32) {
-        // FP32 pencils: eight DOF slots per workgroup (one per CU); FP64: four
-        // (three workgroups per CU, register-limited)
-        constexpr int NS = sizeof(T) == 4 ? 8 : 4;
-        hipLaunchKernelGGL((k_accum_lds<N, T, NS>), dim3(unsigned(int64_t((D + NS - 1) / NS) * ntgt * N)),
-                           dim3(64 * NS), 0, st,
+        // FP32 pencils: eight DOF slots per workgroup (one per CU) unless
+        // FQ_FLAG_ACCUM_QUAD asks for four (two per CU); FP64: four (three
+        // workgroups per CU, register-limited)
+        const bool eight = sizeof(T) == 4 && !(h->flags & FQ_FLAG_ACCUM_QUAD);
+        auto kacc = eight ? k_accum_lds<N, T, 8> : k_accum_lds<N, T, 4>;
+        const int NS = eight ? 8 : 4;
+        hipLaunchKernelGGL(kacc, dim3(unsigned(int64_t((D + NS - 1) / NS) * ntgt * N)), dim3(64 * NS), 0, st,
                            Pg, sc.F, h->W, h->c_group, h->t_c0, h->t_cn, h->l_src, h->t_l0, h->t_ln,
                            int(P.contribs.size()), Y, ngroups, nsrc, ntgt, D);
     } else {
@@ -2105,7 +2107,8 @@ int eval_impl(fq_handle *h, const Scratch &sc, int64_t chunk, bool prof, const A
     // of the device's workgroup slots when the scratch holds at least that
     // many: the last batch is then the short one
     if (h->P.N == 32 && h->fused && nbatch > 1) {
-        // FP32: 8 slots per workgroup, one per CU; FP64: 4 slots, three per CU
+        // FP32: 8 slots per workgroup and one workgroup per CU, or 4 slots and
+        // two; FP64: 4 slots, three workgroups per CU
         const int64_t align = int64_t(h->cus) * (h->precision == FQ_PREC_MIXED ? 8 : 12) / 32;
         if (align > 0 && chunk >= align) {
             const int64_t pmax = chunk / align * align;

@@ -74,8 +74,13 @@ enum fq_flags {
                                  items of a product separately instead of as one
                                  packed complex volume with Nyquist-plane
                                  corrections (for comparison) */
-    FQ_FLAG_NO_SCRATCH = 16   /* fq_create allocates the tables only, no scratch:
+    FQ_FLAG_NO_SCRATCH = 16,  /* fq_create allocates the tables only, no scratch:
                                  the handle evaluates through fq_eval_noalloc */
+    FQ_FLAG_ACCUM_QUAD = 32   /* 32^3 grid, FP32 transforms: the accumulation
+                                 kernel shares each weight row over four DOFs per
+                                 workgroup (two workgroups per CU) instead of
+                                 eight (one per CU); a schedule candidate of the
+                                 autotuner */
 };
 
 /* Size of the static device scratch of fq_eval_noalloc (one copy per GPU the

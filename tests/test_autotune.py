@@ -10,7 +10,7 @@ class _FakeHandle:
 
     def __init__(self, params):
         fused = not (params.c.flags & 2)
-        if params.c.max_chunk == 512 and fused:
+        if params.c.max_chunk == 1024 and fused:
             from frfs_amd._lib import ERR_ALLOC, FQError
             raise FQError(ERR_ALLOC, 'no memory')
         self.cost = 0.001 if not fused else (0.003 if params.c.flags & 4 else 0.005)
@@ -42,7 +42,8 @@ def test_tune_picks_fastest_and_caches(monkeypatch):
     assert rep['choice'] == {'fused': 0, 'max_chunk': 0, 'two_lane': 0}
     assert not h.closed
     # every timed handle is released before the next is built; the winner is rebuilt
-    assert [x.closed for x in _FakeHandle.made] == [True, True, True, False]
+    ntimed = len(autotune.CANDIDATES) - 1                           # one fails to build
+    assert [x.closed for x in _FakeHandle.made] == [True] * ntimed + [False]
     assert _FakeHandle.made[-1] is h
     assert any('error' in t for t in rep['timings'])                 # failed candidate reported
     assert p.c.flags == 1 | 2 and p.c.max_chunk == 0                 # params keep the choice
@@ -81,8 +82,9 @@ def test_user_max_chunk_bounds_every_candidate(monkeypatch):
     p = _P()
     p.c.max_chunk = 256
     h, rep = autotune.tune(p, lambda h: None, key=('bounded',), reps=1)
-    assert all(mc == 256 for _, mc in seen)
-    # fused/auto and fused/512 collapse into one candidate under the bound
+    assert all(mc == min(256, c['max_chunk'] or 256)
+               for (_, mc), c in zip(seen, [c for c in autotune.CANDIDATES if c['max_chunk'] != 1024]))
+    # fused/auto and fused/1024 collapse into one candidate under the bound
     assert len(rep['timings']) == len(autotune.CANDIDATES) - 1
     assert p.c.max_chunk == 256
     autotune.clear_cache()
