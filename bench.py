@@ -73,6 +73,8 @@ def parse():
     ap.add_argument('--fno-train-acts', default='fp32', choices=['fp32', 'fp16', 'bf16'],
                     help='fno16/32: activation storage while training')
     ap.add_argument('--fno-dofs', type=int, default=8192)
+    ap.add_argument('--fno-cons-weight', type=float, default=10.0,
+                    help='fno16/32: weight of the conservation penalty in training')
     ap.add_argument('--fno-batch', type=int, default=64, help='training batch (DOFs)')
     ap.add_argument('--calib-copy-mib', type=int, default=0,
                     help='run one device-to-device copy of this size first (counter calibration)')
@@ -355,6 +357,7 @@ def bench_fno(a):
         pairs = a.fno_pairs or (50000 if grid == 'vss32' else 200000)
         sm, rep = ts.train(be, pairs=pairs, epochs=a.fno_epochs, time_limit=a.fno_train_seconds,
                            seed=a.seed, train_acts=a.fno_train_acts, grid=grid, batch=a.fno_batch,
+                           cons_weight=a.fno_cons_weight,
                            log=lambda m: print(m, file=sys.stderr, flush=True))
         sm.tol = a.fno_tol
         vm = sm.vm

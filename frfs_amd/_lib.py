@@ -106,6 +106,7 @@ SIGNATURES = {
     'fq_sur_forward_fused': (C.c_int, [_vp, _vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp]),
     'fq_sur_backward': (C.c_int, [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
     'fq_sur_loss': (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
+    'fq_sur_cons': (C.c_int, [_i32, C.c_double, C.c_double, C.c_double, _i64, _vp, _vp, _vp, _vp, _vp]),
     'fq_sur_adam': (C.c_int, [_i64, _vp, _vp, _vp, _vp, C.c_float, C.c_float, C.c_float, C.c_float,
                               _i32, _vp]),
     'fq_sur_gather': (C.c_int, [_i32, C.c_double, C.c_double, _i32, _vp, _i32, _i64, _i32, _i64, _i64,

@@ -828,6 +828,49 @@ __global__ __launch_bounds__(256) void k_sur_finish(int N, double c0, double dc,
     }
 }
 
+// Conservation penalty of a training batch (one workgroup per DOF b): the
+// invariant component u_b = Phi^T G^{-1} Phi out_b cw of the network output,
+// the orthogonal projection of out_b onto span{1, cx, cy, cz, |c|^2} (G the
+// Gram matrix Phi Phi^T cw); gout_b += (2 lambda / count) u_b and sums[2] +=
+// |u_b|^2, the gradient and value of lambda |u|^2 / count added to the mean
+// square loss.  The targets conserve (they are projected), so this weights
+// the error in the invariant directions lambda + 1 times.
+__global__ __launch_bounds__(256) void k_sur_cons(int N, double c0, double dc, double cw, SurGram gm, double coef,
+                                                  const float *__restrict__ out, float *__restrict__ gout,
+                                                  double *__restrict__ sums) {
+    __shared__ double red[256];
+    const int n = N * N * N;
+    const int64_t b = blockIdx.x;
+    const float *ob = out + b * n;
+    float *gb = gout + b * n;
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int x = threadIdx.x; x < n; x += 256) {
+        const double cx = c0 + dc * (x / (N * N)), cy = c0 + dc * ((x / N) % N), cz = c0 + dc * (x % N);
+        const double v = ob[x];
+        acc[0] += v;
+        acc[1] += v * cx;
+        acc[2] += v * cy;
+        acc[3] += v * cz;
+        acc[4] += v * (cx * cx + cy * cy + cz * cz);
+    }
+    double mom[5], lam[5];
+    for (int j = 0; j < 5; ++j) mom[j] = block_sum(acc[j], red) * cw;
+    for (int j = 0; j < 5; ++j) {
+        double t = 0;
+        for (int k = 0; k < 5; ++k) t += gm.ginv[j * 5 + k] * mom[k];
+        lam[j] = t;
+    }
+    double uu = 0;
+    for (int x = threadIdx.x; x < n; x += 256) {
+        const double cx = c0 + dc * (x / (N * N)), cy = c0 + dc * ((x / N) % N), cz = c0 + dc * (x % N);
+        const double u = lam[0] + lam[1] * cx + lam[2] * cy + lam[3] * cz + lam[4] * (cx * cx + cy * cy + cz * cz);
+        gb[x] += float(coef * u);
+        uu += u * u;
+    }
+    const double tot = block_sum(uu, red);
+    if (threadIdx.x == 0) atomicAdd(&sums[2], tot);
+}
+
 // ---------------------------------------------------------------------------
 // Inference with the lift folded into the first layer and the projection into
 // the last (layers >= 2).  With Ghat = DFT(g)/n (one channel):
@@ -1222,6 +1265,21 @@ int fq_sur_loss(int64_t count, const float *out, const float *target, float *gou
     hipLaunchKernelGGL(k_sur_loss, dim3(nblk(count)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), out,
                        target, gout, sums, count);
     return hip_status("fq_sur_loss");
+}
+
+int fq_sur_cons(int32_t Nv, double L, double cw, double lambda, int64_t batch, const float *out,
+                const double *gram_inv, float *gout, double *sums, void *stream) {
+    if ((Nv != 16 && Nv != 32) || !(L > 0) || !(cw > 0) || !(lambda >= 0) || batch < 0 || batch > FQ_SUR_MAX_BATCH ||
+        !out || !gram_inv || !gout || !sums)
+        return fq_internal_fail(FQ_ERR_INVALID, "fq_sur_cons: invalid arguments");
+    if (batch == 0 || lambda == 0) return FQ_SUCCESS;
+    SurGram gm;
+    for (int j = 0; j < 25; ++j) gm.ginv[j] = gram_inv[j];
+    for (int j = 0; j < 5; ++j) gm.gdiag[j] = 0;
+    const double count = double(batch) * Nv * Nv * Nv;
+    hipLaunchKernelGGL(k_sur_cons, dim3(unsigned(batch)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), Nv,
+                       -L + L / Nv, 2 * L / Nv, cw, gm, 2.0 * lambda / count, out, gout, sums);
+    return hip_status("fq_sur_cons");
 }
 
 int fq_sur_adam(int64_t nparams, float *theta, const float *grad, float *m, float *v, float lr, float beta1,

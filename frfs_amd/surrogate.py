@@ -98,6 +98,10 @@ class FourierSurrogate:
         self._ver, self._fused_ver = 0, -1
         self.acts = 'fp32'           # activation storage of inference: fp32, fp16 or bf16
         self.train_acts = 'fp32'     # activation storage of training (arithmetic and gradients FP32)
+        # conservation penalty of training: weight lambda and (L, cw, device inverse Gram
+        # matrix of the invariants) of the grid; lambda = 0 leaves the plain mean-square loss
+        self.cons_weight = 0.0
+        self.cons = None
 
     def invalidate(self):
         """Call after changing theta in place (train_step does it itself)."""
@@ -161,15 +165,22 @@ class FourierSurrogate:
                                                w.data_ptr(), self._stream()))
 
     def gradient(self, g, target):
-        """(loss sums [sum d^2, sum t^2], grad) of the mean-square loss."""
+        """(loss sums [sum d^2, sum t^2, sum |u|^2], grad) of the mean-square
+        loss plus, with cons_weight > 0, the conservation penalty
+        cons_weight |u|^2 / count of the output's invariant component u."""
         torch = self.torch
         out = torch.empty_like(g)
         acts = self.forward(g, out, training=True)
         gout = torch.empty_like(g)
-        sums = torch.zeros(2, dtype=torch.float64, device=self.dev)
+        sums = torch.zeros(3, dtype=torch.float64, device=self.dev)
         grad = torch.zeros(self.nparams, dtype=torch.float32, device=self.dev)
-        _lib.check(_lib.load().fq_sur_loss(g.numel(), out.data_ptr(), target.data_ptr(), gout.data_ptr(),
-                                           sums.data_ptr(), self._stream()))
+        lib = _lib.load()
+        _lib.check(lib.fq_sur_loss(g.numel(), out.data_ptr(), target.data_ptr(), gout.data_ptr(),
+                                   sums.data_ptr(), self._stream()))
+        if self.cons_weight > 0 and self.cons is not None:
+            L, cw, ginv = self.cons
+            _lib.check(lib.fq_sur_cons(self.nv, L, cw, self.cons_weight, g.shape[0], out.data_ptr(),
+                                       ginv.data_ptr(), gout.data_ptr(), sums.data_ptr(), self._stream()))
         self.backward(g, acts, gout, grad)
         return sums, grad
 
@@ -200,7 +211,7 @@ class FourierSurrogate:
         hist, step, t0 = [], 0, time.time()
         for ep in range(epochs):
             perm = torch.randperm(n, generator=gen).to(self.dev)
-            acc = torch.zeros(2, dtype=torch.float64, device=self.dev)
+            acc = torch.zeros(3, dtype=torch.float64, device=self.dev)
             for j in range(nb):
                 idx = perm[j * batch:(j + 1) * batch]
                 a = lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(math.pi * step / total))

@@ -92,14 +92,18 @@ def rel_l2_rows(p, t):
 
 
 def train(backend, *, pairs=200000, heldout=2048, epochs=20, batch=64, lr=2e-3, seed=0, time_limit=None,
-          layers=4, train_acts='fp32', log=print, grid='vhs16', project=1, nrho=None):
+          layers=4, train_acts='fp32', log=print, grid='vhs16', project=1, nrho=None, cons_weight=10.0):
     """Returns (model, report).  The model's network is trained in place;
     train_acts is the activation storage of training ('fp32' or 'bf16':
-    BF16 storage with FP32 arithmetic, gradients and weights)."""
+    BF16 storage with FP32 arithmetic, gradients and weights); cons_weight
+    is the weight of the conservation penalty (the output's component along
+    the collision invariants, which the projected targets do not have)."""
     import torch
     cfg = config(layers=layers, grid=grid, project=project, nrho=nrho)
     sm = model(backend, cfg)
     sm.net.train_acts = train_acts
+    sm.net.cons_weight = cons_weight
+    sm.net.cons = (sm.vm.L(), sm.vm.cw(), sm._ginv)
     t0 = time.time()
     g, t, _ = make_pairs(sm, pairs + heldout, seed)
     torch.cuda.synchronize()
@@ -115,6 +119,7 @@ def train(backend, *, pairs=200000, heldout=2048, epochs=20, batch=64, lr=2e-3, 
     t_fit = time.time() - t0
     r = rel_l2_rows(sm.net.predict(g[pairs:]), t[pairs:])
     rep = {'grid': grid, 'targets': 'FP64 operator' + (' with conservation projection' if project else ''),
+           'conservation_penalty_weight': cons_weight,
            'pairs': pairs, 'heldout': heldout, 'epochs_run': len(hist), 'batch': batch, 'lr': lr,
            'layers': layers, 'train_acts': train_acts, 'generate_s': round(t_gen, 2),
            'train_s': round(t_fit, 2),
@@ -130,6 +135,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--grid', default='vhs16', choices=GRIDS)
     ap.add_argument('--no-projection', action='store_true', help='train on unprojected FP64 targets')
+    ap.add_argument('--cons-weight', type=float, default=10.0,
+                    help='weight of the conservation penalty (0: plain mean-square loss)')
     ap.add_argument('--pairs', type=int, default=200000)
     ap.add_argument('--heldout', type=int, default=2048)
     ap.add_argument('--epochs', type=int, default=20)
@@ -145,7 +152,8 @@ def main():
     be = HIPBackend()
     sm, rep = train(be, pairs=a.pairs, heldout=a.heldout, epochs=a.epochs, batch=a.batch, lr=a.lr,
                     seed=a.seed, time_limit=a.time_limit, layers=a.layers, train_acts=a.train_acts,
-                    log=lambda s: print(s, flush=True), grid=a.grid, project=0 if a.no_projection else 1)
+                    log=lambda s: print(s, flush=True), grid=a.grid, project=0 if a.no_projection else 1,
+                    cons_weight=a.cons_weight)
     sm.net.save(a.out)
     print(json.dumps(rep), flush=True)
 

@@ -90,6 +90,16 @@ int fq_sur_backward(const fq_sur_shape *shape, const float *theta, const float *
 int fq_sur_loss(int64_t count, const float *out, const float *target, float *gout, double *sums,
                 void *stream);
 
+/* Conservation penalty added to the loss of fq_sur_loss (training only; no
+ * reference counterpart): for each of the batch rows of out (the network
+ * output on the Nv^3 grid on [-L, L]^3, cell volume cw), u = its orthogonal
+ * projection onto span{1, cx, cy, cz, |c|^2} (gram_inv = the inverse of
+ * that basis' Gram matrix in the cw inner product, 5 x 5 row major);
+ * gout += 2 lambda u / count and sums[2] += |u|^2 (count = batch Nv^3), the
+ * gradient and value of lambda |u|^2 / count.  sums holds three doubles. */
+int fq_sur_cons(int32_t Nv, double L, double cw, double lambda, int64_t batch, const float *out,
+                const double *gram_inv, float *gout, double *sums, void *stream);
+
 /* One Adam step (bias-corrected, step >= 1) over the whole parameter vector. */
 int fq_sur_adam(int64_t nparams, float *theta, const float *grad, float *m, float *v, float lr,
                 float beta1, float beta2, float eps, int32_t step, void *stream);

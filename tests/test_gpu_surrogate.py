@@ -364,3 +364,40 @@ def test_16bit_activation_storage(backend, acts, tol):
     for b in range(3):
         assert 0 < rel_l2(o16[b], o32[b]) <= tol, b
         assert rel_l2(o16[b], want[b]) <= 1.5 * tol, b
+
+
+@pytest.mark.parametrize('NG', [16, 32])
+def test_conservation_penalty_gradient(backend, NG):
+    """fq_sur_cons adds 2 lambda u / count to the loss gradient, u the
+    orthogonal projection of each output row onto the collision invariants
+    {1, cx, cy, cz, |c|^2} of the grid (computed here in float64 NumPy by least
+    squares), and sum |u|^2 to sums[2]."""
+    import ctypes as C
+    import torch
+    from frfs_amd import _lib
+    from frfs_amd.surrogate import gram
+    L0, B, lam = 5.0, 3, 7.5
+    c0 = np.linspace(-L0 + L0 / NG, L0 - L0 / NG, NG)
+    idx = np.arange(NG ** 3)
+    cv = np.stack([c0[idx // (NG * NG)], c0[(idx // NG) % NG], c0[idx % NG]])
+    cw = (2 * L0 / NG) ** 3
+    gi, _ = gram(cv, cw)
+    rng = np.random.default_rng(NG)
+    out = rng.standard_normal((B, NG ** 3)).astype(np.float32)
+    # gout holds an earlier gradient: a small one, so that the FP32 sum keeps
+    # the increment's digits (the increment is ~1e-4 of a unit-size row)
+    base = (1e-4 * rng.standard_normal((B, NG ** 3))).astype(np.float32)
+    phi = np.stack([np.ones(NG ** 3), cv[0], cv[1], cv[2], (cv ** 2).sum(axis=0)])
+    coef, *_ = np.linalg.lstsq(phi.T, out.astype(np.float64).T, rcond=None)
+    u = (phi.T @ coef).T                                     # (B, n) projections
+    od, gd = _dev(out), _dev(base)
+    sums = torch.zeros(3, dtype=torch.float64, device='cuda')
+    ginv = torch.tensor(gi.reshape(-1), dtype=torch.float64, device='cuda')
+    print(f'[conservation penalty {NG}^3]', flush=True)
+    _lib.check(_lib.load().fq_sur_cons(NG, L0, cw, lam, B, od.data_ptr(), ginv.data_ptr(), gd.data_ptr(),
+                                       sums.data_ptr(), None))
+    torch.cuda.synchronize()
+    want = base + 2 * lam / (B * NG ** 3) * u
+    got = gd.cpu().numpy()
+    assert rel_l2(got - base, want - base) <= 1e-5
+    assert abs(float(sums[2]) - float((u ** 2).sum())) <= 1e-6 * float((u ** 2).sum())
