@@ -28,14 +28,17 @@ sys.path.insert(0, ROOT)
 PEAK = {'fp64': 78.6, 'fp32': 157.3}     # MI355X vector TFLOP/s (published)
 HBM_PEAK = 8000.0                          # GB/s
 # HBM bytes per DOF of the dominant (product) kernel at the 32^3 workload
-# (N = 32, M = 12, Nrho = 16, Mpre = 32, 4096 DOFs), from rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE runs of this script, scaled by what the same
-# counters read for a 1 GiB device copy (FETCH x2.0, WRITE x1.0).  The bench
-# line carries this figure with its source, and only for that shape.
+# (N = 32, M = 12, Nrho = 16, Mpre = 32, 4096 DOFs in batches of 256 (mixed)
+# and 96 (FP64) DOFs), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
+# this script, scaled by what the same counters read for a 1 GiB device copy
+# (FETCH x2.0, WRITE x1.0).  The bench line carries this figure with its
+# source, and only for that shape.
 K_PROD_TRAFFIC = {
     'source': 'profiles/k_prod_vss32_traffic.txt',
     'shape': {'Nv': 32, 'M': 12, 'Nrho': 16, 'Mpre': 32},
-    'per_dof': {'mixed': 34.47e6, 'fp64': 84.67e6},
+    'per_dof': {'mixed': 27.95e6, 'fp64': 61.64e6},
+    # the accumulation kernel (HBM-bound), same runs
+    'accum_per_dof': {'mixed': 28.73e6, 'fp64': 62.12e6},
 }
 
 
@@ -724,6 +727,16 @@ def main():
         traffic = K_PROD_TRAFFIC['per_dof']['mixed' if head.startswith('mixed') else 'fp64'] * dofs_per_launch
         traffic_src = (K_PROD_TRAFFIC['source'] + ': FETCH_SIZE + WRITE_SIZE per dispatch of this kernel at '
                        'this shape, scaled by a 1 GiB copy, per DOF times the DOFs of one launch')
+    # live HBM rate of the accumulation kernel (the operator's HBM-bound kernel):
+    # its counter bytes per DOF x DOFs per launch / its HIP-event launch time
+    accum_hbm = None
+    if {'Nv': N, 'M': a.M, 'Nrho': a.Nrho, 'Mpre': a.Mpre} == K_PROD_TRAFFIC['shape'] and 'k_accum' in tot:
+        acc_ms, acc_n = tot['k_accum']
+        if acc_n:
+            per = K_PROD_TRAFFIC['accum_per_dof']['mixed' if head.startswith('mixed') else 'fp64']
+            gbps = per * (a.ndof * a.steps / acc_n) / (acc_ms / acc_n * 1e-3) / 1e9
+            accum_hbm = {'kernel': 'k_accum', 'GBps': gbps, 'hbm_frac': gbps / HBM_PEAK,
+                         'bytes_per_dof': per, 'source': K_PROD_TRAFFIC['source']}
     err = None
 
     def relerr(x, y):
@@ -814,6 +827,10 @@ def main():
                      'avg_launch_ms': dom_avg_s * 1e3, 'launches': dom_n,
                      'executed_tflops': executed,
                      'operator_effective_tflops': op_eff, 'operator_frac': op_eff / prec_peak,
+                     'accum_hbm': accum_hbm,
+                     'mfma_util': 0.0,
+                     'mfma_note': 'the transforms run on the vector ALU: the Q path issues no MFMA '
+                                  'instructions (dense DFT stages as GEMMs cost more than the radix form)',
                      'kernel_ms_per_step': {k: v[0] / a.steps for k, v in tot.items()}},
         'cpu_baseline': cpu,
     }
