@@ -62,6 +62,8 @@ def parse():
     ap.add_argument('--cpu-procs', type=int, default=0, help='0 = min(cpus, 16)')
     ap.add_argument('--cpu-runs', type=int, default=5, help='CPU baseline: median of this many runs')
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--autotune', type=int, default=1,
+                    help='1: run the schedule autotuner (untimed) before the warmup steps')
     ap.add_argument('--no-reference-proj', action='store_true',
                     help='skip the untimed FP64-with-projection reference evaluation')
     ap.add_argument('--modes', default='fp64,mixed,mixedproj')
@@ -224,6 +226,8 @@ def bench_multi(a):
             f = mixture_device(vm.cv(), fpar, mass=m) * (sh if s == 1 else 1 - sh)
             to_aosoa(f, fin[s])
         torch.cuda.synchronize()
+        # the schedule autotuner (untimed) before the warmup steps
+        sched = model.autotune(*fin, *qout)['choice'] if (timed and a.autotune) else None
         for _ in range(a.warmup if timed else 1):
             model.fs_batch(*fin, *qout)
         torch.cuda.synchronize()
@@ -233,7 +237,8 @@ def bench_multi(a):
                 model.fs_batch(*fin, *qout)
             torch.cuda.synchronize()
             dt = (time.perf_counter() - t0) / a.steps
-            res[mode] = dict(ms=dt * 1e3, rate=2 * ndof / dt, info=model.plan_info())
+            res[mode] = dict(ms=dt * 1e3, rate=2 * ndof / dt, info=model.plan_info(), schedule=sched,
+                             chunk=model._h.chunk_dofs(), scratch=model._h.scratch_bytes())
         qs[mode] = [from_aosoa(q).cpu().numpy() for q in qout]
         del model, fin, qout
         torch.cuda.empty_cache()
@@ -253,9 +258,12 @@ def bench_multi(a):
            'dtype': 'f32+f64acc' if head == 'mixed' else 'f64', 'data': 'synthetic',
            'config': {'workload': f'vss-gll two species, masses 1:2, {N}^3 grid on [-6,6]^3, M={a.M}, '
                                   f'Nrho={a.Nrho}, Mpre={a.Mpre}, {ndof} DOFs per species, species-1 '
-                                  f'density share U(0.2,0.8)', 'mode': head, 'ndof_per_species': ndof},
+                                  f'density share U(0.2,0.8)', 'mode': head, 'ndof_per_species': ndof,
+                      'schedule': res[head]['schedule'], 'chunk_dofs': res[head]['chunk'],
+                      'scratch_GiB': res[head]['scratch'] / 2 ** 30},
            'plan': res[head]['info'],
-           'fp64': ({'value': res['fp64']['rate'], 'ms_per_step': res['fp64']['ms']} if 'fp64' in res else None),
+           'fp64': ({'value': res['fp64']['rate'], 'ms_per_step': res['fp64']['ms'],
+                     'schedule': res['fp64']['schedule']} if 'fp64' in res else None),
            'speedup_mixed_vs_fp64': (res['mixed']['rate'] / res['fp64']['rate']) if len(res) == 2 else None,
            'rel_l2_per_species': err}
     print(json.dumps(out))
@@ -560,7 +568,8 @@ def bench_rhs(a):
             'spherical-design-rule': {'ssrule': 'womersley', 'M': 6},
             'scattering-model': sm_opts(a, {'type': 'vhs-gll', 'omega': 0.5, 'dRef': 4.17e-10, 'Tref': 273,
                                             'precision': 'mixed' if mode == 'mixed' else 'double',
-                                            'conservation-projection': 1 if mode == 'mixed' else 0})})
+                                            'conservation-projection': 1 if mode == 'mixed' else 0,
+                                            'autotune': a.autotune})})
         sysm = DGFSCollisionSystem(be, cfg, 9, nel * nel)
         adv = PeriodicQuadAdvection(be, sysm.vm, nel, nel, 1.0 / nel, 1.0 / nel, order=2)
         sysm.rhs = lambda t, uin, fout: adv(sysm.banks[uin], sysm.banks[fout])
@@ -660,6 +669,9 @@ def main():
             del sm, fin, qout
             torch.cuda.empty_cache()
             continue
+        # the schedule autotuner (untimed): candidates timed on these arrays,
+        # the fastest kept for the timed steps
+        sched = sm.autotune(fin, qout)['choice'] if a.autotune else None
         for _ in range(a.warmup):
             sm.fs_batch(fin, qout)
         torch.cuda.synchronize()
@@ -681,7 +693,7 @@ def main():
         info = sm.plan_info()
         res[mode] = dict(ms=dt * 1e3, rate=a.ndof / dt, prof=prof, info=info, step_ms=step_ms,
                          median_ms=float(np.median(step_ms)), chunk=sm._h.chunk_dofs(),
-                         scratch=sm._h.scratch_bytes())
+                         scratch=sm._h.scratch_bytes(), schedule=sched)
         qs[mode] = from_aosoa(qout).cpu().numpy()
         del sm, fin, qout
         torch.cuda.empty_cache()
@@ -726,7 +738,8 @@ def main():
     if (dom == 'k_prod' and {'Nv': N, 'M': a.M, 'Nrho': a.Nrho, 'Mpre': a.Mpre} == K_PROD_TRAFFIC['shape']):
         traffic = K_PROD_TRAFFIC['per_dof']['mixed' if head.startswith('mixed') else 'fp64'] * dofs_per_launch
         traffic_src = (K_PROD_TRAFFIC['source'] + ': FETCH_SIZE + WRITE_SIZE per dispatch of this kernel at '
-                       'this shape, scaled by a 1 GiB copy, per DOF times the DOFs of one launch')
+                       'this shape (counter runs with 256-DOF mixed and 96-DOF FP64 batches), scaled by a '
+                       '1 GiB copy, per DOF times the DOFs of one launch')
     # live HBM rate of the accumulation kernel (the operator's HBM-bound kernel):
     # its counter bytes per DOF x DOFs per launch / its HIP-event launch time
     accum_hbm = None
@@ -805,8 +818,11 @@ def main():
                                + ('; FP32 transforms and products, FP64 weight accumulation and FP64 '
                                   'conservation projection' if head == 'mixedproj' else ''),
                    'mode': head, 'ndof': a.ndof, 'Nv': N, 'M': a.M, 'Nrho': a.Nrho, 'Mpre': a.Mpre,
-                   'seed': a.seed, 'chunk_dofs': r['chunk'], 'scratch_GiB': r['scratch'] / 2 ** 30},
-        'fp64': {'value': res['fp64']['rate'], 'ms_per_step': res['fp64']['ms']} if 'fp64' in res else None,
+                   'seed': a.seed, 'chunk_dofs': r['chunk'], 'scratch_GiB': r['scratch'] / 2 ** 30,
+                   'schedule': r['schedule']},
+        'fp64': {'value': res['fp64']['rate'], 'ms_per_step': res['fp64']['ms'], 'chunk_dofs': res['fp64']['chunk'],
+                 'scratch_GiB': res['fp64']['scratch'] / 2 ** 30,
+                 'schedule': res['fp64']['schedule']} if 'fp64' in res else None,
         'speedup_vs_fp64': (r['rate'] / res['fp64']['rate']) if 'fp64' in res else None,
         'mixed_without_projection': {'value': res['mixed']['rate'], 'ms_per_step': res['mixed']['ms'],
                                      'speedup_vs_fp64': (res['mixed']['rate'] / res['fp64']['rate'])
