@@ -18,7 +18,7 @@ LIBPATH = os.environ.get('FRFS_FQ_LIB') or os.path.join(
 ABI_VERSION = 1
 
 SUCCESS, ERR_INVALID, ERR_UNSUPPORTED, ERR_HIP, ERR_ALLOC = 0, -1, -2, -3, -4
-NKERNELS = 9
+NKERNELS = 10
 AXNPBY_MAX = 4
 NMOMENTS = 14
 MODEL_VHS, MODEL_VHS_NODAL, MODEL_VSS = 0, 1, 2
@@ -29,6 +29,7 @@ FLAG_TWO_LANE = 4
 FLAG_NO_PACK = 8
 FLAG_NO_SCRATCH = 16
 FLAG_ACCUM_QUAD = 32
+FLAG_NO_HPAIR = 64
 NOALLOC_SCRATCH_BYTES = 256 << 20
 
 _dp = C.POINTER(C.c_double)

@@ -903,12 +903,29 @@ __device__ __forceinline__ void plane_inv1(cpx<T> *Lp, cpx<T> *v, int c, bool ac
 // gives S_1 = U.y and S_2 = -U.x, hence with R = E - U
 //   E_1 = (R.x, U.y),  E_2 = (R.y, -U.x).
 // This halves the z stage's transforms and the x,y transforms of the items.
-template <int N, typename T, bool MULTI, bool PACK, bool CONJ>
+//
+// Group pairs (HP, 32^3 FP32, single-product groups): only the Hermitian part
+// of the gain spectrum reaches Q (Q takes Re DFT+ of it), and with
+// G_g = a_g + i b_g (a, b real) that part is
+//   sum_c U_c DFT-(a_g) + V_c DFT-(b_g),
+//   U_c = (W_c + W~_c)/2, V_c = i (W_c - W~_c)/2, W~_c(xi) = conj W_c(-xi),
+// where V_c vanishes off the Nyquist set.  So the workgroup evaluates the
+// two groups 2u and 2u + 1 and transforms a_2u + i a_2u+1 once; the
+// accumulation separates the two Hermitian spectra (k_accum_h).  Of b_g it
+// keeps the sums that give DFT-(b_g) on the three Nyquist planes
+// (K' = N/2: sum over vz of (-1)^vz b; J' = N/2: over vy; I' = N/2: over vx),
+// written to NB as per-workgroup partials for k_nyqb:
+//   NB[slot][g] = { K: [G][vx][vy] (signed by (-1)^G), J: [vz][vx], I: [vz][vy] }.
+constexpr int kNbPer32 = 6 * 32 * 32;             // floats of NB per (slot, group), N = 32
+
+template <int N, typename T, bool MULTI, bool PACK, bool CONJ, bool HP = false>
 __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
     const cpx<T> *__restrict__ FT, cpx<T> *__restrict__ P, const cpx<T> *__restrict__ ph,
     const cpx<T> *__restrict__ zc, const DevProd *__restrict__ prods, const int *__restrict__ g_p0,
     const int *__restrict__ g_np, const int *__restrict__ item_src, int ngroups, int nsrc,
-    const cpx<T> *__restrict__ NY, int nprods, const cpx<T> *__restrict__ zq, const int *__restrict__ pconj) {
+    const cpx<T> *__restrict__ NY, int nprods, const cpx<T> *__restrict__ zq, const int *__restrict__ pconj,
+    float *__restrict__ NB) {
+    static_assert(!HP || (PACK && !MULTI && N == 32 && sizeof(T) == 4), "group pairs: 32^3 FP32 packed kernel");
     using PG = ProdGeo<N>;
     using P1 = Prod1Geo<N>;
     using G3 = Geo<N>;
@@ -917,20 +934,25 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
     // K piece [vy][vx], J piece [k][vx] and I piece [k][vy] for vz = G + 4k
     constexpr int NYL = PACK ? N * N + 2 * PG::S * N : 0;
     __shared__ cpx<T> lds[PG::S * PG::PLANE + NYL];
-    // XCD-contiguous order when the grid allows it: the 4 * ngroups
+    // XCD-contiguous order when the grid allows it: the 4 * nvol
     // workgroups of a DOF then share one L2 for its spectrum and corrections
     const int64_t unit = (gridDim.x & 7) == 0 ? xcd_block() : int64_t(blockIdx.x);
+    const int nvol = HP ? (ngroups + 1) / 2 : ngroups;    // volumes written per DOF
     const int G = int(unit % 4);
-    const int g = int((unit / 4) % ngroups);
-    const int slot = int(unit / (4 * int64_t(ngroups)));
+    const int gv = int((unit / 4) % nvol);
+    const int slot = int(unit / (4 * int64_t(nvol)));
     const int k = int(threadIdx.x) / 32;           // plane of this half-wave
     const int c = int(threadIdx.x) % 32;           // pencil (column, then row) of this lane
     const bool act = c < N;
     const int vz = G + 4 * k;
     cpx<T> *L1 = lds + k * PG::PLANE;
     cpx<T> acc[N];
+    T re0[HP ? N : 1];
+    for (int m = 0; m < (HP ? 2 : 1); ++m) {
+    const int g = HP ? 2 * gv + m : gv;
 #pragma unroll
     for (int e = 0; e < N; ++e) acc[e] = {0, 0};
+    if (HP && g >= ngroups) break;                 // odd group count: the pair's second member is zero
     const int p0 = g_p0[g], np = g_np[g];
     for (int pi = p0; pi < p0 + np; ++pi) {
         const DevProd pr = prods[pi];
@@ -1026,6 +1048,52 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
             for (int e = 0; e < N; ++e) acc[e] = fq::cscale(acc[e], cf);
         }
     }
+    if constexpr (HP) {
+        if (NB) {
+            // Nyquist-plane sums of b = Im G_g.  The plane region is free: its
+            // last reads (plane_inv1) precede the barrier before the unpack.
+            float *nb = NB + (int64_t(slot) * ngroups + g) * kNbPer32;
+            T sj = 0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) sj += (j & 1) ? -acc[j].y : acc[j].y;
+            if (act) nb[4 * N * N + vz * N + c] = sj;                   // J: (vz, vx = c)
+            T *bl = reinterpret_cast<T *>(lds);                        // [S][N][N + 1]
+            if (act) {
+#pragma unroll
+                for (int j = 0; j < N; ++j) bl[(k * N + c) * (N + 1) + j] = acc[j].y;
+            }
+            __syncthreads();
+            const int tid = int(threadIdx.x);
+            const T sgn = (G & 1) ? T(-1) : T(1);                      // (-1)^vz on vz = G + 4k
+#pragma unroll
+            for (int r = 0; r < N * N / P1::THREADS; ++r) {
+                const int idx = tid + r * P1::THREADS, x = idx / N, y = idx % N;
+                T s = 0;
+#pragma unroll
+                for (int kk = 0; kk < PG::S; ++kk) s += bl[(kk * N + x) * (N + 1) + y];
+                nb[G * N * N + idx] = sgn * s;                          // K: (vx, vy)
+            }
+            {
+                static_assert(P1::THREADS == PG::S * N, "one (plane, vy) per thread");
+                const int kk = tid / N, y = tid % N;
+                T s = 0;
+#pragma unroll
+                for (int x = 0; x < N; ++x) s += (x & 1) ? -bl[(kk * N + x) * (N + 1) + y] : bl[(kk * N + x) * (N + 1) + y];
+                nb[5 * N * N + (G + 4 * kk) * N + y] = s;               // I: (vz, vy)
+            }
+            __syncthreads();                                           // LDS reads done
+        }
+        if (m == 0) {
+#pragma unroll
+            for (int j = 0; j < N; ++j) re0[j] = acc[j].x;
+        }
+    }
+    }
+    if constexpr (HP) {
+        // a_2u + i a_2u+1 (acc is zero when the second member is absent)
+#pragma unroll
+        for (int j = 0; j < N; ++j) acc[j] = {re0[j], acc[j].x};
+    }
     // forward transforms of the group sum: lane c holds row vx = c over vy
     fq::LFFT<N, -1>::run(acc);                            // -> J'
     plane_sync<32>();
@@ -1037,7 +1105,7 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
 #pragma unroll
     for (int i = 0; i < N; ++i) acc[i] = act ? L1[i * PG::ROW + c] : cpx<T>{0, 0};   // column J' = c over vx
     fq::LFFT<N, -1>::run(acc);                            // -> I'
-    cpx<T> *dst = P + (int64_t(slot) * ngroups + g) * G3::VOL + int64_t(vz) * G3::N2;
+    cpx<T> *dst = P + (int64_t(slot) * nvol + gv) * G3::VOL + int64_t(vz) * G3::N2;
     // 16-byte stores: lanes c and c ^ 1 swap one value per row pair, the
     // even lane then writes row 2m and the odd lane row 2m + 1 of columns
     // (c & ~1, c | 1)
@@ -1450,6 +1518,348 @@ __global__ __launch_bounds__(64 * NS, 8 / NS) void k_accum_lds(
 }
 
 // ---------------------------------------------------------------------------
+// Group-pair accumulation on the 32^3 grid (k_prod1 with HP).
+//
+// Volume u of a DOF holds Xu = DFT-_{x,y}(a_2u + i a_2u+1), a_g = Re G_g.
+// With Z = DFT-_z Xu and xi- = -xi (mod N) the two Hermitian spectra
+// A_g = DFT-(a_g) separate as
+//   Z(xi) + conj Z(xi-) = 2 A_2u(xi),   Z(xi) - conj Z(xi-) = 2i A_2u+1(xi),
+// and conj Z(xi-) along a z pencil is the z transform of the conjugated
+// pencil at (-I', -J').  The Hermitian part of the gain spectrum,
+//   H(xi) = sum_c U_c A_g(c) + V_c B_g(c)
+// (U, V: k_prod1), is itself Hermitian, so only the rows I' = 0..N/2 are
+// evaluated and row -I' is their conjugate.  B_g = DFT-(Im G_g) enters only
+// on the Nyquist set, where V_c can be non-zero (k_nyqb).
+//
+// Tables (FP64):
+//   X [entry][r][2][K'][J']: U_c/2 (m = 0, group 2u) and U_c'/(2i) (m = 1,
+//     group 2u + 1) of an entry (volume u, target t) on row I' = r;
+//   VN[contrib][kCPer32]:    V_c on the rows r <= N/2 of the three Nyquist
+//     planes, each point once: VK[r][J'] (K' = N/2), VJ[r][K'] (J' = N/2,
+//     K' != N/2), VI[J'][K'] (I' = N/2, J' and K' != N/2);
+//   C [slot][target][kCPer32]: sum_c V_c B_g(c) in the same layout (k_nyqb).
+// W is stored [K][I][J] with l = (l0[I], l0[J], l0[K]).
+// ---------------------------------------------------------------------------
+constexpr int kHRows32 = 17;                                 // rows I' = 0..N/2 (N = 32)
+constexpr int kCPer32 = 2 * kHRows32 * 32 + 32 * 32;         // V / C entries per contribution / target
+
+__global__ void k_htab(cpx<double> *X, const cpx<double> *W, const int *ent_c, int N, int R) {
+    const int64_t n = int64_t(N) * N * N;
+    const int per = R * 2 * N * N;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = blockIdx.y;
+    if (idx >= per) return;
+    const int J = idx % N, K = (idx / N) % N, m = (idx / (N * N)) % 2, r = idx / (2 * N * N);
+    const int c = ent_c[2 * e + m];
+    cpx<double> out = {0, 0};
+    if (c >= 0) {
+        const cpx<double> w = W[c * n + (int64_t(K) * N + r) * N + J];
+        const cpx<double> wm = W[c * n + (int64_t((N - K) % N) * N + (N - r) % N) * N + (N - J) % N];
+        const double ux = 0.5 * (w.x + wm.x), uy = 0.5 * (w.y - wm.y);     // U = (W + conj W(-xi)) / 2
+        out = m == 0 ? cpx<double>{0.5 * ux, 0.5 * uy} : cpx<double>{0.5 * uy, -0.5 * ux};
+    }
+    X[int64_t(e) * per + idx] = out;
+}
+
+__global__ void k_vtab(cpx<double> *VN, const cpx<double> *W, int N, int R) {
+    const int64_t n = int64_t(N) * N * N;
+    const int H = N / 2;
+    const int per = 2 * R * N + N * N;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (idx >= per) return;
+    int I, J, K;
+    bool keep;
+    if (idx < R * N) {
+        I = idx / N; J = idx % N; K = H; keep = true;
+    } else if (idx < 2 * R * N) {
+        const int i2 = idx - R * N;
+        I = i2 / N; K = i2 % N; J = H; keep = K != H;
+    } else {
+        const int i3 = idx - 2 * R * N;
+        J = i3 / N; K = i3 % N; I = H; keep = J != H && K != H;
+    }
+    cpx<double> out = {0, 0};
+    if (keep) {
+        const cpx<double> w = W[c * n + (int64_t(K) * N + I) * N + J];
+        const cpx<double> wm = W[c * n + (int64_t((N - K) % N) * N + (N - I) % N) * N + (N - J) % N];
+        const double dx = w.x - wm.x, dy = w.y + wm.y;                    // W - conj W(-xi)
+        out = {-0.5 * dy, 0.5 * dx};                                       // V = i (W - conj W(-xi)) / 2
+    }
+    VN[int64_t(c) * per + idx] = out;
+}
+
+// ---------------------------------------------------------------------------
+// k_nyqb: C = sum_c V_c B_g(c) per (slot, target) from the Nyquist sums NB
+// of k_prod1.  On the three planes
+//   B(I', J', N/2) = DFT-_{x,y} s_K,  B(I', N/2, K') = DFT-_{x,z} s_J,
+//   B(N/2, J', K') = DFT-_{y,z} s_I,
+// s_K summed over the four vz-group partials.  s_K and s_J are real, so one
+// complex transform of Z = s_K + i s_J gives both: with Z~(p, q) = Z(-p, -q),
+// B_K = (Z + conj Z~)/2 and B_J = (Z - conj Z~)/(2i).  One workgroup per
+// (slot, target), two waves: wave 0 transforms s_K + i s_J, wave 1 s_I, of
+// contribution c on lanes 0-31 and c + 1 on lanes 32-63 (one plane per
+// half-wave, one row per lane, in-register 32-point transforms, an LDS
+// transpose).  The two halves' FP64 sums are added at the end.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(128) void k_nyqb(const float *__restrict__ NB, const cpx<double> *__restrict__ VN,
+                                              const int *__restrict__ c_group, const int *__restrict__ t_c0,
+                                              const int *__restrict__ t_cn, cpx<double> *__restrict__ C,
+                                              int ngroups, int ntgt) {
+    constexpr int N = 32, R = kHRows32, ROW = N + 1;
+    __shared__ cpx<float> lds[4][N * ROW];
+    const int slot = int(blockIdx.x) / ntgt, t = int(blockIdx.x) % ntgt;
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / 64);
+    const int hw = (int(threadIdx.x) / 32) & 1;
+    const int c = int(threadIdx.x) % 32;
+    cpx<float> *L = lds[wave * 2 + hw];
+    const int c0 = t_c0[t], cn = t_cn[t];
+    cpx<double> acc[2 * R];
+#pragma unroll
+    for (int i = 0; i < 2 * R; ++i) acc[i] = {0, 0};
+    for (int it = 0; 2 * it < cn; ++it) {
+        const int ci = c0 + 2 * it + hw;
+        const bool live = ci < c0 + cn;
+        const float *nb = NB + (int64_t(slot) * ngroups + (live ? c_group[ci] : 0)) * kNbPer32;
+        cpx<float> z[N];
+        if (wave == 0) {
+            // row vx = c: s_K(vx, vy = w) + i s_J(vx, vz = w)
+#pragma unroll
+            for (int w4 = 0; w4 < N / 4; ++w4) {
+                float4 sk = {0, 0, 0, 0};
+                if (live) {
+#pragma unroll
+                    for (int G = 0; G < 4; ++G) {
+                        const float4 q = *reinterpret_cast<const float4 *>(nb + G * N * N + c * N + 4 * w4);
+                        sk.x += q.x; sk.y += q.y; sk.z += q.z; sk.w += q.w;
+                    }
+                }
+                const float sk4[4] = {sk.x, sk.y, sk.z, sk.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int w = 4 * w4 + j;
+                    z[w] = {sk4[j], live ? nb[4 * N * N + w * N + c] : 0.f};
+                }
+            }
+        } else {
+            // row vy = c: s_I(vy, vz = w)
+#pragma unroll
+            for (int w = 0; w < N; ++w) z[w] = {live ? nb[5 * N * N + w * N + c] : 0.f, 0.f};
+        }
+        fq::LFFT<N, -1>::run(z);                               // along w
+        plane_sync<32>();
+#pragma unroll
+        for (int w = 0; w < N; ++w) L[c * ROW + w] = z[w];
+        plane_sync<32>();
+#pragma unroll
+        for (int i = 0; i < N; ++i) z[i] = L[i * ROW + c];
+        fq::LFFT<N, -1>::run(z);                               // along the rows: z[p] = Z(p, q = c)
+        if (wave == 0) {
+            plane_sync<32>();
+#pragma unroll
+            for (int p = 0; p < N; ++p) L[p * ROW + c] = z[p];
+            plane_sync<32>();
+            const int qm = (N - c) % N;
+            const cpx<double> *vk = VN + int64_t(live ? ci : 0) * kCPer32;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const cpx<float> zp = L[((N - r) % N) * ROW + qm];
+                const cpx<double> bk = {0.5 * (double(z[r].x) + zp.x), 0.5 * (double(z[r].y) - zp.y)};
+                const cpx<double> bj = {0.5 * (double(z[r].y) + zp.y), -0.5 * (double(z[r].x) - zp.x)};
+                if (live) {
+                    fq::cmac(acc[r], vk[r * N + c], bk);
+                    fq::cmac(acc[R + r], vk[R * N + r * N + c], bj);
+                }
+            }
+            plane_sync<32>();                                  // partner reads done
+        } else {
+            const cpx<double> *vi = VN + int64_t(live ? ci : 0) * kCPer32 + 2 * R * N;
+            if (live) {
+#pragma unroll
+                for (int p = 0; p < N; ++p) fq::cmac(acc[p], vi[p * N + c], cvt<double>(z[p]));
+            }
+        }
+    }
+    // the two contribution parities of each output
+#pragma unroll
+    for (int i = 0; i < 2 * R; ++i) {
+        acc[i].x += __shfl_xor(acc[i].x, 32);
+        acc[i].y += __shfl_xor(acc[i].y, 32);
+    }
+    if (hw) return;
+    cpx<double> *cc = C + (int64_t(slot) * ntgt + t) * kCPer32;
+    if (wave == 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            cc[r * N + c] = acc[r];
+            cc[R * N + r * N + c] = acc[R + r];
+        }
+    } else {
+#pragma unroll
+        for (int p = 0; p < N; ++p) cc[2 * R * N + p * N + c] = acc[p];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_accum_h: the group-pair accumulation.  One workgroup = NS consecutive
+// DOF slots of one (target, row pair r, -r), r = 0..N/2, one wave each; lane
+// (J, half) holds the z pencil (r, J) of the volume and the conjugated pencil
+// (-r, -J).  Per entry (volume u, target t):
+//   s = DFT-_z(v1 + v2) = 2 A_2u,  d = DFT-_z(v1 - v2) = 2i A_2u+1,
+//   H += X[e][r][0] s + X[e][r][1] d                               (FP64)
+// with the entry's weight block X[e][r] (2 N^2 FP64 values, the same for the
+// NS slots) staged in double-buffered LDS one entry ahead, as k_accum_lds
+// does.  Then C (k_nyqb) is added on the Nyquist points of row r, the gain
+// row r is transformed back along z, and row -r is written as its conjugate.
+// The loss rows r and -r are evaluated directly (L = sum Lt . F).
+// ---------------------------------------------------------------------------
+template <int NS>
+__global__ __launch_bounds__(64 * NS, 8 / NS) void k_accum_h(
+    const cpx<float> *__restrict__ P, const cpx<double> *__restrict__ F, const cpx<double> *__restrict__ X,
+    const int *__restrict__ e_vol, const int *__restrict__ t_e0, const int *__restrict__ t_en,
+    const cpx<double> *__restrict__ C, const cpx<double> *__restrict__ W, const int *__restrict__ l_src,
+    const int *__restrict__ t_l0, const int *__restrict__ t_ln, int ncontribs, cpx<double> *__restrict__ Y,
+    int nvol, int nsrc, int ntgt, int D) {
+    constexpr int N = 32, R = kHRows32;
+    using G = Geo<N>;
+    constexpr int E = G::E;
+    constexpr int WR = 2 * N * N;                 // weight block of one (entry, row)
+    constexpr int TH = 64 * NS;
+    constexpr int CP = WR / TH;                   // 16-byte copies per thread
+    static_assert(WR % TH == 0, "whole copies");
+    __shared__ cpx<double> wl[2][WR];
+    const int nsg = (D + NS - 1) / NS;
+    const int b = blockIdx.x;
+    const int sg = b % nsg;
+    const int t = (b / nsg) % ntgt;
+    const int r = b / (nsg * ntgt);
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / 64);
+    const int slot = sg * NS + wave;
+    const bool live = slot < D;                   // wave-uniform
+    const int lt = threadIdx.x % 64;
+    const int J = lt >> 1, half = lt & 1;
+    const int I1 = r, I2 = (N - r) % N, Jp = (N - J) % N;
+    const int jo = J & 1;
+    const int e0 = t_e0[t], en = t_en[t];
+    cpx<double> wp[CP];
+    const cpx<double> *xr = X + int64_t(r) * WR;  // entry e: + e * R * WR
+    const cpx<float> *pslot = P + int64_t(live ? slot : 0) * nvol * G::VOL;
+    // v1: 16-byte column pairs at the planes half + 2(jo + 2f), completed by
+    // the exchange with lane ^ 2 below (as in k_accum_lds); v2: the partner
+    // column -J of row -r at the planes half + 2e
+    auto load = [&](int ei, cpx<float> *v1, cpx<float> *v2) {
+        const cpx<float> *vol = pslot + int64_t(e_vol[ei]) * G::VOL;
+        const cpx<float> *s1 = vol + int64_t(I1) * N + (J & ~1);
+        const cpx<float> *s2 = vol + int64_t(I2) * N + Jp;
+#pragma unroll
+        for (int f = 0; f < E / 2; ++f) {
+            cpx2f w = {{0, 0}, {0, 0}};
+            if (live) w = *reinterpret_cast<const cpx2f *>(s1 + int64_t(half + 2 * (jo + 2 * f)) * G::N2);
+            v1[2 * f] = w.a;
+            v1[2 * f + 1] = w.b;
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) v2[e] = live ? s2[int64_t(half + 2 * e) * G::N2] : cpx<float>{0, 0};
+    };
+    cpx<double> H[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) H[e] = {0, 0};
+    cpx<float> n1[E], n2[E];
+    if (en > 0) {
+        fq::sfor<0, CP>([&](auto kk) {
+            constexpr int k = decltype(kk)::value;
+            const cpx<double> *q = xr + int64_t(e0) * R * WR + int(threadIdx.x) + TH * k;
+            wp[k] = {q->x, q->y};
+        });
+        load(e0, n1, n2);
+    }
+    for (int ei = e0, it = 0; ei < e0 + en; ++ei, ++it) {
+        const int buf = it & 1;
+        fq::sfor<0, CP>([&](auto kk) {
+            constexpr int k = decltype(kk)::value;
+            const int idx = int(threadIdx.x) + TH * k;
+            wl[buf][idx].x = wp[k].x;
+            wl[buf][idx].y = wp[k].y;
+        });
+        const int en1 = ei + 1 < e0 + en ? ei + 1 : ei;
+        fq::sfor<0, CP>([&](auto kk) {
+            constexpr int k = decltype(kk)::value;
+            const cpx<double> *q = xr + int64_t(en1) * R * WR + int(threadIdx.x) + TH * k;
+            wp[k] = {q->x, q->y};
+        });
+        __syncthreads();
+        cpx<float> v1[E], v2[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) { v1[e] = n1[e]; v2[e] = n2[e]; }
+        if (ei + 1 < e0 + en) load(ei + 1, n1, n2);
+        if (live) {
+#pragma unroll
+            for (int f = 0; f < E / 2; ++f) {
+                const cpx<float> a = v1[2 * f], bb = v1[2 * f + 1];
+                const cpx<float> recv = fq::swap_q2(jo ? a : bb);
+                v1[2 * f] = jo ? recv : a;
+                v1[2 * f + 1] = jo ? bb : recv;
+            }
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const cpx<float> a = v1[e], q = v2[e];
+                v1[e] = {a.x + q.x, a.y - q.y};            // v1 + conj v2
+                v2[e] = {a.x - q.x, a.y + q.y};            // v1 - conj v2
+            }
+            fq::fft2_dit<N, -1>(v1, half);                 // -> K' = half*E + e
+            fq::fft2_dit<N, -1>(v2, half);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int K = half * E + e;
+                fq::cmac(H[e], wl[buf][K * N + J], cvt<double>(v1[e]));
+                fq::cmac(H[e], wl[buf][N * N + K * N + J], cvt<double>(v2[e]));
+            }
+        }
+    }
+    if (!live) return;                                // no barrier follows
+    if (C) {
+        const cpx<double> *cc = C + (int64_t(slot) * ntgt + t) * kCPer32;
+        if (half == 1) H[0] = H[0] + cc[r * N + J];                      // K' = N/2
+        if (J == N / 2) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) H[e] = H[e] + cc[R * N + r * N + half * E + e];
+        }
+        if (r == N / 2) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) H[e] = H[e] + cc[2 * R * N + J * N + half * E + e];
+        }
+    }
+    fq::fft2_dif<N, 1>(H, half);                      // -> vz = half + 2e
+    cpx<double> *yg = Y + ((int64_t(slot) * ntgt + t) * 2 + 0) * G::VOL;
+#pragma unroll
+    for (int e = 0; e < E; ++e) yg[int64_t(half + 2 * e) * G::N2 + I1 * N + J] = H[e];
+    if (I2 != I1) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) yg[int64_t(half + 2 * e) * G::N2 + I2 * N + Jp] = {H[e].x, -H[e].y};
+    }
+    const int l0 = t_l0[t], ln = t_ln[t];
+    cpx<double> *yl = Y + ((int64_t(slot) * ntgt + t) * 2 + 1) * G::VOL;
+    for (int rr = 0; rr < (I2 != I1 ? 2 : 1); ++rr) {
+        const int64_t off = int64_t(rr ? I2 : I1) * N + J;
+#pragma unroll
+        for (int e = 0; e < E; ++e) H[e] = {0, 0};
+        for (int li = l0; li < l0 + ln; ++li) {
+            const int s = l_src[li];
+            const cpx<double> *fs = F + (int64_t(slot) * nsrc + s) * G::VOL + off;
+            const cpx<double> *w = W + int64_t(ncontribs + li) * G::VOL + off;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int64_t k = int64_t(half * E + e) * G::N2;
+                H[e] = H[e] + fq::cmul(w[k], fs[k]);
+            }
+        }
+        fq::fft2_dif<N, 1>(H, half);
+#pragma unroll
+        for (int e = 0; e < E; ++e) yl[int64_t(half + 2 * e) * G::N2 + off] = H[e];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // k_out: unit = (slot, target, vz).  gain = DFT+_{x,y}(Yg), loss = DFT+_{x,y}(Yl)
 // Q[v] = Re gain - f[v] Re loss, scattered into the AoSoA matrix
 // (output kernel, vhs-gll.mako:246-266; vss-gll.mako:275-300).
@@ -1628,6 +2038,8 @@ struct Scratch {
     cpx<float> *F32 = nullptr;
     void *NY = nullptr;
     void *FJ = nullptr;
+    float *NB = nullptr;          // group pairs: Nyquist sums of Im G_g (k_prod1 -> k_nyqb)
+    cpx<double> *C = nullptr;     // group pairs: Nyquist corrections (k_nyqb -> k_accum_h)
 };
 
 struct fq_handle {
@@ -1657,6 +2069,17 @@ struct fq_handle {
     int *pconj = nullptr;        // general FP32 plans: per product, 1 = conjugate pair (zstage_conj)
     void *zq = nullptr;          // [products][4][3 + 2 N/4] cpx<T> zstage_conj coefficients
     cpx<float> *F32 = nullptr;   // FP32 copy of F (mixed, fused)
+    // group pairs (32^3 FP32, single-product groups): k_prod1 transforms two
+    // groups' real parts as one volume, k_accum_h accumulates the Hermitian
+    // part of the gain spectrum on rows 0..N/2, k_nyqb the Nyquist terms
+    bool hpair = false;
+    bool hp_nyq = false;         // the plan's V tables are not all zero (VSS)
+    int nvol = 0;                // product volumes per DOF: ngroups, or ceil(ngroups / 2)
+    cpx<double> *X = nullptr;    // [entries][17][2][N][N]
+    cpx<double> *VN = nullptr;   // [contribs][kCPer32]
+    int *e_vol = nullptr, *t_e0 = nullptr, *t_en = nullptr;
+    float *NB = nullptr;         // scratch [chunk][groups][kNbPer32]
+    cpx<double> *Cn = nullptr;   // scratch [chunk][targets][kCPer32]
     // scratch
     cpx<double> *F = nullptr;
     void *Z = nullptr;           // also T0 and Y
@@ -1665,6 +2088,7 @@ struct fq_handle {
     // per-DOF bytes of each scratch region (for carving the static array of
     // fq_eval_noalloc) and that array's device address on this handle's device
     size_t per_dof_F = 0, per_dof_Z = 0, per_dof_P = 0, per_dof_F32 = 0, per_dof_NY = 0, per_dof_FJ = 0;
+    size_t per_dof_NB = 0, per_dof_C = 0;
     unsigned char *noalloc = nullptr;
     int cus = 256;               // compute units of the device
     // optional per-kernel HIP-event timing
@@ -1674,7 +2098,7 @@ struct fq_handle {
 };
 
 static const char *kKernelNames[FQ_NKERNELS] = {"k_fwd_z", "k_fwd_xy", "k_inv_z", "k_plane", "k_accum",
-                                                "k_out",   "k_project", "k_prod",  "k_nyq"};
+                                                "k_out",   "k_project", "k_prod",  "k_nyq",   "k_nyqb"};
 
 namespace {
 
@@ -1919,6 +2343,68 @@ int build_projection(fq_handle *h) {
     return rc;
 }
 
+// Group-pair tables: the entries (volume u, target t) with the contributions
+// of groups 2u and 2u + 1 to t, their weights X and the Nyquist weights VN of
+// every contribution, both derived from W on the device.
+int build_hpair(fq_handle *h) {
+    const fq::Plan &P = h->P;
+    const int N = P.N, R = kHRows32;
+    std::map<std::pair<int, int>, int> cix;           // (group, target) -> contribution
+    for (size_t ci = 0; ci < P.contribs.size(); ++ci) cix[{P.contribs[ci].group, P.contribs[ci].tgt}] = int(ci);
+    std::vector<int> ent_c, e_vol, te0(P.ntgt, 0), ten(P.ntgt, 0);
+    for (int t = 0; t < P.ntgt; ++t) {
+        te0[t] = int(e_vol.size());
+        for (int u = 0; u < h->nvol; ++u) {
+            int cc[2] = {-1, -1};
+            for (int m = 0; m < 2; ++m) {
+                auto it = cix.find({2 * u + m, t});
+                if (it != cix.end()) cc[m] = it->second;
+            }
+            if (cc[0] < 0 && cc[1] < 0) continue;
+            ent_c.push_back(cc[0]);
+            ent_c.push_back(cc[1]);
+            e_vol.push_back(u);
+        }
+        ten[t] = int(e_vol.size()) - te0[t];
+    }
+    int rc = FQ_SUCCESS;
+    auto chk = [&](int r) { if (rc == FQ_SUCCESS) rc = r; };
+    int *d_ent = nullptr;
+    chk(upload(&d_ent, ent_c));
+    chk(upload(&h->e_vol, e_vol));
+    chk(upload(&h->t_e0, te0));
+    chk(upload(&h->t_en, ten));
+    const int nent = int(e_vol.size());
+    const size_t xper = size_t(R) * 2 * N * N;
+    if (rc == FQ_SUCCESS && nent > 0) {
+        hipError_t e = hipMalloc(&h->X, nent * xper * sizeof(cpx<double>));
+        if (e != hipSuccess) rc = fail(FQ_ERR_ALLOC, std::string("hipMalloc group-pair tables: ") + hipGetErrorString(e));
+        if (rc == FQ_SUCCESS) {
+            hipLaunchKernelGGL(k_htab, dim3(unsigned((xper + 255) / 256), unsigned(nent)), dim3(256), 0, 0, h->X,
+                               h->W, d_ent, N, R);
+            e = hipGetLastError();
+            if (e != hipSuccess) rc = fail(FQ_ERR_HIP, std::string("k_htab: ") + hipGetErrorString(e));
+        }
+    }
+    const int nc = int(P.contribs.size());
+    if (rc == FQ_SUCCESS && h->hp_nyq && nc > 0) {
+        hipError_t e = hipMalloc(&h->VN, size_t(nc) * kCPer32 * sizeof(cpx<double>));
+        if (e != hipSuccess) rc = fail(FQ_ERR_ALLOC, std::string("hipMalloc Nyquist tables: ") + hipGetErrorString(e));
+        if (rc == FQ_SUCCESS) {
+            hipLaunchKernelGGL(k_vtab, dim3(unsigned((kCPer32 + 255) / 256), unsigned(nc)), dim3(256), 0, 0, h->VN,
+                               h->W, N, R);
+            e = hipGetLastError();
+            if (e != hipSuccess) rc = fail(FQ_ERR_HIP, std::string("k_vtab: ") + hipGetErrorString(e));
+        }
+    }
+    if (rc == FQ_SUCCESS) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) rc = fail(FQ_ERR_HIP, std::string("group-pair tables: ") + hipGetErrorString(e));
+    }
+    if (d_ent) (void)hipFree(d_ent);
+    return rc;
+}
+
 template <int N, typename T> int run_chunk(fq_handle *h, const Scratch &sc, const AosoaPtrs &ptr, const Dof &dof,
                                            int D, bool prof, hipStream_t st) {
     using G = Geo<N>;
@@ -1988,14 +2474,28 @@ template <int N, typename T> int run_chunk(fq_handle *h, const Scratch &sc, cons
                 }
                 using KP = decltype(&k_prod1<N, T, true, true, false>);
                 KP kern;
+                dim3 pgrid = grid;
                 if (!h->pack) kern = multi ? k_prod1<N, T, true, false, false> : k_prod1<N, T, false, false, false>;
                 else if (cj) kern = multi ? k_prod1<N, T, true, true, true> : k_prod1<N, T, false, true, true>;
                 else kern = multi ? k_prod1<N, T, true, true, false> : k_prod1<N, T, false, true, false>;
+                if constexpr (N == 32) {
+                    if (h->hpair) {
+                        kern = cj ? k_prod1<N, T, false, true, true, true> : k_prod1<N, T, false, true, false, true>;
+                        pgrid = dim3(unsigned(int64_t(D) * h->nvol * 4));
+                    }
+                }
                 const cpx<T> *zq = reinterpret_cast<const cpx<T> *>(h->zq);
                 pre();
-                hipLaunchKernelGGL(kern, grid, blk1, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0, h->g_np,
-                                   h->item_src, ngroups, nsrc, NY, nprods, zq, h->pconj);
+                hipLaunchKernelGGL(kern, pgrid, blk1, 0, st, FT, Pg, ph, zc, h->prods, h->g_p0, h->g_np,
+                                   h->item_src, ngroups, nsrc, NY, nprods, zq, h->pconj,
+                                   h->hpair && h->hp_nyq ? sc.NB : nullptr);
                 post(7);
+                if (h->hpair && h->hp_nyq) {
+                    pre();
+                    hipLaunchKernelGGL(k_nyqb, dim3(unsigned(int64_t(D) * ntgt)), dim3(128), 0, st, sc.NB, h->VN,
+                                       h->c_group, h->t_c0, h->t_cn, sc.C, ngroups, ntgt);
+                    post(9);
+                }
             }
         }
         if (!one_lane) {
@@ -2044,7 +2544,19 @@ template <int N, typename T> int run_chunk(fq_handle *h, const Scratch &sc, cons
     }
     u = int64_t(D) * ntgt * N;
     pre();
-    if constexpr (N == 32) {
+    if (h->hpair) {
+        if constexpr (N == 32 && sizeof(T) == 4) {
+            // group pairs: rows r = 0..N/2, eight (or with FQ_FLAG_ACCUM_QUAD four)
+            // DOF slots per workgroup
+            const bool eight = !(h->flags & FQ_FLAG_ACCUM_QUAD);
+            auto kacc = eight ? k_accum_h<8> : k_accum_h<4>;
+            const int NS = eight ? 8 : 4;
+            hipLaunchKernelGGL(kacc, dim3(unsigned(int64_t((D + NS - 1) / NS) * ntgt * kHRows32)), dim3(64 * NS), 0,
+                               st, reinterpret_cast<const cpx<float> *>(sc.Pg), sc.F, h->X, h->e_vol, h->t_e0,
+                               h->t_en, h->hp_nyq ? sc.C : nullptr, h->W, h->l_src, h->t_l0, h->t_ln,
+                               int(P.contribs.size()), Y, h->nvol, nsrc, ntgt, D);
+        }
+    } else if constexpr (N == 32) {
         // FP32 pencils: eight DOF slots per workgroup (one per CU) unless
         // FQ_FLAG_ACCUM_QUAD asks for four (two per CU); FP64: four (three
         // workgroups per CU, register-limited)
@@ -2109,7 +2621,9 @@ int eval_impl(fq_handle *h, const Scratch &sc, int64_t chunk, bool prof, const A
     if (h->P.N == 32 && h->fused && nbatch > 1) {
         // FP32: 8 slots per workgroup and one workgroup per CU, or 4 slots and
         // two; FP64: 4 slots, three workgroups per CU
-        const int64_t align = int64_t(h->cus) * (h->precision == FQ_PREC_MIXED ? 8 : 12) / 32;
+        // group pairs: 8 slots x 17 row pairs per workgroup, one workgroup per CU
+        const int64_t align = h->hpair ? int64_t(h->cus) * 8 / kHRows32 / 8 * 8
+                                       : int64_t(h->cus) * (h->precision == FQ_PREC_MIXED ? 8 : 12) / 32;
         if (align > 0 && chunk >= align) {
             const int64_t pmax = chunk / align * align;
             nbatch = (ndof + pmax - 1) / pmax;
@@ -2139,6 +2653,7 @@ int eval_own(fq_handle *h, const AosoaPtrs &ptr, int32_t nupts, int64_t ldim, in
         return fail(FQ_ERR_INVALID, "handle created with FQ_FLAG_NO_SCRATCH has no scratch; use fq_eval_noalloc");
     Scratch sc;
     sc.F = h->F; sc.Z = h->Z; sc.Pg = h->Pg; sc.F32 = h->F32; sc.NY = h->NY; sc.FJ = h->FJ;
+    sc.NB = h->NB; sc.C = h->Cn;
     return eval_impl(h, sc, h->chunk, h->prof, ptr, nupts, ldim, neles, first_dof, ndof, stream);
 }
 
@@ -2166,7 +2681,8 @@ constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 size_t noalloc_bytes(const fq_handle *h, int64_t c) {
     const size_t cc = size_t(c);
     return align256(h->per_dof_F * cc) + align256(h->per_dof_Z * cc) + align256(h->per_dof_P * cc) +
-           align256(h->per_dof_F32 * cc) + align256(h->per_dof_NY * cc) + align256(h->per_dof_FJ * cc);
+           align256(h->per_dof_F32 * cc) + align256(h->per_dof_NY * cc) + align256(h->per_dof_FJ * cc) +
+           align256(h->per_dof_NB * cc) + align256(h->per_dof_C * cc);
 }
 
 // Carves the regions of a batch of c DOFs out of the static scratch at base.
@@ -2186,6 +2702,8 @@ Scratch noalloc_carve(const fq_handle *h, unsigned char *base, int64_t c) {
     sc.F32 = static_cast<cpx<float> *>(take(h->per_dof_F32));
     sc.NY = take(h->per_dof_NY);
     sc.FJ = take(h->per_dof_FJ);
+    sc.NB = static_cast<float *>(take(h->per_dof_NB));
+    sc.C = static_cast<cpx<double> *>(take(h->per_dof_C));
     return sc;
 }
 
@@ -2267,6 +2785,15 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     h->two_lane = (prm->flags & FQ_FLAG_TWO_LANE) != 0;
     h->pack = h->fused && !(h->two_lane && h->precision == FQ_PREC_MIXED) && (prm->Nv == 24 || prm->Nv == 32) &&
               !(prm->flags & FQ_FLAG_NO_PACK);
+    h->nvol = h->P.ngroups;
+    if (h->pack && !h->two_lane && prm->Nv == 32 && h->precision == FQ_PREC_MIXED && !(prm->flags & FQ_FLAG_NO_HPAIR)) {
+        bool single = true;
+        for (int g = 0; g < h->P.ngroups; ++g) single = single && h->P.group_np[g] == 1;
+        h->hpair = single;
+        // VHS weights are real and even (sinc of |l|), so V vanishes there
+        h->hp_nyq = single && h->P.model == FQ_MODEL_VSS;
+        if (single) h->nvol = (h->P.ngroups + 1) / 2;
+    }
     if (hipGetDevice(&h->device) != hipSuccess) {
         delete h;
         return fail(FQ_ERR_HIP, "no HIP device");
@@ -2302,6 +2829,7 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     if (rc == FQ_SUCCESS) rc = build_projection(h);
     if (rc == FQ_SUCCESS && h->pack)
         rc = h->precision == FQ_PREC_MIXED ? upload_nyq<float>(h) : upload_nyq<double>(h);
+    if (rc == FQ_SUCCESS && h->hpair) rc = build_hpair(h);
     // scratch: F (FP64, nsrc) + max(Z (items, T), T0 (nsrc, FP64), Y (2 ntgt, FP64)) + P (groups, T)
     const size_t tsz = h->precision == FQ_PREC_MIXED ? sizeof(cpx<float>) : sizeof(cpx<double>);
     const size_t per_dof_F = size_t(P.nsrc) * n * sizeof(cpx<double>);
@@ -2309,15 +2837,20 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     const size_t per_dof_Z = std::max({h->fused ? size_t(0) : P.items.size() * n * tsz,
                                        size_t(P.nsrc) * n * sizeof(cpx<double>),
                                        size_t(2 * P.ntgt) * n * sizeof(cpx<double>)});
-    const size_t per_dof_P = size_t(P.ngroups) * n * tsz;
+    const size_t per_dof_P = size_t(h->nvol) * n * tsz;
     const size_t per_dof_F32 = h->precision == FQ_PREC_MIXED ? size_t(P.nsrc) * n * sizeof(cpx<float>) : 0;
     // Nyquist-plane corrections of the pair-packed product kernel and the
     // spectrum slices they are computed from
     const size_t per_dof_NY = h->pack ? P.prods.size() * 3 * size_t(N) * N * tsz : 0;
     const size_t per_dof_FJ = h->pack ? size_t(P.nsrc) * N * N * tsz : 0;
-    const size_t per_dof = per_dof_F + per_dof_Z + per_dof_P + per_dof_F32 + per_dof_NY + per_dof_FJ;
+    // group pairs: the Nyquist sums of Im G_g and the per-target corrections
+    const size_t per_dof_NB = h->hp_nyq ? size_t(P.ngroups) * kNbPer32 * sizeof(float) : 0;
+    const size_t per_dof_C = h->hp_nyq ? size_t(P.ntgt) * kCPer32 * sizeof(cpx<double>) : 0;
+    const size_t per_dof = per_dof_F + per_dof_Z + per_dof_P + per_dof_F32 + per_dof_NY + per_dof_FJ +
+                           per_dof_NB + per_dof_C;
     h->per_dof_F = per_dof_F; h->per_dof_Z = per_dof_Z; h->per_dof_P = per_dof_P;
     h->per_dof_F32 = per_dof_F32; h->per_dof_NY = per_dof_NY; h->per_dof_FJ = per_dof_FJ;
+    h->per_dof_NB = per_dof_NB; h->per_dof_C = per_dof_C;
     // the static scratch of fq_eval_noalloc, resolved with this device current
     h->noalloc = noalloc_resolve(h->device);
     if (rc == FQ_SUCCESS && (h->flags & FQ_FLAG_NO_SCRATCH) && !h->noalloc)
@@ -2342,6 +2875,10 @@ int fq_create(const fq_params *prm, fq_handle **out) {
         if (h->pack) {
             e5 = hipMalloc(&h->NY, per_dof_NY * chunk);
             if (e5 == hipSuccess) e5 = hipMalloc(&h->FJ, per_dof_FJ * chunk);
+        }
+        if (e5 == hipSuccess && h->hp_nyq) {
+            e5 = hipMalloc(&h->NB, per_dof_NB * chunk);
+            if (e5 == hipSuccess) e5 = hipMalloc(&h->Cn, per_dof_C * chunk);
         }
         if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess || e5 != hipSuccess)
             rc = fail(FQ_ERR_ALLOC, "hipMalloc scratch failed (reduce max_chunk)");
@@ -2402,7 +2939,8 @@ int fq_destroy(fq_handle *h) {
                     (void *)h->c_group, (void *)h->t_c0, (void *)h->t_cn, (void *)h->l_src, (void *)h->t_l0,
                     (void *)h->t_ln, (void *)h->W, (void *)h->ginv, (void *)h->pmass, (void *)h->F, h->Z,
                     h->Pg, h->zc, (void *)h->F32, h->NY, h->FJ, (void *)h->nyv, (void *)h->zq,
-                    (void *)h->pconj})
+                    (void *)h->pconj, (void *)h->X, (void *)h->VN, (void *)h->e_vol, (void *)h->t_e0,
+                    (void *)h->t_en, (void *)h->NB, (void *)h->Cn})
         if (p) (void)hipFree(p);
     (void)hipThreadExchangeStreamCaptureMode(&mode);
     delete h;

@@ -63,6 +63,8 @@ def _flags(cfg):
         fl |= _lib.FLAG_TWO_LANE
     if not cfg.getint('scattering-model', 'pair-pack', 1):
         fl |= _lib.FLAG_NO_PACK
+    if not cfg.getint('scattering-model', 'group-pairs', 1):
+        fl |= _lib.FLAG_NO_HPAIR
     return fl
 
 

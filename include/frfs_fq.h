@@ -76,11 +76,16 @@ enum fq_flags {
                                  corrections (for comparison) */
     FQ_FLAG_NO_SCRATCH = 16,  /* fq_create allocates the tables only, no scratch:
                                  the handle evaluates through fq_eval_noalloc */
-    FQ_FLAG_ACCUM_QUAD = 32   /* 32^3 grid, FP32 transforms: the accumulation
+    FQ_FLAG_ACCUM_QUAD = 32,  /* 32^3 grid, FP32 transforms: the accumulation
                                  kernel shares each weight row over four DOFs per
                                  workgroup (two workgroups per CU) instead of
                                  eight (one per CU); a schedule candidate of the
                                  autotuner */
+    FQ_FLAG_NO_HPAIR = 64     /* 32^3 grid, FP32 transforms: write one product
+                                 volume per group instead of one per group pair
+                                 (the two groups' real parts as one complex
+                                 volume, the Hermitian part of the gain spectrum
+                                 accumulated on half the rows); for comparison */
 };
 
 /* Size of the static device scratch of fq_eval_noalloc (one copy per GPU the
@@ -195,7 +200,7 @@ int fq_eval_bilinear(fq_handle *h, const double *f, const double *g, double *q,
 
 /* ---- instrumentation --------------------------------------------------- */
 
-#define FQ_NKERNELS 9
+#define FQ_NKERNELS 10
 /* Bracket every kernel launch with HIP events on the launch stream (on = 1)
  * or stop (on = 0); either call discards earlier records.  No reference
  * counterpart: the reference times only its precompute with cuda.Event

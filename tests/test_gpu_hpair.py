@@ -1,0 +1,88 @@
+"""Group pairs on the 32^3 grid (FP32 transforms): the product kernel writes
+one volume per pair of groups (their real parts as one complex volume), the
+accumulation evaluates the Hermitian part of the gain spectrum on the rows
+I' = 0..N/2, and the imaginary parts of the products enter through the
+Nyquist-plane corrections.  The result is compared with the same plan run
+with one volume per group (FQ_FLAG_NO_HPAIR): both are FP32 transforms with
+FP64 accumulation, so they differ by rounding only (rel-L2 <= 2e-6), and
+with the CPU oracle (<= 1e-5).
+
+The inputs carry multiplicative noise: for smooth Maxwellians on [-6, 6]^3
+the Nyquist-plane part of the gain is about 3e-7 of it, below the mixed
+tolerance; with 30% noise it is about 0.7%, so a wrong or missing Nyquist
+correction fails these tests.
+"""
+import numpy as np
+import pytest
+
+from tests.cases import oracle_vss, rel_l2, vss_cfg
+from tests.test_gpu_parity import MIXED_TOL, _run_vss
+
+pytestmark = pytest.mark.gpu
+
+PAIR_VS_SINGLE = 2e-6
+
+
+def _rough(f, seed, amp=0.3):
+    rng = np.random.default_rng(seed)
+    return f * (1 + amp * rng.standard_normal(f.shape))
+
+
+def _cfgs(N, Nrho, M, dev, masses=(39.96e-3,), **kw):
+    return (vss_cfg(N, Nrho, M, dev, masses=masses, precision='mixed', **kw),
+            vss_cfg(N, Nrho, M, dev, masses=masses, precision='mixed', hpair=0, **kw))
+
+
+@pytest.mark.parametrize('M,Nrho', [(12, 3), (6, 3)])
+def test_single_species(backend, M, Nrho):
+    """M = 12: 18 groups (9 pairs); M = 6: 9 groups, the last pair has one
+    member."""
+    from oracle import fastspectral as fsp
+    cp, cs = _cfgs(32, Nrho, M, 6.0)
+    vm, prm, tab = oracle_vss(cp, 1)
+    f = _rough(fsp.two_maxwellian_batch(vm['cv'], 3, seed=91), 92)
+    print(f'[hpair vss-1 32^3 M={M} Nrho={Nrho}]', flush=True)
+    (qp,), smp = _run_vss(backend, cp, [f], 1, 3)
+    (qs,), _ = _run_vss(backend, cs, [f], 1, 3)
+    for d in range(3):
+        ref = fsp.vss_fs([f[d]], vm, tab, prm)[0]
+        e_ps, e_or = rel_l2(qp[d], qs[d]), rel_l2(qp[d], ref)
+        print(f'  dof {d}: pairs vs single {e_ps:.2e}, vs oracle {e_or:.2e}', flush=True)
+        assert e_ps <= PAIR_VS_SINGLE
+        assert e_or <= MIXED_TOL
+
+
+def test_two_species(backend):
+    """Two species, masses 1:2: general (cross-species) and conjugate
+    (same-species) products, groups contributing to two targets."""
+    from oracle import fastspectral as fsp
+    m0 = 39.96e-3
+    kw = dict(omega={'00': 0.74, '01': 0.775, '10': 0.775, '11': 0.81},
+              dref={'00': 4.17e-10, '01': 4.17e-10 * 1.15, '10': 4.17e-10 * 1.15, '11': 4.17e-10 * 1.3})
+    cp, cs = _cfgs(32, 2, 12, 6.0, masses=(m0, 2 * m0), **kw)
+    vm, prm, tab = oracle_vss(cp, 2)
+    f0 = _rough(fsp.two_maxwellian_batch(vm['cv'], 2, seed=93, mass=vm['masses'][0]), 94)
+    f1 = _rough(fsp.two_maxwellian_batch(vm['cv'], 2, seed=95, mass=vm['masses'][1], dens_range=(0.2, 0.8)), 96)
+    print('[hpair vss-2 32^3]', flush=True)
+    qp, _ = _run_vss(backend, cp, [f0, f1], 1, 2)
+    qs, _ = _run_vss(backend, cs, [f0, f1], 1, 2)
+    for d in range(2):
+        ref = fsp.vss_fs([f0[d], f1[d]], vm, tab, prm)
+        for s in range(2):
+            e_ps, e_or = rel_l2(qp[s][d], qs[s][d]), rel_l2(qp[s][d], ref[s])
+            print(f'  dof {d} species {s}: pairs vs single {e_ps:.2e}, vs oracle {e_or:.2e}', flush=True)
+            assert e_ps <= PAIR_VS_SINGLE
+            assert e_or <= MIXED_TOL
+
+
+def test_batches(backend):
+    """Batches of 3 DOFs (a partial accumulation workgroup) give the DOFs of
+    one batch bit for bit."""
+    from oracle import fastspectral as fsp
+    cfg = vss_cfg(32, 2, 12, 6.0, precision='mixed')
+    vm, prm, tab = oracle_vss(cfg, 1)
+    f = _rough(fsp.two_maxwellian_batch(vm['cv'], 7, seed=97), 98)
+    print('[hpair vss-1 32^3 batches]', flush=True)
+    (qa,), _ = _run_vss(backend, cfg, [f], 1, 7)
+    (qb,), _ = _run_vss(backend, vss_cfg(32, 2, 12, 6.0, precision='mixed', max_chunk=3), [f], 1, 7)
+    assert np.array_equal(qa, qb)
