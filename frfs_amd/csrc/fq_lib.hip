@@ -1531,7 +1531,8 @@ __global__ __launch_bounds__(64 * NS, 8 / NS) void k_accum_lds(
 // evaluated and row -I' is their conjugate.  B_g = DFT-(Im G_g) enters only
 // on the Nyquist set, where V_c can be non-zero (k_nyqb).
 //
-// Tables (FP64):
+// Tables (computed in FP64 from W; X stored in FP64, VN in FP32: its terms
+// are the Nyquist-plane part of the gain, a small fraction of it):
 //   X [entry][r][2][K'][J']: U_c/2 (m = 0, group 2u) and U_c'/(2i) (m = 1,
 //     group 2u + 1) of an entry (volume u, target t) on row I' = r;
 //   VN[contrib][kCPer32]:    V_c on the rows r <= N/2 of the three Nyquist
@@ -1561,7 +1562,7 @@ __global__ void k_htab(cpx<double> *X, const cpx<double> *W, const int *ent_c, i
     X[int64_t(e) * per + idx] = out;
 }
 
-__global__ void k_vtab(cpx<double> *VN, const cpx<double> *W, int N, int R) {
+__global__ void k_vtab(cpx<float> *VN, const cpx<double> *W, int N, int R) {
     const int64_t n = int64_t(N) * N * N;
     const int H = N / 2;
     const int per = 2 * R * N + N * N;
@@ -1586,7 +1587,7 @@ __global__ void k_vtab(cpx<double> *VN, const cpx<double> *W, int N, int R) {
         const double dx = w.x - wm.x, dy = w.y + wm.y;                    // W - conj W(-xi)
         out = {-0.5 * dy, 0.5 * dx};                                       // V = i (W - conj W(-xi)) / 2
     }
-    VN[int64_t(c) * per + idx] = out;
+    VN[int64_t(c) * per + idx] = {float(out.x), float(out.y)};
 }
 
 // ---------------------------------------------------------------------------
@@ -1596,107 +1597,95 @@ __global__ void k_vtab(cpx<double> *VN, const cpx<double> *W, int N, int R) {
 //   B(N/2, J', K') = DFT-_{y,z} s_I,
 // s_K summed over the four vz-group partials.  s_K and s_J are real, so one
 // complex transform of Z = s_K + i s_J gives both: with Z~(p, q) = Z(-p, -q),
-// B_K = (Z + conj Z~)/2 and B_J = (Z - conj Z~)/(2i).  One workgroup per
-// (slot, target), two waves: wave 0 transforms s_K + i s_J, wave 1 s_I, of
-// contribution c on lanes 0-31 and c + 1 on lanes 32-63 (one plane per
-// half-wave, one row per lane, in-register 32-point transforms, an LDS
-// transpose).  The two halves' FP64 sums are added at the end.
+// B_K = (Z + conj Z~)/2 and B_J = (Z - conj Z~)/(2i).
+// One workgroup per (slot, target), four waves, four contributions per
+// round: wave w transforms contribution 4i + w, Z on lanes 0-31 and s_I on
+// lanes 32-63 (one plane per half-wave, one row per lane, in-register
+// 32-point transforms and an LDS transpose), leaving both spectra in LDS.
+// Then every thread accumulates its own output points (k/256 of the 2112)
+// over the round's four contributions, with coalesced weight reads, in FP64.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(128) void k_nyqb(const float *__restrict__ NB, const cpx<double> *__restrict__ VN,
+__global__ __launch_bounds__(256) void k_nyqb(const float *__restrict__ NB, const cpx<float> *__restrict__ VN,
                                               const int *__restrict__ c_group, const int *__restrict__ t_c0,
                                               const int *__restrict__ t_cn, cpx<double> *__restrict__ C,
                                               int ngroups, int ntgt) {
-    constexpr int N = 32, R = kHRows32, ROW = N + 1;
-    __shared__ cpx<float> lds[4][N * ROW];
+    constexpr int N = 32, R = kHRows32, ROW = N + 1, NPT = kCPer32, PPT = (NPT + 255) / 256;
+    __shared__ cpx<float> lds[4][2][N * ROW];          // [wave][Z, s_I spectrum]
     const int slot = int(blockIdx.x) / ntgt, t = int(blockIdx.x) % ntgt;
-    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / 64);
-    const int hw = (int(threadIdx.x) / 32) & 1;
-    const int c = int(threadIdx.x) % 32;
-    cpx<float> *L = lds[wave * 2 + hw];
+    const int tid = int(threadIdx.x);
+    const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int hw = (tid / 32) & 1;
+    const int c = tid % 32;
+    cpx<float> *L = lds[wave][hw];
     const int c0 = t_c0[t], cn = t_cn[t];
-    cpx<double> acc[2 * R];
+    cpx<double> acc[PPT];
 #pragma unroll
-    for (int i = 0; i < 2 * R; ++i) acc[i] = {0, 0};
-    for (int it = 0; 2 * it < cn; ++it) {
-        const int ci = c0 + 2 * it + hw;
-        const bool live = ci < c0 + cn;
-        const float *nb = NB + (int64_t(slot) * ngroups + (live ? c_group[ci] : 0)) * kNbPer32;
-        cpx<float> z[N];
-        if (wave == 0) {
-            // row vx = c: s_K(vx, vy = w) + i s_J(vx, vz = w)
+    for (int j = 0; j < PPT; ++j) acc[j] = {0, 0};
+    for (int base = 0; base < cn; base += 4) {
+        if (base + wave < cn) {                            // wave-uniform
+            const float *nb = NB + (int64_t(slot) * ngroups + c_group[c0 + base + wave]) * kNbPer32;
+            cpx<float> z[N];
+            if (hw == 0) {
+                // row vx = c: s_K(vx, vy = w) + i s_J(vx, vz = w)
 #pragma unroll
-            for (int w4 = 0; w4 < N / 4; ++w4) {
-                float4 sk = {0, 0, 0, 0};
-                if (live) {
+                for (int w4 = 0; w4 < N / 4; ++w4) {
+                    float4 sk = *reinterpret_cast<const float4 *>(nb + c * N + 4 * w4);
 #pragma unroll
-                    for (int G = 0; G < 4; ++G) {
+                    for (int G = 1; G < 4; ++G) {
                         const float4 q = *reinterpret_cast<const float4 *>(nb + G * N * N + c * N + 4 * w4);
                         sk.x += q.x; sk.y += q.y; sk.z += q.z; sk.w += q.w;
                     }
-                }
-                const float sk4[4] = {sk.x, sk.y, sk.z, sk.w};
+                    const float sk4[4] = {sk.x, sk.y, sk.z, sk.w};
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int w = 4 * w4 + j;
-                    z[w] = {sk4[j], live ? nb[4 * N * N + w * N + c] : 0.f};
+                    for (int j = 0; j < 4; ++j) z[4 * w4 + j] = {sk4[j], nb[4 * N * N + (4 * w4 + j) * N + c]};
                 }
+            } else {
+                // row vy = c: s_I(vy, vz = w)
+#pragma unroll
+                for (int w = 0; w < N; ++w) z[w] = {nb[5 * N * N + w * N + c], 0.f};
             }
-        } else {
-            // row vy = c: s_I(vy, vz = w)
+            fq::LFFT<N, -1>::run(z);                       // along w
+            plane_sync<32>();
 #pragma unroll
-            for (int w = 0; w < N; ++w) z[w] = {live ? nb[5 * N * N + w * N + c] : 0.f, 0.f};
-        }
-        fq::LFFT<N, -1>::run(z);                               // along w
-        plane_sync<32>();
+            for (int w = 0; w < N; ++w) L[c * ROW + w] = z[w];
+            plane_sync<32>();
 #pragma unroll
-        for (int w = 0; w < N; ++w) L[c * ROW + w] = z[w];
-        plane_sync<32>();
-#pragma unroll
-        for (int i = 0; i < N; ++i) z[i] = L[i * ROW + c];
-        fq::LFFT<N, -1>::run(z);                               // along the rows: z[p] = Z(p, q = c)
-        if (wave == 0) {
+            for (int i = 0; i < N; ++i) z[i] = L[i * ROW + c];
+            fq::LFFT<N, -1>::run(z);                       // along the rows: z[p] = spectrum(p, q = c)
             plane_sync<32>();
 #pragma unroll
             for (int p = 0; p < N; ++p) L[p * ROW + c] = z[p];
-            plane_sync<32>();
-            const int qm = (N - c) % N;
-            const cpx<double> *vk = VN + int64_t(live ? ci : 0) * kCPer32;
+        }
+        __syncthreads();
+        for (int w2 = 0; w2 < 4 && base + w2 < cn; ++w2) {
+            const cpx<float> *v = VN + int64_t(c0 + base + w2) * NPT;
+            const cpx<float> *Z0 = lds[w2][0], *Z1 = lds[w2][1];
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const cpx<float> zp = L[((N - r) % N) * ROW + qm];
-                const cpx<double> bk = {0.5 * (double(z[r].x) + zp.x), 0.5 * (double(z[r].y) - zp.y)};
-                const cpx<double> bj = {0.5 * (double(z[r].y) + zp.y), -0.5 * (double(z[r].x) - zp.x)};
-                if (live) {
-                    fq::cmac(acc[r], vk[r * N + c], bk);
-                    fq::cmac(acc[R + r], vk[R * N + r * N + c], bj);
+            for (int j = 0; j < PPT; ++j) {
+                const int pt = tid + 256 * j;
+                if (pt < NPT) {
+                    cpx<double> bv;
+                    if (pt < 2 * R * N) {
+                        const int i2 = pt < R * N ? pt : pt - R * N;
+                        const int p = i2 / N, q = i2 % N;
+                        const cpx<float> z = Z0[p * ROW + q], zp = Z0[((N - p) % N) * ROW + (N - q) % N];
+                        bv = pt < R * N ? cpx<double>{0.5 * (double(z.x) + zp.x), 0.5 * (double(z.y) - zp.y)}
+                                        : cpx<double>{0.5 * (double(z.y) + zp.y), -0.5 * (double(z.x) - zp.x)};
+                    } else {
+                        const int i3 = pt - 2 * R * N;
+                        bv = cvt<double>(Z1[(i3 / N) * ROW + i3 % N]);
+                    }
+                    fq::cmac(acc[j], cvt<double>(v[pt]), bv);
                 }
             }
-            plane_sync<32>();                                  // partner reads done
-        } else {
-            const cpx<double> *vi = VN + int64_t(live ? ci : 0) * kCPer32 + 2 * R * N;
-            if (live) {
-#pragma unroll
-                for (int p = 0; p < N; ++p) fq::cmac(acc[p], vi[p * N + c], cvt<double>(z[p]));
-            }
         }
+        __syncthreads();                                   // LDS reads done before the next round
     }
-    // the two contribution parities of each output
+    cpx<double> *cc = C + (int64_t(slot) * ntgt + t) * NPT;
 #pragma unroll
-    for (int i = 0; i < 2 * R; ++i) {
-        acc[i].x += __shfl_xor(acc[i].x, 32);
-        acc[i].y += __shfl_xor(acc[i].y, 32);
-    }
-    if (hw) return;
-    cpx<double> *cc = C + (int64_t(slot) * ntgt + t) * kCPer32;
-    if (wave == 0) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            cc[r * N + c] = acc[r];
-            cc[R * N + r * N + c] = acc[R + r];
-        }
-    } else {
-#pragma unroll
-        for (int p = 0; p < N; ++p) cc[2 * R * N + p * N + c] = acc[p];
+    for (int j = 0; j < PPT; ++j) {
+        const int pt = tid + 256 * j;
+        if (pt < NPT) cc[pt] = acc[j];
     }
 }
 
@@ -1709,9 +1698,12 @@ __global__ __launch_bounds__(128) void k_nyqb(const float *__restrict__ NB, cons
 //   H += X[e][r][0] s + X[e][r][1] d                               (FP64)
 // with the entry's weight block X[e][r] (2 N^2 FP64 values, the same for the
 // NS slots) staged in double-buffered LDS one entry ahead, as k_accum_lds
-// does.  Then C (k_nyqb) is added on the Nyquist points of row r, the gain
-// row r is transformed back along z, and row -r is written as its conjugate.
-// The loss rows r and -r are evaluated directly (L = sum Lt . F).
+// does.  Then C (k_nyqb) is added on the Nyquist points of row r.  The loss
+// spectrum is made Hermitian too, Hl = (L(xi) + conj L(xi-))/2 with
+// L = sum Lt . F: like the gain it only enters through the real part of its
+// inverse transform.  Both are transformed back along z and written as one
+// volume Y = DFT+_z(H) + i DFT+_z(Hl) (k_out1); row -r holds
+// conj DFT+_z(H) + i conj DFT+_z(Hl).
 // ---------------------------------------------------------------------------
 template <int NS>
 __global__ __launch_bounds__(64 * NS, 8 / NS) void k_accum_h(
@@ -1742,7 +1734,7 @@ __global__ __launch_bounds__(64 * NS, 8 / NS) void k_accum_h(
     const int jo = J & 1;
     const int e0 = t_e0[t], en = t_en[t];
     cpx<double> wp[CP];
-    const cpx<double> *xr = X + int64_t(r) * WR;  // entry e: + e * R * WR
+    const cpx<double> *xr = X + int64_t(r) * WR;
     const cpx<float> *pslot = P + int64_t(live ? slot : 0) * nvol * G::VOL;
     // v1: 16-byte column pairs at the planes half + 2(jo + 2f), completed by
     // the exchange with lane ^ 2 below (as in k_accum_lds); v2: the partner
@@ -1769,7 +1761,7 @@ __global__ __launch_bounds__(64 * NS, 8 / NS) void k_accum_h(
         fq::sfor<0, CP>([&](auto kk) {
             constexpr int k = decltype(kk)::value;
             const cpx<double> *q = xr + int64_t(e0) * R * WR + int(threadIdx.x) + TH * k;
-            wp[k] = {q->x, q->y};
+            wp[k] = {q->x, q->y};                 // field copies: no aggregate memcpy
         });
         load(e0, n1, n2);
     }
@@ -1777,9 +1769,8 @@ __global__ __launch_bounds__(64 * NS, 8 / NS) void k_accum_h(
         const int buf = it & 1;
         fq::sfor<0, CP>([&](auto kk) {
             constexpr int k = decltype(kk)::value;
-            const int idx = int(threadIdx.x) + TH * k;
-            wl[buf][idx].x = wp[k].x;
-            wl[buf][idx].y = wp[k].y;
+            wl[buf][int(threadIdx.x) + TH * k].x = wp[k].x;
+            wl[buf][int(threadIdx.x) + TH * k].y = wp[k].y;
         });
         const int en1 = ei + 1 < e0 + en ? ei + 1 : ei;
         fq::sfor<0, CP>([&](auto kk) {
@@ -1808,11 +1799,12 @@ __global__ __launch_bounds__(64 * NS, 8 / NS) void k_accum_h(
             }
             fq::fft2_dit<N, -1>(v1, half);                 // -> K' = half*E + e
             fq::fft2_dit<N, -1>(v2, half);
+            const cpx<double> *w0 = wl[buf];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int K = half * E + e;
-                fq::cmac(H[e], wl[buf][K * N + J], cvt<double>(v1[e]));
-                fq::cmac(H[e], wl[buf][N * N + K * N + J], cvt<double>(v2[e]));
+                fq::cmac(H[e], w0[K * N + J], cvt<double>(v1[e]));
+                fq::cmac(H[e], w0[N * N + K * N + J], cvt<double>(v2[e]));
             }
         }
     }
@@ -1829,33 +1821,34 @@ __global__ __launch_bounds__(64 * NS, 8 / NS) void k_accum_h(
             for (int e = 0; e < E; ++e) H[e] = H[e] + cc[2 * R * N + J * N + half * E + e];
         }
     }
-    fq::fft2_dif<N, 1>(H, half);                      // -> vz = half + 2e
-    cpx<double> *yg = Y + ((int64_t(slot) * ntgt + t) * 2 + 0) * G::VOL;
+    // Hermitian loss on row r: the point xi- = (-r, -J, -K') of each K'
+    cpx<double> Hl[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) yg[int64_t(half + 2 * e) * G::N2 + I1 * N + J] = H[e];
+    for (int e = 0; e < E; ++e) Hl[e] = {0, 0};
+    const int l0 = t_l0[t], ln = t_ln[t];
+    const int64_t off1 = int64_t(I1) * N + J, off2 = int64_t(I2) * N + Jp;
+    for (int li = l0; li < l0 + ln; ++li) {
+        const int s = l_src[li];
+        const cpx<double> *fs = F + (int64_t(slot) * nsrc + s) * G::VOL;
+        const cpx<double> *w = W + int64_t(ncontribs + li) * G::VOL;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int K = half * E + e, Km = (N - K) % N;
+            const cpx<double> a = fq::cmul(w[int64_t(K) * G::N2 + off1], fs[int64_t(K) * G::N2 + off1]);
+            const cpx<double> m = fq::cmul(w[int64_t(Km) * G::N2 + off2], fs[int64_t(Km) * G::N2 + off2]);
+            Hl[e] = {Hl[e].x + 0.5 * (a.x + m.x), Hl[e].y + 0.5 * (a.y - m.y)};
+        }
+    }
+    fq::fft2_dif<N, 1>(H, half);                      // -> vz = half + 2e
+    fq::fft2_dif<N, 1>(Hl, half);
+    cpx<double> *y = Y + (int64_t(slot) * ntgt + t) * G::VOL;
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        y[int64_t(half + 2 * e) * G::N2 + off1] = {H[e].x - Hl[e].y, H[e].y + Hl[e].x};
     if (I2 != I1) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) yg[int64_t(half + 2 * e) * G::N2 + I2 * N + Jp] = {H[e].x, -H[e].y};
-    }
-    const int l0 = t_l0[t], ln = t_ln[t];
-    cpx<double> *yl = Y + ((int64_t(slot) * ntgt + t) * 2 + 1) * G::VOL;
-    for (int rr = 0; rr < (I2 != I1 ? 2 : 1); ++rr) {
-        const int64_t off = int64_t(rr ? I2 : I1) * N + J;
-#pragma unroll
-        for (int e = 0; e < E; ++e) H[e] = {0, 0};
-        for (int li = l0; li < l0 + ln; ++li) {
-            const int s = l_src[li];
-            const cpx<double> *fs = F + (int64_t(slot) * nsrc + s) * G::VOL + off;
-            const cpx<double> *w = W + int64_t(ncontribs + li) * G::VOL + off;
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const int64_t k = int64_t(half * E + e) * G::N2;
-                H[e] = H[e] + fq::cmul(w[k], fs[k]);
-            }
-        }
-        fq::fft2_dif<N, 1>(H, half);
-#pragma unroll
-        for (int e = 0; e < E; ++e) yl[int64_t(half + 2 * e) * G::N2 + off] = H[e];
+        for (int e = 0; e < E; ++e)
+            y[int64_t(half + 2 * e) * G::N2 + off2] = {H[e].x + Hl[e].y, Hl[e].x - H[e].y};
     }
 }
 
@@ -1922,6 +1915,44 @@ __global__ __launch_bounds__(kPlaneThreads, 2) void k_out(const cpx<double> *Y, 
             const int64_t vv = (int64_t(c) * N + vy) * N + vz;
             const int64_t a = b + vv * dof.soasz;
             q[a] = gr[e] - f[a] * lv[e].x;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_out1: the output stage of the group-pair path.  Y = Yg + i Yl with the
+// gain and loss spectra Hermitian, so DFT+_{x,y}(Y) = gain + i loss with
+// both real: one complex transform per plane, Q = Re - f Im.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(kPlaneThreads, 2) void k_out1(const cpx<double> *Y, AosoaPtrs ptr, Dof dof, int ntgt,
+                                                 int64_t nunits) {
+    using G = Geo<N>;
+    constexpr int E = G::E;
+    constexpr int UPB = kPlaneThreads / G::LPP;
+    __shared__ cpx<double> lds[UPB][N * G::LDSROW];
+    const int u = threadIdx.x / G::LPP;
+    const int lt = threadIdx.x % G::LPP;
+    const int64_t unit = int64_t(blockIdx.x) * UPB + u;
+    const bool valid = unit < nunits;
+    const int vz = int(unit % N);
+    const int64_t dt = unit / N;
+    const int t = int(dt % ntgt);
+    const int slot = int(dt / ntgt);
+    const int c = lt >> 1, half = lt & 1;
+    const bool act = valid && c < N;
+    cpx<double> v[E];
+    plane_inv64<N>(Y + (int64_t(slot) * ntgt + t) * G::VOL + int64_t(vz) * G::N2, lds[u], v, c, half, act);
+    if (act) {
+        const double *f = ptr.f[t];
+        double *q = ptr.q[t];
+        const int64_t b = dof.base(slot);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int vy = half + 2 * e;
+            const int64_t vv = (int64_t(c) * N + vy) * N + vz;
+            const int64_t a = b + vv * dof.soasz;
+            q[a] = v[e].x - f[a] * v[e].y;
         }
     }
 }
@@ -2075,8 +2106,8 @@ struct fq_handle {
     bool hpair = false;
     bool hp_nyq = false;         // the plan's V tables are not all zero (VSS)
     int nvol = 0;                // product volumes per DOF: ngroups, or ceil(ngroups / 2)
-    cpx<double> *X = nullptr;    // [entries][17][2][N][N]
-    cpx<double> *VN = nullptr;   // [contribs][kCPer32]
+    cpx<double> *X = nullptr;     // [entries][17][2][N][N]
+    cpx<float> *VN = nullptr;    // [contribs][kCPer32]
     int *e_vol = nullptr, *t_e0 = nullptr, *t_en = nullptr;
     float *NB = nullptr;         // scratch [chunk][groups][kNbPer32]
     cpx<double> *Cn = nullptr;   // scratch [chunk][targets][kCPer32]
@@ -2388,7 +2419,7 @@ int build_hpair(fq_handle *h) {
     }
     const int nc = int(P.contribs.size());
     if (rc == FQ_SUCCESS && h->hp_nyq && nc > 0) {
-        hipError_t e = hipMalloc(&h->VN, size_t(nc) * kCPer32 * sizeof(cpx<double>));
+        hipError_t e = hipMalloc(&h->VN, size_t(nc) * kCPer32 * sizeof(cpx<float>));
         if (e != hipSuccess) rc = fail(FQ_ERR_ALLOC, std::string("hipMalloc Nyquist tables: ") + hipGetErrorString(e));
         if (rc == FQ_SUCCESS) {
             hipLaunchKernelGGL(k_vtab, dim3(unsigned((kCPer32 + 255) / 256), unsigned(nc)), dim3(256), 0, 0, h->VN,
@@ -2492,7 +2523,7 @@ template <int N, typename T> int run_chunk(fq_handle *h, const Scratch &sc, cons
                 post(7);
                 if (h->hpair && h->hp_nyq) {
                     pre();
-                    hipLaunchKernelGGL(k_nyqb, dim3(unsigned(int64_t(D) * ntgt)), dim3(128), 0, st, sc.NB, h->VN,
+                    hipLaunchKernelGGL(k_nyqb, dim3(unsigned(int64_t(D) * ntgt)), dim3(256), 0, st, sc.NB, h->VN,
                                        h->c_group, h->t_c0, h->t_cn, sc.C, ngroups, ntgt);
                     post(9);
                 }
@@ -2572,7 +2603,15 @@ template <int N, typename T> int run_chunk(fq_handle *h, const Scratch &sc, cons
     }
     post(4);
     pre();
-    hipLaunchKernelGGL(k_out<N>, dim3(pblocks(u)), pt, 0, st, Y, ptr, dof, ntgt, u);
+    bool out1 = false;
+    if constexpr (N == 32) {
+        // group pairs: gain + i loss in one volume, one transform per plane
+        if (h->hpair) {
+            hipLaunchKernelGGL(k_out1<N>, dim3(pblocks(u)), pt, 0, st, Y, ptr, dof, ntgt, u);
+            out1 = true;
+        }
+    }
+    if (!out1) hipLaunchKernelGGL(k_out<N>, dim3(pblocks(u)), pt, 0, st, Y, ptr, dof, ntgt, u);
     post(5);
     if (h->flags & FQ_FLAG_PROJECT) {
         // per batch, right after k_out: the batch's Q is still in L2
