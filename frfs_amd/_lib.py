@@ -30,6 +30,7 @@ FLAG_NO_PACK = 8
 FLAG_NO_SCRATCH = 16
 FLAG_ACCUM_QUAD = 32
 FLAG_NO_HPAIR = 64
+FLAG_OVERLAP = 128
 NOALLOC_SCRATCH_BYTES = 256 << 20
 
 _dp = C.POINTER(C.c_double)

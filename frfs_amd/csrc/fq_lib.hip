@@ -915,7 +915,7 @@ __device__ __forceinline__ void plane_inv1(cpx<T> *Lp, cpx<T> *v, int c, bool ac
 // keeps the sums that give DFT-(b_g) on the three Nyquist planes
 // (K' = N/2: sum over vz of (-1)^vz b; J' = N/2: over vy; I' = N/2: over vx),
 // written to NB as per-workgroup partials for k_nyqb:
-//   NB[slot][g] = { K: [G][vx][vy] (signed by (-1)^G), J: [vz][vx], I: [vz][vy] }.
+//   NB[slot][g] = { K: [G][vy][vx] (signed by (-1)^G), J: [vz][vx], I: [vz][vy] }.
 constexpr int kNbPer32 = 6 * 32 * 32;             // floats of NB per (slot, group), N = 32
 
 template <int N, typename T, bool MULTI, bool PACK, bool CONJ, bool HP = false>
@@ -1057,29 +1057,42 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
 #pragma unroll
             for (int j = 0; j < N; ++j) sj += (j & 1) ? -acc[j].y : acc[j].y;
             if (act) nb[4 * N * N + vz * N + c] = sj;                   // J: (vz, vx = c)
-            T *bl = reinterpret_cast<T *>(lds);                        // [S][N][N + 1]
+            // b planes as [k][vy][vx] with rows of BR = N + 4 floats: lane c's
+            // stores for one vy are consecutive, a row is 16-byte aligned, and
+            // 16 rows read by 16 lanes start on 16 different 4-bank groups
+            constexpr int BR = N + 4;
+            T *bl = reinterpret_cast<T *>(lds);                        // [S][N][BR]
+            static_assert(PG::S * N * BR * sizeof(T) <= PG::S * PG::PLANE * sizeof(cpx<T>), "fits the planes");
             if (act) {
 #pragma unroll
-                for (int j = 0; j < N; ++j) bl[(k * N + c) * (N + 1) + j] = acc[j].y;
+                for (int j = 0; j < N; ++j) bl[(k * N + j) * BR + c] = acc[j].y;
             }
             __syncthreads();
             const int tid = int(threadIdx.x);
             const T sgn = (G & 1) ? T(-1) : T(1);                      // (-1)^vz on vz = G + 4k
+            static_assert(N * N == 4 * P1::THREADS && P1::THREADS == PG::S * N, "thread map");
+            {
+                // K: four consecutive vx of one vy per thread, summed over the S planes
+                const int y = tid / (N / 4), x0 = 4 * (tid % (N / 4));
+                float4 sk = {0, 0, 0, 0};
 #pragma unroll
-            for (int r = 0; r < N * N / P1::THREADS; ++r) {
-                const int idx = tid + r * P1::THREADS, x = idx / N, y = idx % N;
-                T s = 0;
-#pragma unroll
-                for (int kk = 0; kk < PG::S; ++kk) s += bl[(kk * N + x) * (N + 1) + y];
-                nb[G * N * N + idx] = sgn * s;                          // K: (vx, vy)
+                for (int kk = 0; kk < PG::S; ++kk) {
+                    const float4 q = *reinterpret_cast<const float4 *>(bl + (kk * N + y) * BR + x0);
+                    sk.x += q.x; sk.y += q.y; sk.z += q.z; sk.w += q.w;
+                }
+                *reinterpret_cast<float4 *>(nb + G * N * N + y * N + x0) =
+                    float4{sgn * sk.x, sgn * sk.y, sgn * sk.z, sgn * sk.w};       // K: [G][vy][vx]
             }
             {
-                static_assert(P1::THREADS == PG::S * N, "one (plane, vy) per thread");
+                // I: the row (plane kk, vy) summed over vx with the sign (-1)^vx
                 const int kk = tid / N, y = tid % N;
-                T s = 0;
+                T sI = 0;
 #pragma unroll
-                for (int x = 0; x < N; ++x) s += (x & 1) ? -bl[(kk * N + x) * (N + 1) + y] : bl[(kk * N + x) * (N + 1) + y];
-                nb[5 * N * N + (G + 4 * kk) * N + y] = s;               // I: (vz, vy)
+                for (int x4 = 0; x4 < N / 4; ++x4) {
+                    const float4 q = *reinterpret_cast<const float4 *>(bl + (kk * N + y) * BR + 4 * x4);
+                    sI += (q.x - q.y) + (q.z - q.w);
+                }
+                nb[5 * N * N + (G + 4 * kk) * N + y] = sI;               // I: (vz, vy)
             }
             __syncthreads();                                           // LDS reads done
         }
@@ -1600,8 +1613,9 @@ __global__ void k_vtab(cpx<float> *VN, const cpx<double> *W, int N, int R) {
 // B_K = (Z + conj Z~)/2 and B_J = (Z - conj Z~)/(2i).
 // One workgroup per (slot, target), four waves, four contributions per
 // round: wave w transforms contribution 4i + w, Z on lanes 0-31 and s_I on
-// lanes 32-63 (one plane per half-wave, one row per lane, in-register
-// 32-point transforms and an LDS transpose), leaving both spectra in LDS.
+// lanes 32-63 (one plane per half-wave, one pencil per lane read as
+// coalesced row segments, in-register 32-point transforms and an LDS
+// transpose), leaving both spectra in LDS as [p][q].
 // Then every thread accumulates its own output points (k/256 of the 2112)
 // over the round's four contributions, with coalesced weight reads, in FP64.
 // ---------------------------------------------------------------------------
@@ -1624,37 +1638,32 @@ __global__ __launch_bounds__(256) void k_nyqb(const float *__restrict__ NB, cons
     for (int base = 0; base < cn; base += 4) {
         if (base + wave < cn) {                            // wave-uniform
             const float *nb = NB + (int64_t(slot) * ngroups + c_group[c0 + base + wave]) * kNbPer32;
+            // lane c takes index c of the first axis and w runs over the second,
+            // so every load reads one contiguous row segment across the lanes:
+            // Z(vx = c, w) = s_K(vx, vy = w) + i s_J(vx, vz = w), or s_I(vy = c, vz = w)
             cpx<float> z[N];
             if (hw == 0) {
-                // row vx = c: s_K(vx, vy = w) + i s_J(vx, vz = w)
 #pragma unroll
-                for (int w4 = 0; w4 < N / 4; ++w4) {
-                    float4 sk = *reinterpret_cast<const float4 *>(nb + c * N + 4 * w4);
-#pragma unroll
-                    for (int G = 1; G < 4; ++G) {
-                        const float4 q = *reinterpret_cast<const float4 *>(nb + G * N * N + c * N + 4 * w4);
-                        sk.x += q.x; sk.y += q.y; sk.z += q.z; sk.w += q.w;
-                    }
-                    const float sk4[4] = {sk.x, sk.y, sk.z, sk.w};
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) z[4 * w4 + j] = {sk4[j], nb[4 * N * N + (4 * w4 + j) * N + c]};
+                for (int w = 0; w < N; ++w) {
+                    const float sk = (nb[w * N + c] + nb[N * N + w * N + c]) +
+                                     (nb[2 * N * N + w * N + c] + nb[3 * N * N + w * N + c]);
+                    z[w] = {sk, nb[4 * N * N + w * N + c]};
                 }
             } else {
-                // row vy = c: s_I(vy, vz = w)
 #pragma unroll
                 for (int w = 0; w < N; ++w) z[w] = {nb[5 * N * N + w * N + c], 0.f};
             }
-            fq::LFFT<N, -1>::run(z);                       // along w
+            fq::LFFT<N, -1>::run(z);                       // along w: z[q]
             plane_sync<32>();
 #pragma unroll
-            for (int w = 0; w < N; ++w) L[c * ROW + w] = z[w];
+            for (int q = 0; q < N; ++q) L[c * ROW + q] = z[q];
             plane_sync<32>();
 #pragma unroll
             for (int i = 0; i < N; ++i) z[i] = L[i * ROW + c];
-            fq::LFFT<N, -1>::run(z);                       // along the rows: z[p] = spectrum(p, q = c)
+            fq::LFFT<N, -1>::run(z);                       // along the first axis: z[p] = spectrum(p, q = c)
             plane_sync<32>();
 #pragma unroll
-            for (int p = 0; p < N; ++p) L[p * ROW + c] = z[p];
+            for (int p = 0; p < N; ++p) L[p * ROW + c] = z[p];     // [p][q]
         }
         __syncthreads();
         for (int w2 = 0; w2 < 4 && base + w2 < cn; ++w2) {
@@ -1922,11 +1931,19 @@ __global__ __launch_bounds__(kPlaneThreads, 2) void k_out(const cpx<double> *Y, 
 // ---------------------------------------------------------------------------
 // k_out1: the output stage of the group-pair path.  Y = Yg + i Yl with the
 // gain and loss spectra Hermitian, so DFT+_{x,y}(Y) = gain + i loss with
-// both real: one complex transform per plane, Q = Re - f Im.
+// both real: one complex transform per plane, Q = Re - f Im.  With MOM the
+// unit also writes its plane's sums of Q {1, c, |c|^2} (FP64) to
+// mom[slot][t][vz][5] for k_project_m.
 // ---------------------------------------------------------------------------
-template <int N>
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+template <int N, bool MOM>
 __global__ __launch_bounds__(kPlaneThreads, 2) void k_out1(const cpx<double> *Y, AosoaPtrs ptr, Dof dof, int ntgt,
-                                                 int64_t nunits) {
+                                                 int64_t nunits, double *mom, double L) {
     using G = Geo<N>;
     constexpr int E = G::E;
     constexpr int UPB = kPlaneThreads / G::LPP;
@@ -1943,16 +1960,37 @@ __global__ __launch_bounds__(kPlaneThreads, 2) void k_out1(const cpx<double> *Y,
     const bool act = valid && c < N;
     cpx<double> v[E];
     plane_inv64<N>(Y + (int64_t(slot) * ntgt + t) * G::VOL + int64_t(vz) * G::N2, lds[u], v, c, half, act);
+    double m[5] = {0, 0, 0, 0, 0};
     if (act) {
         const double *f = ptr.f[t];
         double *q = ptr.q[t];
         const int64_t b = dof.base(slot);
+        const double h = 2.0 * L / N, c0 = -L + L / N;
+        const double cx = c0 + h * c, cz = c0 + h * vz;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int vy = half + 2 * e;
             const int64_t vv = (int64_t(c) * N + vy) * N + vz;
             const int64_t a = b + vv * dof.soasz;
-            q[a] = v[e].x - f[a] * v[e].y;
+            const double qv = v[e].x - f[a] * v[e].y;
+            q[a] = qv;
+            if constexpr (MOM) {
+                const double cy = c0 + h * vy;
+                m[0] += qv;
+                m[1] += qv * cx;
+                m[2] += qv * cy;
+                m[3] += qv * cz;
+                m[4] += qv * (cx * cx + cy * cy + cz * cz);
+            }
+        }
+    }
+    if constexpr (MOM) {
+        static_assert(G::LPP == 64, "one plane per wave");
+#pragma unroll
+        for (int k = 0; k < 5; ++k) m[k] = wave_sum(m[k]);
+        if (valid && lt == 0) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) mom[((int64_t(slot) * ntgt + t) * N + vz) * 5 + k] = m[k];
         }
     }
 }
@@ -2038,6 +2076,57 @@ __global__ __launch_bounds__(256) void k_project(AosoaPtrs ptr, Dof dof, int ns,
     }
 }
 
+// The same projection with the moments of Q taken from the per-plane sums
+// that k_out1 wrote (mom[slot][s][vz][5], N = 32 planes summed in order by
+// the first wave), so Q is read once and written once.
+__global__ __launch_bounds__(256) void k_project_m(AosoaPtrs ptr, Dof dof, int ns, int N, double L,
+                                                   const double *ginv, const double *mass, const double *mom, int D) {
+    const int per = (D + 7) / 8;
+    const int slot = int(blockIdx.x & 7) * per + int(blockIdx.x >> 3);
+    if (slot >= D) return;
+    const int64_t b = dof.base(slot);
+    const int64_t n = int64_t(N) * N * N;
+    const int K = ns + 4;
+    __shared__ double aq[kProjMaxRows];
+    if (threadIdx.x < 64) {
+        // lanes 0..N-1 hold one plane's sums, summed across the wave
+        double sm[kProjMaxRows];
+#pragma unroll
+        for (int i = 0; i < kProjMaxRows; ++i) sm[i] = 0;
+        for (int s = 0; s < ns; ++s) {
+            double m[5] = {0, 0, 0, 0, 0};
+            if (int(threadIdx.x) < N) {
+#pragma unroll
+                for (int k = 0; k < 5; ++k) m[k] = mom[((int64_t(slot) * ns + s) * N + threadIdx.x) * 5 + k];
+            }
+#pragma unroll
+            for (int k = 0; k < 5; ++k) m[k] = wave_sum(m[k]);
+            sm[s] = m[0];
+            for (int k = 1; k < 5; ++k) sm[ns + k - 1] += mass[s] * m[k];
+        }
+        if (threadIdx.x == 0)
+            for (int i = 0; i < K; ++i) aq[i] = sm[i];
+    }
+    __syncthreads();
+    double lam[kProjMaxRows];
+    for (int i = 0; i < K; ++i) {
+        double acc = 0;
+        for (int k = 0; k < K; ++k) acc += ginv[i * K + k] * aq[k];
+        lam[i] = acc;
+    }
+    for (int s = 0; s < ns; ++s) {
+        double *q = ptr.q[s];
+        const double ms = mass[s];
+        for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
+            double ph[5];
+            phis(v, N, L, ph);
+            const double corr = lam[s] + ms * (lam[ns] * ph[1] + lam[ns + 1] * ph[2] + lam[ns + 2] * ph[3] +
+                                               lam[ns + 3] * ph[4]);
+            q[b + v * dof.soasz] -= corr;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -2071,6 +2160,7 @@ struct Scratch {
     void *FJ = nullptr;
     float *NB = nullptr;          // group pairs: Nyquist sums of Im G_g (k_prod1 -> k_nyqb)
     cpx<double> *C = nullptr;     // group pairs: Nyquist corrections (k_nyqb -> k_accum_h)
+    double *MOM = nullptr;        // group pairs with projection: per-plane sums of Q (k_out1 -> k_project_m)
 };
 
 struct fq_handle {
@@ -2111,15 +2201,20 @@ struct fq_handle {
     int *e_vol = nullptr, *t_e0 = nullptr, *t_en = nullptr;
     float *NB = nullptr;         // scratch [chunk][groups][kNbPer32]
     cpx<double> *Cn = nullptr;   // scratch [chunk][targets][kCPer32]
+    double *MOM = nullptr;       // scratch [chunk][targets][N][5]
     // scratch
     cpx<double> *F = nullptr;
     void *Z = nullptr;           // also T0 and Y
     void *Pg = nullptr;
     size_t scratch = 0;
+    // FQ_FLAG_OVERLAP: two scratch sets; odd batches on a second stream
+    int nsets = 1;
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // per-DOF bytes of each scratch region (for carving the static array of
     // fq_eval_noalloc) and that array's device address on this handle's device
     size_t per_dof_F = 0, per_dof_Z = 0, per_dof_P = 0, per_dof_F32 = 0, per_dof_NY = 0, per_dof_FJ = 0;
-    size_t per_dof_NB = 0, per_dof_C = 0;
+    size_t per_dof_NB = 0, per_dof_C = 0, per_dof_M = 0;
     unsigned char *noalloc = nullptr;
     int cus = 256;               // compute units of the device
     // optional per-kernel HIP-event timing
@@ -2607,7 +2702,10 @@ template <int N, typename T> int run_chunk(fq_handle *h, const Scratch &sc, cons
     if constexpr (N == 32) {
         // group pairs: gain + i loss in one volume, one transform per plane
         if (h->hpair) {
-            hipLaunchKernelGGL(k_out1<N>, dim3(pblocks(u)), pt, 0, st, Y, ptr, dof, ntgt, u);
+            if (h->flags & FQ_FLAG_PROJECT)
+                hipLaunchKernelGGL((k_out1<N, true>), dim3(pblocks(u)), pt, 0, st, Y, ptr, dof, ntgt, u, sc.MOM, P.L);
+            else
+                hipLaunchKernelGGL((k_out1<N, false>), dim3(pblocks(u)), pt, 0, st, Y, ptr, dof, ntgt, u, nullptr, P.L);
             out1 = true;
         }
     }
@@ -2616,6 +2714,10 @@ template <int N, typename T> int run_chunk(fq_handle *h, const Scratch &sc, cons
     if (h->flags & FQ_FLAG_PROJECT) {
         // per batch, right after k_out: the batch's Q is still in L2
         pre();
+        if (h->hpair)
+            hipLaunchKernelGGL(k_project_m, dim3(unsigned(8 * ((D + 7) / 8))), dim3(256), 0, st, ptr, dof, ntgt, N,
+                               P.L, h->ginv, h->pmass, sc.MOM, int(D));
+        else
         hipLaunchKernelGGL(k_project, dim3(unsigned(8 * ((D + 7) / 8))), dim3(256), 0, st, ptr, dof, ntgt, N, P.L,
                            h->ginv, h->pmass, int(D));
         post(6);
@@ -2632,11 +2734,15 @@ int dispatch_prec(fq_handle *h, const Scratch &sc, const AosoaPtrs &ptr, const D
                                          : run_chunk<N, double>(h, sc, ptr, dof, D, prof, st);
 }
 
-// Batches of at most `chunk` DOFs on the scratch `sc`; prof = per-kernel
-// HIP-event brackets (which create events, so the no-alloc entry point
-// passes false).
-int eval_impl(fq_handle *h, const Scratch &sc, int64_t chunk, bool prof, const AosoaPtrs &ptr, int32_t nupts,
-              int64_t ldim, int32_t neles, int64_t first_dof, int64_t ndof, void *stream) {
+// Batches of at most `chunk` DOFs on the scratch sets `sc[0..nsets)`; prof =
+// per-kernel HIP-event brackets (which create events, so the no-alloc entry
+// point passes false).  With two sets, batch i runs on set i % 2: even
+// batches on the caller's stream, odd ones on the handle's second stream,
+// forked from and joined back to the caller's stream by events, so one
+// batch's memory-bound stages can share the device with the next batch's
+// product kernel.  Batches i and i + 2 share a set and a stream.
+int eval_impl(fq_handle *h, const Scratch *sc, int nsets, int64_t chunk, bool prof, const AosoaPtrs &ptr,
+              int32_t nupts, int64_t ldim, int32_t neles, int64_t first_dof, int64_t ndof, void *stream) {
     if (!h) return fail(FQ_ERR_INVALID, "null handle");
     if (nupts <= 0 || neles <= 0 || ndof < 0 || first_dof < 0)
         return fail(FQ_ERR_INVALID, "invalid DOF range or shape");
@@ -2669,18 +2775,30 @@ int eval_impl(fq_handle *h, const Scratch &sc, int64_t chunk, bool prof, const A
             per = ((ndof + nbatch - 1) / nbatch + align - 1) / align * align;
         }
     }
-    for (int64_t c0 = 0; c0 < ndof; c0 += per) {
+    const bool two = nsets > 1 && h->st2 && (ndof + per - 1) / per > 1;
+    if (two) {
+        HIPCHK(hipEventRecord(h->ev_fork, st));
+        HIPCHK(hipStreamWaitEvent(h->st2, h->ev_fork, 0));
+    }
+    int64_t i = 0;
+    for (int64_t c0 = 0; c0 < ndof; c0 += per, ++i) {
         const int D = int(std::min<int64_t>(per, ndof - c0));
         Dof dof{first_dof + c0, nupts, 32, ldim, n};
+        const Scratch &s = sc[two ? i % 2 : 0];
+        const hipStream_t bs = two && (i % 2) ? h->st2 : st;
         int rc;
         switch (h->P.N) {
-        case 8: rc = dispatch_prec<8>(h, sc, ptr, dof, D, prof, st); break;
-        case 16: rc = dispatch_prec<16>(h, sc, ptr, dof, D, prof, st); break;
-        case 24: rc = dispatch_prec<24>(h, sc, ptr, dof, D, prof, st); break;
-        case 32: rc = dispatch_prec<32>(h, sc, ptr, dof, D, prof, st); break;
+        case 8: rc = dispatch_prec<8>(h, s, ptr, dof, D, prof, bs); break;
+        case 16: rc = dispatch_prec<16>(h, s, ptr, dof, D, prof, bs); break;
+        case 24: rc = dispatch_prec<24>(h, s, ptr, dof, D, prof, bs); break;
+        case 32: rc = dispatch_prec<32>(h, s, ptr, dof, D, prof, bs); break;
         default: rc = fail(FQ_ERR_UNSUPPORTED, "grid size not built");
         }
         if (rc != FQ_SUCCESS) return rc;
+    }
+    if (two) {
+        HIPCHK(hipEventRecord(h->ev_join, h->st2));
+        HIPCHK(hipStreamWaitEvent(st, h->ev_join, 0));
     }
     return FQ_SUCCESS;
 }
@@ -2690,10 +2808,24 @@ int eval_own(fq_handle *h, const AosoaPtrs &ptr, int32_t nupts, int64_t ldim, in
              int64_t ndof, void *stream) {
     if (h->flags & FQ_FLAG_NO_SCRATCH)
         return fail(FQ_ERR_INVALID, "handle created with FQ_FLAG_NO_SCRATCH has no scratch; use fq_eval_noalloc");
-    Scratch sc;
-    sc.F = h->F; sc.Z = h->Z; sc.Pg = h->Pg; sc.F32 = h->F32; sc.NY = h->NY; sc.FJ = h->FJ;
-    sc.NB = h->NB; sc.C = h->Cn;
-    return eval_impl(h, sc, h->chunk, h->prof, ptr, nupts, ldim, neles, first_dof, ndof, stream);
+    // set s of each region starts s * (per-DOF bytes * chunk) into it
+    Scratch sc[2];
+    for (int k = 0; k < h->nsets; ++k) {
+        const size_t c = size_t(h->chunk) * k;
+        auto at = [&](void *base, size_t per) -> void * {
+            return base ? static_cast<void *>(static_cast<unsigned char *>(base) + per * c) : nullptr;
+        };
+        sc[k].F = static_cast<cpx<double> *>(at(h->F, h->per_dof_F));
+        sc[k].Z = at(h->Z, h->per_dof_Z);
+        sc[k].Pg = at(h->Pg, h->per_dof_P);
+        sc[k].F32 = static_cast<cpx<float> *>(at(h->F32, h->per_dof_F32));
+        sc[k].NY = at(h->NY, h->per_dof_NY);
+        sc[k].FJ = at(h->FJ, h->per_dof_FJ);
+        sc[k].NB = static_cast<float *>(at(h->NB, h->per_dof_NB));
+        sc[k].C = static_cast<cpx<double> *>(at(h->Cn, h->per_dof_C));
+        sc[k].MOM = static_cast<double *>(at(h->MOM, h->per_dof_M));
+    }
+    return eval_impl(h, sc, h->nsets, h->chunk, h->prof, ptr, nupts, ldim, neles, first_dof, ndof, stream);
 }
 
 std::mutex g_noalloc_mu;
@@ -2721,7 +2853,7 @@ size_t noalloc_bytes(const fq_handle *h, int64_t c) {
     const size_t cc = size_t(c);
     return align256(h->per_dof_F * cc) + align256(h->per_dof_Z * cc) + align256(h->per_dof_P * cc) +
            align256(h->per_dof_F32 * cc) + align256(h->per_dof_NY * cc) + align256(h->per_dof_FJ * cc) +
-           align256(h->per_dof_NB * cc) + align256(h->per_dof_C * cc);
+           align256(h->per_dof_NB * cc) + align256(h->per_dof_C * cc) + align256(h->per_dof_M * cc);
 }
 
 // Carves the regions of a batch of c DOFs out of the static scratch at base.
@@ -2743,6 +2875,7 @@ Scratch noalloc_carve(const fq_handle *h, unsigned char *base, int64_t c) {
     sc.FJ = take(h->per_dof_FJ);
     sc.NB = static_cast<float *>(take(h->per_dof_NB));
     sc.C = static_cast<cpx<double> *>(take(h->per_dof_C));
+    sc.MOM = static_cast<double *>(take(h->per_dof_M));
     return sc;
 }
 
@@ -2885,43 +3018,55 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     // group pairs: the Nyquist sums of Im G_g and the per-target corrections
     const size_t per_dof_NB = h->hp_nyq ? size_t(P.ngroups) * kNbPer32 * sizeof(float) : 0;
     const size_t per_dof_C = h->hp_nyq ? size_t(P.ntgt) * kCPer32 * sizeof(cpx<double>) : 0;
+    const size_t per_dof_M = h->hpair && (h->flags & FQ_FLAG_PROJECT) ? size_t(P.ntgt) * N * 5 * sizeof(double) : 0;
     const size_t per_dof = per_dof_F + per_dof_Z + per_dof_P + per_dof_F32 + per_dof_NY + per_dof_FJ +
-                           per_dof_NB + per_dof_C;
+                           per_dof_NB + per_dof_C + per_dof_M;
     h->per_dof_F = per_dof_F; h->per_dof_Z = per_dof_Z; h->per_dof_P = per_dof_P;
     h->per_dof_F32 = per_dof_F32; h->per_dof_NY = per_dof_NY; h->per_dof_FJ = per_dof_FJ;
-    h->per_dof_NB = per_dof_NB; h->per_dof_C = per_dof_C;
+    h->per_dof_NB = per_dof_NB; h->per_dof_C = per_dof_C; h->per_dof_M = per_dof_M;
     // the static scratch of fq_eval_noalloc, resolved with this device current
     h->noalloc = noalloc_resolve(h->device);
     if (rc == FQ_SUCCESS && (h->flags & FQ_FLAG_NO_SCRATCH) && !h->noalloc)
         rc = fail(FQ_ERR_HIP, "hipGetSymbolAddress of the static scratch failed");
+    // two scratch sets (FQ_FLAG_OVERLAP) share the budget; max_chunk bounds each set's batch
+    h->nsets = (h->flags & FQ_FLAG_OVERLAP) && !(h->flags & FQ_FLAG_NO_SCRATCH) ? 2 : 1;
+    const size_t ns = size_t(h->nsets);
     int64_t chunk = prm->max_chunk;
     if (chunk <= 0) {
         const size_t budget = size_t(8) << 30;    // 8 GiB default scratch budget (of 288 GB HBM)
-        chunk = std::max<int64_t>(1, int64_t(budget / per_dof));
+        chunk = std::max<int64_t>(1, int64_t(budget / (per_dof * ns)));
         chunk = std::min<int64_t>(chunk, 4096);
     }
     h->chunk = chunk;
     if (rc == FQ_SUCCESS && (h->flags & FQ_FLAG_NO_SCRATCH)) {
         h->chunk = 0;                      // evaluation only through fq_eval_noalloc
     } else if (rc == FQ_SUCCESS) {
-        hipError_t e1 = hipMalloc(&h->F, per_dof_F * chunk);
-        hipError_t e2 = hipMalloc(&h->Z, per_dof_Z * chunk);
-        hipError_t e3 = hipMalloc(&h->Pg, per_dof_P * chunk);
+        const size_t cs = size_t(chunk) * ns;
+        hipError_t e1 = hipMalloc(&h->F, per_dof_F * cs);
+        hipError_t e2 = hipMalloc(&h->Z, per_dof_Z * cs);
+        hipError_t e3 = hipMalloc(&h->Pg, per_dof_P * cs);
         hipError_t e4 = hipSuccess;
-        if (h->precision == FQ_PREC_MIXED)
-            e4 = hipMalloc(&h->F32, size_t(P.nsrc) * n * sizeof(cpx<float>) * chunk);
+        if (h->precision == FQ_PREC_MIXED) e4 = hipMalloc(&h->F32, per_dof_F32 * cs);
         hipError_t e5 = hipSuccess;
         if (h->pack) {
-            e5 = hipMalloc(&h->NY, per_dof_NY * chunk);
-            if (e5 == hipSuccess) e5 = hipMalloc(&h->FJ, per_dof_FJ * chunk);
+            e5 = hipMalloc(&h->NY, per_dof_NY * cs);
+            if (e5 == hipSuccess) e5 = hipMalloc(&h->FJ, per_dof_FJ * cs);
         }
         if (e5 == hipSuccess && h->hp_nyq) {
-            e5 = hipMalloc(&h->NB, per_dof_NB * chunk);
-            if (e5 == hipSuccess) e5 = hipMalloc(&h->Cn, per_dof_C * chunk);
+            e5 = hipMalloc(&h->NB, per_dof_NB * cs);
+            if (e5 == hipSuccess) e5 = hipMalloc(&h->Cn, per_dof_C * cs);
         }
+        if (e5 == hipSuccess && per_dof_M) e5 = hipMalloc(&h->MOM, per_dof_M * cs);
         if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess || e5 != hipSuccess)
             rc = fail(FQ_ERR_ALLOC, "hipMalloc scratch failed (reduce max_chunk)");
-        h->scratch = per_dof * chunk;
+        h->scratch = per_dof * cs;
+        if (rc == FQ_SUCCESS && h->nsets == 2) {
+            // the second stream keeps the default priority
+            if (hipStreamCreateWithFlags(&h->st2, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess)
+                rc = fail(FQ_ERR_HIP, "second stream or events for FQ_FLAG_OVERLAP");
+        }
     }
     if (rc != FQ_SUCCESS) {
         std::string keep = g_err;
@@ -2974,12 +3119,15 @@ int fq_destroy(fq_handle *h) {
     (void)hipThreadExchangeStreamCaptureMode(&mode);
     for (auto &r : h->ev_rec) { (void)hipEventDestroy(r.second.first); (void)hipEventDestroy(r.second.second); }
     for (auto e : h->ev_pool) (void)hipEventDestroy(e);
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+    if (h->st2) (void)hipStreamDestroy(h->st2);
     for (void *p : {h->ph, (void *)h->item_src, (void *)h->prods, (void *)h->g_p0, (void *)h->g_np,
                     (void *)h->c_group, (void *)h->t_c0, (void *)h->t_cn, (void *)h->l_src, (void *)h->t_l0,
                     (void *)h->t_ln, (void *)h->W, (void *)h->ginv, (void *)h->pmass, (void *)h->F, h->Z,
                     h->Pg, h->zc, (void *)h->F32, h->NY, h->FJ, (void *)h->nyv, (void *)h->zq,
                     (void *)h->pconj, (void *)h->X, (void *)h->VN, (void *)h->e_vol, (void *)h->t_e0,
-                    (void *)h->t_en, (void *)h->NB, (void *)h->Cn})
+                    (void *)h->t_en, (void *)h->NB, (void *)h->Cn, (void *)h->MOM})
         if (p) (void)hipFree(p);
     (void)hipThreadExchangeStreamCaptureMode(&mode);
     delete h;
@@ -3029,7 +3177,8 @@ int fq_eval_noalloc(fq_handle *h, const double *f, double *q, int32_t nupts, int
     AosoaPtrs ptr{};
     ptr.f[0] = f;
     ptr.q[0] = q;
-    return eval_impl(h, noalloc_carve(h, h->noalloc, c), c, false, ptr, nupts, ldim, neles, first_dof, ndof,
+    const Scratch sc = noalloc_carve(h, h->noalloc, c);
+    return eval_impl(h, &sc, 1, c, false, ptr, nupts, ldim, neles, first_dof, ndof,
                      stream);
 }
 

@@ -65,6 +65,8 @@ def _flags(cfg):
         fl |= _lib.FLAG_NO_PACK
     if not cfg.getint('scattering-model', 'group-pairs', 1):
         fl |= _lib.FLAG_NO_HPAIR
+    if cfg.getint('scattering-model', 'overlap', 0):
+        fl |= _lib.FLAG_OVERLAP
     return fl
 
 

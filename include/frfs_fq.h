@@ -81,11 +81,17 @@ enum fq_flags {
                                  workgroup (two workgroups per CU) instead of
                                  eight (one per CU); a schedule candidate of the
                                  autotuner */
-    FQ_FLAG_NO_HPAIR = 64     /* 32^3 grid, FP32 transforms: write one product
+    FQ_FLAG_NO_HPAIR = 64,    /* 32^3 grid, FP32 transforms: write one product
                                  volume per group instead of one per group pair
                                  (the two groups' real parts as one complex
                                  volume, the Hermitian part of the gain spectrum
                                  accumulated on half the rows); for comparison */
+    FQ_FLAG_OVERLAP = 128     /* two scratch sets: consecutive internal batches
+                                 alternate between them, the odd ones on a second
+                                 stream of the handle (forked from and joined
+                                 back to the caller's stream with events), so
+                                 one batch's memory-bound stages can run beside
+                                 the next batch's product kernel */
 };
 
 /* Size of the static device scratch of fq_eval_noalloc (one copy per GPU the
