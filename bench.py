@@ -28,17 +28,17 @@ sys.path.insert(0, ROOT)
 PEAK = {'fp64': 78.6, 'fp32': 157.3}     # MI355X vector TFLOP/s (published)
 HBM_PEAK = 8000.0                          # GB/s
 # HBM bytes per DOF of the dominant (product) kernel at the 32^3 workload
-# (N = 32, M = 12, Nrho = 16, Mpre = 32, 4096 DOFs in batches of 256 (mixed)
-# and 96 (FP64) DOFs), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
-# this script, scaled by what the same counters read for a 1 GiB device copy
-# (FETCH x2.0, WRITE x1.0).  The bench line carries this figure with its
-# source, and only for that shape.
+# (N = 32, M = 12, Nrho = 16, Mpre = 32, 4096 DOFs with 1024-DOF scratch:
+# batches of 840 (mixed, group pairs) and 864 (FP64) DOFs), from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE runs of this script, scaled by what the same
+# counters read for a 1 GiB device copy (FETCH x2.0, WRITE x1.0).  The bench
+# line carries this figure with its source, and only for that shape.
 K_PROD_TRAFFIC = {
     'source': 'profiles/k_prod_vss32_traffic.txt',
     'shape': {'Nv': 32, 'M': 12, 'Nrho': 16, 'Mpre': 32},
-    'per_dof': {'mixed': 27.95e6, 'fp64': 61.64e6},
+    'per_dof': {'mixed': 17.67e6, 'fp64': 61.63e6},
     # the accumulation kernel (HBM-bound), same runs
-    'accum_per_dof': {'mixed': 28.73e6, 'fp64': 62.12e6},
+    'accum_per_dof': {'mixed': 14.25e6, 'fp64': 58.43e6},
 }
 
 
@@ -693,7 +693,7 @@ def main():
         info = sm.plan_info()
         res[mode] = dict(ms=dt * 1e3, rate=a.ndof / dt, prof=prof, info=info, step_ms=step_ms,
                          median_ms=float(np.median(step_ms)), chunk=sm._h.chunk_dofs(),
-                         scratch=sm._h.scratch_bytes(), schedule=sched)
+                         scratch=sm._h.scratch_bytes(), schedule=sched, paths=sm._h.paths())
         qs[mode] = from_aosoa(qout).cpu().numpy()
         del sm, fin, qout
         torch.cuda.empty_cache()
@@ -727,18 +727,20 @@ def main():
              'k_nyq': 0.0, 'k_project': 0.0}
     achieved = share.get(dom, 0.0) * f_alg * dofs_per_launch / dom_avg_s / 1e12
     # what the dominant kernel executes: with pair packing one complex 3D
-    # inverse transform and the x, y passes of one forward transform per
-    # product (5 N^2 passes), 5 N log2 N flops per pass
+    # inverse transform per product (3 N^2 passes) and the x, y passes of one
+    # forward transform per group (2 N^2), or per group pair with group
+    # pairs (N^2 per group); 5 N log2 N flops per pass
     executed = None
     if dom == 'k_prod':
         npr = info['n_groups']
-        executed = npr * 5 * N * N * 5 * N * math.log2(N) * dofs_per_launch / dom_avg_s / 1e12
+        fwd = 1 if 'group_pairs' in r['paths'] else 2
+        executed = npr * (3 + fwd) * N * N * 5 * N * math.log2(N) * dofs_per_launch / dom_avg_s / 1e12
     op_eff = a.ndof / (r['median_ms'] * 1e-3) * f_alg / 1e12
     traffic, traffic_src = None, None
     if (dom == 'k_prod' and {'Nv': N, 'M': a.M, 'Nrho': a.Nrho, 'Mpre': a.Mpre} == K_PROD_TRAFFIC['shape']):
         traffic = K_PROD_TRAFFIC['per_dof']['mixed' if head.startswith('mixed') else 'fp64'] * dofs_per_launch
         traffic_src = (K_PROD_TRAFFIC['source'] + ': FETCH_SIZE + WRITE_SIZE per dispatch of this kernel at '
-                       'this shape (counter runs with 256-DOF mixed and 96-DOF FP64 batches), scaled by a '
+                       'this shape (counter runs with 840-DOF mixed and 864-DOF FP64 batches), scaled by a '
                        '1 GiB copy, per DOF times the DOFs of one launch')
     # live HBM rate of the accumulation kernel (the operator's HBM-bound kernel):
     # its counter bytes per DOF x DOFs per launch / its HIP-event launch time
@@ -819,7 +821,7 @@ def main():
                                   'conservation projection' if head == 'mixedproj' else ''),
                    'mode': head, 'ndof': a.ndof, 'Nv': N, 'M': a.M, 'Nrho': a.Nrho, 'Mpre': a.Mpre,
                    'seed': a.seed, 'chunk_dofs': r['chunk'], 'scratch_GiB': r['scratch'] / 2 ** 30,
-                   'schedule': r['schedule']},
+                   'schedule': r['schedule'], 'paths': r['paths']},
         'fp64': {'value': res['fp64']['rate'], 'ms_per_step': res['fp64']['ms'], 'chunk_dofs': res['fp64']['chunk'],
                  'scratch_GiB': res['fp64']['scratch'] / 2 ** 30,
                  'schedule': res['fp64']['schedule']} if 'fp64' in res else None,

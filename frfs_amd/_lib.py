@@ -31,6 +31,7 @@ FLAG_NO_SCRATCH = 16
 FLAG_ACCUM_QUAD = 32
 FLAG_NO_HPAIR = 64
 FLAG_OVERLAP = 128
+PATHS = ((1, 'fused'), (2, 'one_lane'), (4, 'pair_pack'), (8, 'group_pairs'), (16, 'overlap'))
 NOALLOC_SCRATCH_BYTES = 256 << 20
 
 _dp = C.POINTER(C.c_double)
@@ -91,6 +92,7 @@ SIGNATURES = {
     'fq_profile_read': (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(_i64)]),
     'fq_kernel_name': (C.c_char_p, [C.c_int]),
     'fq_chunk_dofs': (C.c_int, [_vp, C.POINTER(_i64)]),
+    'fq_get_paths': (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     'fq_axnpby': (C.c_int, [_i64, _vp, _i32, C.POINTER(_vp), C.POINTER(C.c_double), _vp]),
     'fq_copy_dgfs': (C.c_int, [_i32, _i32, _i32, _vp, _i64, _i32, _i32, _vp, _i64, _i32, _i32, _vp]),
     'fq_moments': (C.c_int, [_i32, C.c_double, C.c_double, C.c_double, _vp, _i32, _i64, _i32, _i64, _i64,
@@ -241,6 +243,12 @@ class Handle:
         b = C.c_int64()
         check(load().fq_chunk_dofs(self._h, C.byref(b)))
         return b.value
+
+    def paths(self):
+        """Names of the kernel paths this handle runs (fq_get_paths)."""
+        b = C.c_int32()
+        check(load().fq_get_paths(self._h, C.byref(b)))
+        return [name for bit, name in PATHS if b.value & bit]
 
     def profile(self, on=True):
         check(load().fq_profile_enable(self._h, 1 if on else 0))

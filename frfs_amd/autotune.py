@@ -7,7 +7,8 @@ writes the inverse z transforms to HBM; for FP32 transforms on the 24^3 and
 (radix-2 exchange between lanes), and the two items of a product transformed
 as one packed volume or separately; on the 32^3 grid, eight or four DOFs per
 accumulation workgroup sharing each weight row, and one product volume per
-group pair or per group; and the batch the scratch
+group pair or per group; batches on one stream, or alternating between two
+scratch sets and two streams; and the batch the scratch
 budget allows, a larger or a smaller one.
 ``tune`` builds one handle per candidate on the current device, times each on
 the caller's own inputs (one untimed call, then ``reps`` timed calls between
@@ -29,6 +30,7 @@ CANDIDATES = (
     {'fused': 1, 'max_chunk': 0, 'two_lane': 0, 'accum_quad': 1},
     {'fused': 1, 'max_chunk': 0, 'two_lane': 0, 'no_pack': 1},
     {'fused': 1, 'max_chunk': 0, 'two_lane': 0, 'no_hpair': 1},
+    {'fused': 1, 'max_chunk': 512, 'two_lane': 0, 'overlap': 1},
     {'fused': 1, 'max_chunk': 0, 'two_lane': 1},
     {'fused': 0, 'max_chunk': 0, 'two_lane': 0},
 )
@@ -52,7 +54,7 @@ def _bounded(cand, limit):
 def _apply(params, cand):
     c = params.c
     fl = c.flags & ~(_lib.FLAG_UNFUSED | _lib.FLAG_TWO_LANE | _lib.FLAG_NO_PACK | _lib.FLAG_ACCUM_QUAD
-                     | _lib.FLAG_NO_HPAIR)
+                     | _lib.FLAG_NO_HPAIR | _lib.FLAG_OVERLAP)
     if not cand['fused']:
         fl |= _lib.FLAG_UNFUSED
     if cand.get('two_lane'):
@@ -63,6 +65,8 @@ def _apply(params, cand):
         fl |= _lib.FLAG_ACCUM_QUAD
     if cand.get('no_hpair'):
         fl |= _lib.FLAG_NO_HPAIR
+    if cand.get('overlap'):
+        fl |= _lib.FLAG_OVERLAP
     c.flags = fl
     c.max_chunk = cand['max_chunk']
 

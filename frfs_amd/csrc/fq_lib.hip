@@ -3110,6 +3110,16 @@ int fq_chunk_dofs(const fq_handle *h, int64_t *chunk) {
     return FQ_SUCCESS;
 }
 
+int fq_get_paths(const fq_handle *h, int32_t *paths) {
+    if (!h || !paths) return fail(FQ_ERR_INVALID, "null argument");
+    const int N = h->P.N;
+    const bool one_lane = h->fused && (N == 24 || N == 32) && h->precision == FQ_PREC_MIXED && !h->two_lane;
+    *paths = (h->fused ? FQ_PATH_FUSED : 0) | (one_lane ? FQ_PATH_ONE_LANE : 0) |
+             (h->pack ? FQ_PATH_PAIR_PACK : 0) | (h->hpair ? FQ_PATH_GROUP_PAIRS : 0) |
+             (h->nsets > 1 ? FQ_PATH_OVERLAP : 0);
+    return FQ_SUCCESS;
+}
+
 int fq_destroy(fq_handle *h) {
     if (!h) return FQ_SUCCESS;
     // A handle may be released (e.g. by a garbage collector) while another

@@ -218,6 +218,16 @@ int fq_profile_read(fq_handle *h, double *ms, int64_t *launches);
 const char *fq_kernel_name(int k);
 /* DOFs per internal batch (kernel launch sequence). */
 int fq_chunk_dofs(const fq_handle *h, int64_t *chunk);
+/* The kernel paths this handle runs (a bitmask of FQ_PATH_*), chosen at
+ * fq_create from the grid, precision and flags.  No reference counterpart. */
+enum fq_path {
+    FQ_PATH_FUSED = 1,        /* product kernel with the inverse z stage inside */
+    FQ_PATH_ONE_LANE = 2,     /* FP32 product kernel with one lane per pencil */
+    FQ_PATH_PAIR_PACK = 4,    /* the two items of a product as one complex volume */
+    FQ_PATH_GROUP_PAIRS = 8,  /* two groups' real parts as one product volume */
+    FQ_PATH_OVERLAP = 16      /* two scratch sets and two streams */
+};
+int fq_get_paths(const fq_handle *h, int32_t *paths);
 
 /* ---- kernels on either side of Q (stateless, AoSoA banks) -------------- */
 

@@ -22,11 +22,12 @@ def vhs_cfg(Nv=16, Nrho=8, M=6, dev=5.0, omega=0.5, precision='double', project=
 
 def vss_cfg(Nv=16, Nrho=8, M=12, dev=5.0, masses=(39.96e-3,), alpha=None, omega=None,
             dref=None, Mpre=32, precision='double', project=0, max_chunk=0, fused=1, lanes=1, pack=1,
-            hpair=1):
+            hpair=1, overlap=0):
     ns = len(masses)
     sm = {'type': 'vss-gll', 'ssrulepre': 'womersley', 'Mpre': Mpre,
           'precision': precision, 'conservation-projection': project, 'max-chunk': max_chunk,
-          'fused': fused, 'pencil-lanes': lanes, 'pair-pack': pack, 'group-pairs': hpair}
+          'fused': fused, 'pencil-lanes': lanes, 'pair-pack': pack, 'group-pairs': hpair,
+          'overlap': overlap}
     for p in range(ns):
         for q in range(ns):
             pq = f'{p}{q}'

@@ -43,7 +43,8 @@ def test_single_species(backend, M, Nrho):
     f = _rough(fsp.two_maxwellian_batch(vm['cv'], 3, seed=91), 92)
     print(f'[hpair vss-1 32^3 M={M} Nrho={Nrho}]', flush=True)
     (qp,), smp = _run_vss(backend, cp, [f], 1, 3)
-    (qs,), _ = _run_vss(backend, cs, [f], 1, 3)
+    (qs,), sms = _run_vss(backend, cs, [f], 1, 3)
+    assert 'group_pairs' in smp._h.paths() and 'group_pairs' not in sms._h.paths()
     for d in range(3):
         ref = fsp.vss_fs([f[d]], vm, tab, prm)[0]
         e_ps, e_or = rel_l2(qp[d], qs[d]), rel_l2(qp[d], ref)
@@ -85,4 +86,19 @@ def test_batches(backend):
     print('[hpair vss-1 32^3 batches]', flush=True)
     (qa,), _ = _run_vss(backend, cfg, [f], 1, 7)
     (qb,), _ = _run_vss(backend, vss_cfg(32, 2, 12, 6.0, precision='mixed', max_chunk=3), [f], 1, 7)
+    assert np.array_equal(qa, qb)
+
+
+def test_overlap_matches_serial(backend):
+    """Two scratch sets with odd batches on a second stream give the DOFs of
+    the one-stream evaluation bit for bit (batches of 2 DOFs, 7 DOFs)."""
+    from oracle import fastspectral as fsp
+    cfg = vss_cfg(32, 2, 12, 6.0, precision='mixed', max_chunk=2, project=1)
+    vm, prm, tab = oracle_vss(cfg, 1)
+    f = _rough(fsp.two_maxwellian_batch(vm['cv'], 7, seed=99), 100)
+    print('[hpair vss-1 32^3 overlap]', flush=True)
+    (qa,), sma = _run_vss(backend, cfg, [f], 1, 7)
+    (qb,), smb = _run_vss(backend, vss_cfg(32, 2, 12, 6.0, precision='mixed', max_chunk=2, project=1,
+                                           overlap=1), [f], 1, 7)
+    assert 'overlap' in smb._h.paths() and 'overlap' not in sma._h.paths()
     assert np.array_equal(qa, qb)
