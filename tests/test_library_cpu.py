@@ -196,6 +196,106 @@ def test_plan_execution_matches_oracle_vss(ns, pack):
         assert rel_l2(q[t], ref[t]) < 1e-13
 
 
+def _emulate_group_pairs(plan, F, weight_of_contrib, loss_tables, f_src, N):
+    """NumPy execution of the group-pair path of the 32^3 FP32 kernels at any
+    N (complex128): two groups' real parts as one volume, the Hermitian part of
+    the gain accumulated on rows I' <= N/2 from each pencil and its conjugate
+    partner, the imaginary parts entering through their Nyquist-plane sums
+    (k_nyqb), rows I' > N/2 as conjugates, a Hermitian loss and one packed
+    inverse transform of gain + i loss (k_out1)."""
+    n, h = N ** 3, N // 2
+    l0 = np.concatenate((np.arange(0, h), np.arange(-h, 0))).astype(float)
+    E = []
+    for s, th in zip(plan['item_src'], plan['item_theta']):
+        ph = np.exp(1j * (th[0] * l0[:, None, None] + th[1] * l0[None, :, None] + th[2] * l0[None, None, :]))
+        E.append(np.fft.ifftn(ph * F[s].reshape(N, N, N)) * n)
+    ngroups = plan['prod_group'].max() + 1
+    G = [np.zeros((N, N, N), complex) for _ in range(ngroups)]
+    for i1, i2, cf, g in zip(plan['prod_i1'], plan['prod_i2'], plan['prod_coef'], plan['prod_group']):
+        G[g] += cf * E[i1] * E[i2]
+    nvol = (ngroups + 1) // 2
+    Z = [np.fft.fftn(G[2 * u].real + 1j * (G[2 * u + 1].real if 2 * u + 1 < ngroups else 0)) for u in range(nvol)]
+    mi = lambda a: np.roll(a[::-1, ::-1, ::-1], 1, axis=(0, 1, 2))      # a(-xi)
+    sg = (-1.0) ** np.arange(N)
+    # B_g on the three Nyquist planes from the sums of b = Im G (k_prod1 / k_nyqb)
+    BK, BJ, BI = {}, {}, {}
+    for g in range(ngroups):
+        b = G[g].imag                                                    # [vx][vy][vz]
+        BK[g] = np.fft.fft2(np.einsum('xyz,z->xy', b, sg))               # (I', J')
+        BJ[g] = np.fft.fft2(np.einsum('xyz,y->xz', b, sg))               # (I', K')
+        BI[g] = np.fft.fft2(np.einsum('xyz,x->yz', b, sg))               # (J', K')
+    ntgt = plan['contrib_target'].max() + 1
+    out = []
+    for t in range(ntgt):
+        H = np.zeros((N, N, N), complex)
+        for c, (g, tt) in enumerate(zip(plan['contrib_group'], plan['contrib_target'])):
+            if tt != t:
+                continue
+            W = weight_of_contrib(c).reshape(N, N, N)
+            U, V = (W + np.conj(mi(W))) / 2, 1j * (W - np.conj(mi(W))) / 2
+            u, m = divmod(int(g), 2)
+            s_, d_ = Z[u] + np.conj(mi(Z[u])), Z[u] - np.conj(mi(Z[u]))
+            H += U / 2 * s_ if m == 0 else U / 2j * d_
+            # V on the Nyquist set, each point once (k_vtab)
+            H[:, :, h] += V[:, :, h] * BK[g]
+            mj = np.ones(N, bool); mj[h] = False
+            H[:, h, mj] += (V[:, h, :] * BJ[g])[:, mj]
+            H[h][np.ix_(mj, mj)] += (V[h] * BI[g])[np.ix_(mj, mj)]
+        # rows I' <= N/2 evaluated, the others their conjugates
+        H[h + 1:] = 0
+        H[h + 1:] = np.conj(mi(H))[h + 1:]
+        Lt = np.zeros(n, complex)
+        for j, (s, tt) in enumerate(zip(plan['loss_src'], plan['loss_target'])):
+            if tt == t:
+                Lt += loss_tables[j] * F[s]
+        Lt = Lt.reshape(N, N, N)
+        Hl = (Lt + np.conj(mi(Lt))) / 2
+        y = np.fft.ifftn(H + 1j * Hl) * n
+        out.append(y.real.ravel() - f_src[t] * y.imag.ravel())
+    return out
+
+
+@pytest.mark.parametrize('ns', [1, 2])
+def test_group_pair_execution_matches_oracle_vss(ns):
+    """The group-pair restatement (used by the 32^3 FP32 kernels) against the
+    oracle to 1e-13 on inputs with multiplicative noise, where the
+    Nyquist-plane terms are a visible part of the gain."""
+    m0 = ARGON['molarMass0']
+    cfg = vss_cfg(8, 3, 12, 4.0, masses=(m0, 2 * m0)[:ns], Mpre=32)
+    p, _ = _vss_params(cfg, ns)
+    plan = _lib.Plan(p).export()
+    vm, prm, tab = oracle_vss(cfg, ns)
+    N, n = 8, 512
+    rng = np.random.default_rng(40)
+    fs = [fsp.two_maxwellian_batch(vm['cv'], 1, seed=30 + s, mass=vm['masses'][s])[0] for s in range(ns)]
+    fs = [f * (1 + 0.3 * rng.standard_normal(f.shape)) for f in fs]
+    F = [np.fft.fftn(f.reshape(N, N, N)).ravel() / n for f in fs]
+    key = {(int(s), *map(float, th)): i for i, (s, th) in enumerate(zip(plan['item_src'], plan['item_theta']))}
+    prod_of = {}
+    for g, (i1, i2) in enumerate(zip(plan['prod_i1'], plan['prod_i2'])):
+        prod_of[(min(i1, i2), max(i1, i2))] = plan['prod_group'][g]
+    contrib_of = {(int(g), int(t)): c for c, (g, t) in enumerate(zip(plan['contrib_group'], plan['contrib_target']))}
+    W = {}
+    ms = p.c.masses
+    for pp in range(ns):
+        for qq in range(ns):
+            kq, kp = ms[qq] / (ms[pp] + ms[qq]), ms[pp] / (ms[pp] + ms[qq])
+            for x in range(vm['Nrho']):
+                c = np.pi / vm['L'] * vm['qz'][x]
+                for y in range(vm['M']):
+                    s = [c * vm['sz'][y, d] for d in range(3)]
+                    i1 = key[(pp, s[0] * kq, s[1] * kq, s[2] * kq)]
+                    i2 = key[(qq, -(s[0] * kp), -(s[1] * kp), -(s[2] * kp))]
+                    cidx = contrib_of[(prod_of[(min(i1, i2), max(i1, i2))], pp)]
+                    pq = f'{pp}{qq}'
+                    W[cidx] = W.get(cidx, 0) + prm['prefac'][pq] * (vm['qw'][x] * vm['sw'] / n) * tab['b'][pq][x, y]
+    Lt = [prm['prefac'][f'{t}{s}'] * tab['c'][f'{t}{s}'] for s, t in zip(plan['loss_src'], plan['loss_target'])]
+    q = _emulate_group_pairs(plan, F, lambda c: W[c], Lt, fs, N)
+    ref = fsp.vss_fs(fs, vm, tab, prm)
+    for t in range(ns):
+        assert rel_l2(q[t], ref[t]) < 1e-13
+
+
 def test_invalid_parameters_raise():
     p, _ = _vhs_params(vhs_cfg(16, 8, 6, 5.0))
     p.c.Nv = 15
