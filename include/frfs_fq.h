@@ -182,7 +182,7 @@ int fq_eval(fq_handle *h, const double *f, double *q,
  * handle's scratch: the DOF range is evaluated in batches of as many DOFs as
  * fit there.  Returns FQ_ERR_UNSUPPORTED when one DOF of the plan needs
  * more than FQ_NOALLOC_SCRATCH_BYTES (e.g. a 32^3 two-species FP64 plan is
- * not served; 32^3 single-species mixed takes 9 DOFs per batch).  Works on
+ * not served; 32^3 single-species mixed takes 13 DOFs per batch).  Works on
  * any single-species handle, including one created with
  * FQ_FLAG_NO_SCRATCH.  Every call on a GPU shares that GPU's array, which
  * keeps the last call's contents (every kernel writes what it later reads,
