@@ -187,6 +187,7 @@ def bench_multi(a):
     species 1 taking a share x ~ U(0.2, 0.8) of the number density.  One
     DOF-eval is Q_p for one species at one DOF, all its pairs included."""
     import torch
+    from frfs_amd.autotune import ONE_STREAM
     from frfs_amd.backend import HIPBackend
     from frfs_amd.inifile import Inifile
     from frfs_amd.inputs import from_aosoa, mixture_device, mixture_params, to_aosoa
@@ -227,7 +228,7 @@ def bench_multi(a):
             to_aosoa(f, fin[s])
         torch.cuda.synchronize()
         # the schedule autotuner (untimed) before the warmup steps
-        sched = model.autotune(*fin, *qout)['choice'] if (timed and a.autotune) else None
+        sched = model.autotune(*fin, *qout, candidates=ONE_STREAM)['choice'] if (timed and a.autotune) else None
         for _ in range(a.warmup if timed else 1):
             model.fs_batch(*fin, *qout)
         torch.cuda.synchronize()
@@ -631,6 +632,7 @@ def main():
     cpu, ref = (None, {}) if a.no_cpu_baseline else cpu_baseline(a, fpar)
 
     import torch
+    from frfs_amd.autotune import ONE_STREAM
     from frfs_amd.backend import HIPBackend
     from frfs_amd.inputs import from_aosoa, mixture_device, to_aosoa
     from frfs_amd.scattering import DGFSBiVSSGLLScatteringModel
@@ -671,7 +673,7 @@ def main():
             continue
         # the schedule autotuner (untimed): candidates timed on these arrays,
         # the fastest kept for the timed steps
-        sched = sm.autotune(fin, qout)['choice'] if a.autotune else None
+        sched = sm.autotune(fin, qout, candidates=ONE_STREAM)['choice'] if a.autotune else None
         for _ in range(a.warmup):
             sm.fs_batch(fin, qout)
         torch.cuda.synchronize()

@@ -35,6 +35,11 @@ CANDIDATES = (
     {'fused': 0, 'max_chunk': 0, 'two_lane': 0},
 )
 
+# the candidates that run every internal batch on the caller's stream: with two
+# streams the kernels of consecutive batches share the device, so per-kernel
+# launch times (and a roofline taken from them) no longer describe one kernel
+ONE_STREAM = tuple(c for c in CANDIDATES if not c.get('overlap'))
+
 _CACHE = {}
 
 

@@ -115,12 +115,14 @@ class DGFSScatteringModel(metaclass=ABCMeta):
         return self._h.info()
 
     # -- schedule autotuning (shared by every model) --------------------------
-    def _tune(self, run, ndof):
+    def _tune(self, run, ndof, candidates=None):
         info = self._h.info()
+        cands = _autotune.CANDIDATES if candidates is None else tuple(candidates)
         key = (type(self).__name__, self._params.c.precision, self._params.c.flags & _lib.FLAG_PROJECT,
-               self._params.c.Nv, info['n_items'], info['n_groups'], info['n_targets'], int(ndof))
+               self._params.c.Nv, info['n_items'], info['n_groups'], info['n_targets'], int(ndof),
+               tuple(tuple(sorted(c.items())) for c in cands))
         self._h.close()
-        self._h, self.tune_report = _autotune.tune(self._params, run, key)
+        self._h, self.tune_report = _autotune.tune(self._params, run, key, candidates=cands)
         self._tuned = True
 
     def _maybe_tune(self, run, ndof):
@@ -171,11 +173,12 @@ class DGFSVHSGLLScatteringModel(DGFSScatteringModel):
         self._maybe_tune(run, ndof)
         run(self._h)
 
-    def autotune(self, d_arr_in, d_arr_out, first_dof=0, ndof=None):
-        """Time the candidate schedules on these arrays and keep the fastest
-        (``d_arr_out`` is overwritten).  Returns the tuning report."""
+    def autotune(self, d_arr_in, d_arr_out, first_dof=0, ndof=None, candidates=None):
+        """Time the candidate schedules (default ``autotune.CANDIDATES``) on
+        these arrays and keep the fastest (``d_arr_out`` is overwritten).
+        Returns the tuning report."""
         run, ndof = self._runner(d_arr_in, d_arr_out, first_dof, ndof)
-        self._tune(run, ndof)
+        self._tune(run, ndof, candidates)
         return self.tune_report
 
 
@@ -207,9 +210,9 @@ class DGFSVHSGLLNodalScatteringModel(DGFSVHSGLLScatteringModel):
         self._maybe_tune(run, ndof)
         run(self._h)
 
-    def autotune(self, d_arr_in1, d_arr_in2, d_arr_out, first_dof=0, ndof=None):
+    def autotune(self, d_arr_in1, d_arr_in2, d_arr_out, first_dof=0, ndof=None, candidates=None):
         run, ndof = self._runner(d_arr_in1, d_arr_in2, d_arr_out, first_dof, ndof)
-        self._tune(run, ndof)
+        self._tune(run, ndof, candidates)
         return self.tune_report
 
 
@@ -316,9 +319,10 @@ class DGFSBiVSSGLLScatteringModel(DGFSBiScatteringModel):
         self._maybe_tune(run, ndof)
         run(self._h)
 
-    def autotune(self, *args, first_dof=0, ndof=None):
-        """Time the candidate schedules on these arrays (f_0..f_{S-1},
-        Q_0..Q_{S-1}; the Q arrays are overwritten) and keep the fastest."""
+    def autotune(self, *args, first_dof=0, ndof=None, candidates=None):
+        """Time the candidate schedules (default ``autotune.CANDIDATES``) on
+        these arrays (f_0..f_{S-1}, Q_0..Q_{S-1}; the Q arrays are overwritten)
+        and keep the fastest."""
         run, ndof = self._runner(args, first_dof, ndof)
-        self._tune(run, ndof)
+        self._tune(run, ndof, candidates)
         return self.tune_report

@@ -17,6 +17,7 @@ import gc
 import numpy as np
 
 from . import _lib
+from . import autotune as _autotune
 from .scattering import (DGFSBiScatteringModel, DGFSScatteringModel, subclass_where)
 from .velocitymesh import DGFSBiVelocityMesh, DGFSVelocityMesh
 
@@ -42,9 +43,10 @@ class DGFSCollisionSystem:
 
     def prepare(self):
         """With ``[scattering-model] autotune = 1``, tune the collision
-        schedule on the solution bank (the last bank is overwritten)."""
+        schedule on the solution bank (the last bank is overwritten).  Only
+        one-stream schedules: a step may be captured in a HIP graph."""
         if self.cfg.getint('scattering-model', 'autotune', 0) and not getattr(self.sm, '_tuned', False):
-            self.sm.autotune(self.banks[0], self.banks[-1])
+            self.sm.autotune(self.banks[0], self.banks[-1], candidates=_autotune.ONE_STREAM)
 
     def axnpby(self, dst, terms):
         """banks[dst] = a0*banks[dst] + sum(a*banks[i] for a, i in terms[1:])."""
@@ -79,7 +81,8 @@ class DGFSBiCollisionSystem:
 
     def prepare(self):
         if self.cfg.getint('scattering-model', 'autotune', 0) and not getattr(self.sm, '_tuned', False):
-            self.sm.autotune(*([b[0] for b in self.banks] + [b[-1] for b in self.banks]))
+            self.sm.autotune(*([b[0] for b in self.banks] + [b[-1] for b in self.banks]),
+                             candidates=_autotune.ONE_STREAM)
 
     def axnpby(self, dst, terms):
         for sb in self.banks:
