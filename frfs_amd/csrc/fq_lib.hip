@@ -1933,7 +1933,7 @@ __global__ __launch_bounds__(kPlaneThreads, 2) void k_out(const cpx<double> *Y, 
 // gain and loss spectra Hermitian, so DFT+_{x,y}(Y) = gain + i loss with
 // both real: one complex transform per plane, Q = Re - f Im.  With MOM the
 // unit also writes its plane's sums of Q {1, c, |c|^2} (FP64) to
-// mom[slot][t][vz][5] for k_project_m.
+// mom[slot][t][vz][5] for k_project_b.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
@@ -2077,47 +2077,56 @@ __global__ __launch_bounds__(256) void k_project(AosoaPtrs ptr, Dof dof, int ns,
 }
 
 // The same projection with the moments of Q taken from the per-plane sums
-// that k_out1 wrote (mom[slot][s][vz][5], N = 32 planes summed in order by
-// the first wave), so Q is read once and written once.
-__global__ __launch_bounds__(256) void k_project_m(AosoaPtrs ptr, Dof dof, int ns, int N, double L,
+// that k_out1 wrote (mom[slot][s][vz][5]), so Q is read once and written once,
+// over blocks of 32 consecutive batch slots and chunks of kProjChunk grid
+// points: lane e of each 32-lane group takes slot s0 + e, so with one solution
+// point per element and 32-aligned DOFs the group's accesses are one
+// contiguous AoSoA row segment (32 elements x 8 bytes) per grid point.
+constexpr int kProjChunk = 1024;
+
+__global__ __launch_bounds__(256) void k_project_b(AosoaPtrs ptr, Dof dof, int ns, int N, double L,
                                                    const double *ginv, const double *mass, const double *mom, int D) {
-    const int per = (D + 7) / 8;
-    const int slot = int(blockIdx.x & 7) * per + int(blockIdx.x >> 3);
-    if (slot >= D) return;
-    const int64_t b = dof.base(slot);
     const int64_t n = int64_t(N) * N * N;
+    const int nchunk = int(n / kProjChunk);
+    const int blk = int(blockIdx.x) / nchunk, ch = int(blockIdx.x) % nchunk;
+    const int s0 = 32 * blk;
     const int K = ns + 4;
-    __shared__ double aq[kProjMaxRows];
-    if (threadIdx.x < 64) {
-        // lanes 0..N-1 hold one plane's sums, summed across the wave
-        double sm[kProjMaxRows];
+    __shared__ double lam_s[32][kProjMaxRows];
+    const int tid = int(threadIdx.x);
+    if (tid < 32) {
+        const int slot = s0 + tid;
+        double aq[kProjMaxRows];
 #pragma unroll
-        for (int i = 0; i < kProjMaxRows; ++i) sm[i] = 0;
-        for (int s = 0; s < ns; ++s) {
-            double m[5] = {0, 0, 0, 0, 0};
-            if (int(threadIdx.x) < N) {
+        for (int i = 0; i < kProjMaxRows; ++i) aq[i] = 0;
+        if (slot < D) {
+            for (int s = 0; s < ns; ++s) {
+                double m[5] = {0, 0, 0, 0, 0};
+                const double *ms = mom + (int64_t(slot) * ns + s) * N * 5;
+                for (int z = 0; z < N; ++z) {
 #pragma unroll
-                for (int k = 0; k < 5; ++k) m[k] = mom[((int64_t(slot) * ns + s) * N + threadIdx.x) * 5 + k];
+                    for (int k = 0; k < 5; ++k) m[k] += ms[z * 5 + k];
+                }
+                aq[s] = m[0];
+                for (int k = 1; k < 5; ++k) aq[ns + k - 1] += mass[s] * m[k];
             }
-#pragma unroll
-            for (int k = 0; k < 5; ++k) m[k] = wave_sum(m[k]);
-            sm[s] = m[0];
-            for (int k = 1; k < 5; ++k) sm[ns + k - 1] += mass[s] * m[k];
         }
-        if (threadIdx.x == 0)
-            for (int i = 0; i < K; ++i) aq[i] = sm[i];
+        for (int i = 0; i < K; ++i) {
+            double acc = 0;
+            for (int k = 0; k < K; ++k) acc += ginv[i * K + k] * aq[k];
+            lam_s[tid][i] = acc;
+        }
     }
     __syncthreads();
-    double lam[kProjMaxRows];
-    for (int i = 0; i < K; ++i) {
-        double acc = 0;
-        for (int k = 0; k < K; ++k) acc += ginv[i * K + k] * aq[k];
-        lam[i] = acc;
-    }
+    const int e = tid % 32, vg = tid / 32;
+    const int slot = s0 + e;
+    if (slot >= D) return;                        // no barrier follows
+    const int64_t b = dof.base(slot);
+    const double *lam = lam_s[e];
     for (int s = 0; s < ns; ++s) {
         double *q = ptr.q[s];
         const double ms = mass[s];
-        for (int64_t v = threadIdx.x; v < n; v += blockDim.x) {
+        for (int i = vg; i < kProjChunk; i += 8) {
+            const int64_t v = int64_t(ch) * kProjChunk + i;
             double ph[5];
             phis(v, N, L, ph);
             const double corr = lam[s] + ms * (lam[ns] * ph[1] + lam[ns + 1] * ph[2] + lam[ns + 2] * ph[3] +
@@ -2160,7 +2169,7 @@ struct Scratch {
     void *FJ = nullptr;
     float *NB = nullptr;          // group pairs: Nyquist sums of Im G_g (k_prod1 -> k_nyqb)
     cpx<double> *C = nullptr;     // group pairs: Nyquist corrections (k_nyqb -> k_accum_h)
-    double *MOM = nullptr;        // group pairs with projection: per-plane sums of Q (k_out1 -> k_project_m)
+    double *MOM = nullptr;        // group pairs with projection: per-plane sums of Q (k_out1 -> k_project_b)
 };
 
 struct fq_handle {
@@ -2715,8 +2724,8 @@ template <int N, typename T> int run_chunk(fq_handle *h, const Scratch &sc, cons
         // per batch, right after k_out: the batch's Q is still in L2
         pre();
         if (h->hpair)
-            hipLaunchKernelGGL(k_project_m, dim3(unsigned(8 * ((D + 7) / 8))), dim3(256), 0, st, ptr, dof, ntgt, N,
-                               P.L, h->ginv, h->pmass, sc.MOM, int(D));
+            hipLaunchKernelGGL(k_project_b, dim3(unsigned((D + 31) / 32 * (int64_t(N) * N * N / kProjChunk))),
+                               dim3(256), 0, st, ptr, dof, ntgt, N, P.L, h->ginv, h->pmass, sc.MOM, int(D));
         else
         hipLaunchKernelGGL(k_project, dim3(unsigned(8 * ((D + 7) / 8))), dim3(256), 0, st, ptr, dof, ntgt, N, P.L,
                            h->ginv, h->pmass, int(D));
