@@ -1931,9 +1931,12 @@ __global__ __launch_bounds__(kPlaneThreads, 2) void k_out(const cpx<double> *Y, 
 // ---------------------------------------------------------------------------
 // k_out1: the output stage of the group-pair path.  Y = Yg + i Yl with the
 // gain and loss spectra Hermitian, so DFT+_{x,y}(Y) = gain + i loss with
-// both real: one complex transform per plane, Q = Re - f Im.  With MOM the
-// unit also writes its plane's sums of Q {1, c, |c|^2} (FP64) to
-// mom[slot][t][vz][5] for k_project_b.
+// both real: one complex transform per plane, Q = Re - f Im.  A workgroup
+// transforms one plane of four consecutive batch slots (one wave each), stages
+// the results in LDS as [point][slot], and reads f and writes Q with the four
+// slots on adjacent lanes, so that consecutive elements of one solution point
+// meet in one AoSoA row segment.  With MOM it also writes each slot's plane
+// sums of Q {1, c, |c|^2} (FP64) to mom[slot][t][vz][5] for k_project_b.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
@@ -1942,40 +1945,52 @@ __device__ __forceinline__ double wave_sum(double x) {
 }
 
 template <int N, bool MOM>
-__global__ __launch_bounds__(kPlaneThreads, 2) void k_out1(const cpx<double> *Y, AosoaPtrs ptr, Dof dof, int ntgt,
-                                                 int64_t nunits, double *mom, double L) {
+__global__ __launch_bounds__(4 * Geo<N>::LPP, 2) void k_out1(const cpx<double> *Y, AosoaPtrs ptr, Dof dof, int ntgt,
+                                                   int D, double *mom, double L) {
     using G = Geo<N>;
-    constexpr int E = G::E;
-    constexpr int UPB = kPlaneThreads / G::LPP;
-    __shared__ cpx<double> lds[UPB][N * G::LDSROW];
-    const int u = threadIdx.x / G::LPP;
-    const int lt = threadIdx.x % G::LPP;
-    const int64_t unit = int64_t(blockIdx.x) * UPB + u;
-    const bool valid = unit < nunits;
-    const int vz = int(unit % N);
-    const int64_t dt = unit / N;
-    const int t = int(dt % ntgt);
-    const int slot = int(dt / ntgt);
+    static_assert(G::LPP == 64, "one plane per wave");
+    constexpr int E = G::E, U = 4;                 // batch slots per workgroup, one wave each
+    constexpr int TH = U * G::LPP;
+    constexpr int PPT = G::N2 / (TH / U);          // grid points per thread in the write phase
+    __shared__ cpx<double> lds[U][N * G::LDSROW];
+    static_assert(U * G::N2 <= U * N * G::LDSROW, "staging fits the transpose buffers");
+    const int tid = int(threadIdx.x);
+    const int u = tid / G::LPP, lt = tid % G::LPP;
+    const int b = int(blockIdx.x);
+    const int vz = b % N, t = (b / N) % ntgt, sg = b / (N * ntgt);
+    const int slot = sg * U + u;
     const int c = lt >> 1, half = lt & 1;
-    const bool act = valid && c < N;
+    const bool act = slot < D && c < N;
     cpx<double> v[E];
-    plane_inv64<N>(Y + (int64_t(slot) * ntgt + t) * G::VOL + int64_t(vz) * G::N2, lds[u], v, c, half, act);
-    double m[5] = {0, 0, 0, 0, 0};
+    plane_inv64<N>(Y + (int64_t(slot < D ? slot : 0) * ntgt + t) * G::VOL + int64_t(vz) * G::N2, lds[u], v, c,
+                   half, act);
+    // stage (gain, loss) as [point][slot], point = vx * N + vy
+    __syncthreads();
+    cpx<double> *st = &lds[0][0];
     if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) st[(c * N + half + 2 * e) * U + u] = v[e];
+    }
+    __syncthreads();
+    // thread (slot u2, points p): the U slots of one point are adjacent AoSoA
+    // elements when they are consecutive elements of one solution point
+    const int u2 = tid % U, slot2 = sg * U + u2;
+    double m[5] = {0, 0, 0, 0, 0};
+    if (slot2 < D) {
         const double *f = ptr.f[t];
         double *q = ptr.q[t];
-        const int64_t b = dof.base(slot);
-        const double h = 2.0 * L / N, c0 = -L + L / N;
-        const double cx = c0 + h * c, cz = c0 + h * vz;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int vy = half + 2 * e;
-            const int64_t vv = (int64_t(c) * N + vy) * N + vz;
-            const int64_t a = b + vv * dof.soasz;
-            const double qv = v[e].x - f[a] * v[e].y;
+        const int64_t b2 = dof.base(slot2);
+        const double h = 2.0 * L / N, c0 = -L + L / N, cz = c0 + h * vz;
+#pragma unroll 4
+        for (int i = 0; i < PPT; ++i) {
+            const int pnt = tid / U + (TH / U) * i;
+            const int vx = pnt / N, vy = pnt % N;
+            const int64_t a = b2 + ((int64_t(vx) * N + vy) * N + vz) * dof.soasz;
+            const cpx<double> g = st[pnt * U + u2];
+            const double qv = g.x - f[a] * g.y;
             q[a] = qv;
             if constexpr (MOM) {
-                const double cy = c0 + h * vy;
+                const double cx = c0 + h * vx, cy = c0 + h * vy;
                 m[0] += qv;
                 m[1] += qv * cx;
                 m[2] += qv * cy;
@@ -1985,12 +2000,27 @@ __global__ __launch_bounds__(kPlaneThreads, 2) void k_out1(const cpx<double> *Y,
         }
     }
     if constexpr (MOM) {
-        static_assert(G::LPP == 64, "one plane per wave");
+        // sums of the threads of one slot: lanes u2 + 4j of each wave, then the U waves
 #pragma unroll
-        for (int k = 0; k < 5; ++k) m[k] = wave_sum(m[k]);
-        if (valid && lt == 0) {
+        for (int k = 0; k < 5; ++k) {
 #pragma unroll
-            for (int k = 0; k < 5; ++k) mom[((int64_t(slot) * ntgt + t) * N + vz) * 5 + k] = m[k];
+            for (int o = U; o < 64; o <<= 1) m[k] += __shfl_xor(m[k], o);
+        }
+        __syncthreads();                             // staging reads done
+        double *red = reinterpret_cast<double *>(&lds[0][0]);       // [wave][u2][5]
+        if (lt < U) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) red[(u * U + lt) * 5 + k] = m[k];
+        }
+        __syncthreads();
+        if (tid < U * 5) {
+            const int uu = tid / 5, k = tid % 5, sl = sg * U + uu;
+            if (sl < D) {
+                double acc = 0;
+#pragma unroll
+                for (int w = 0; w < U; ++w) acc += red[(w * U + uu) * 5 + k];
+                mom[((int64_t(sl) * ntgt + t) * N + vz) * 5 + k] = acc;
+            }
         }
     }
 }
@@ -2711,10 +2741,11 @@ template <int N, typename T> int run_chunk(fq_handle *h, const Scratch &sc, cons
     if constexpr (N == 32) {
         // group pairs: gain + i loss in one volume, one transform per plane
         if (h->hpair) {
+            const dim3 g1(unsigned((D + 3) / 4 * int64_t(ntgt) * N)), b1(4 * Geo<N>::LPP);
             if (h->flags & FQ_FLAG_PROJECT)
-                hipLaunchKernelGGL((k_out1<N, true>), dim3(pblocks(u)), pt, 0, st, Y, ptr, dof, ntgt, u, sc.MOM, P.L);
+                hipLaunchKernelGGL((k_out1<N, true>), g1, b1, 0, st, Y, ptr, dof, ntgt, D, sc.MOM, P.L);
             else
-                hipLaunchKernelGGL((k_out1<N, false>), dim3(pblocks(u)), pt, 0, st, Y, ptr, dof, ntgt, u, nullptr, P.L);
+                hipLaunchKernelGGL((k_out1<N, false>), g1, b1, 0, st, Y, ptr, dof, ntgt, D, nullptr, P.L);
             out1 = true;
         }
     }
