@@ -237,6 +237,47 @@ __global__ __launch_bounds__(256) void k_fwd_z(AosoaPtrs ptr, Dof dof, cpx<doubl
     }
 }
 
+// k_fwd_z4 (N = 32): the same for four consecutive batch slots per workgroup
+// (one wave each, same source and row I).  The workgroup first reads the
+// row's 1024 values of the four slots with the slots on adjacent lanes, so
+// consecutive elements of one solution point meet in one AoSoA row segment,
+// and stages them in LDS as [J][half][e][slot] (half-rows of 65 and rows of
+// 130 doubles: lanes J and J + 16 share a bank pair, no other conflicts).
+template <int N>
+__global__ __launch_bounds__(256) void k_fwd_z4(AosoaPtrs ptr, Dof dof, cpx<double> *T0, int nsrc, int D) {
+    using G = Geo<N>;
+    static_assert(G::LPP == 64, "one row per wave");
+    constexpr int E = G::E, U = 4, HS = U * E + 1, JS = 2 * HS;
+    __shared__ double st[N * JS];
+    const int tid = int(threadIdx.x);
+    const int b = int(blockIdx.x);
+    const int I = b % N, s = (b / N) % nsrc, sg = b / (N * nsrc);
+    const double *f = ptr.f[s];
+    {
+        const int u2 = tid % U, slot2 = sg * U + u2;
+        const int64_t b2 = slot2 < D ? dof.base(slot2) : 0;
+#pragma unroll 4
+        for (int i = 0; i < G::N2 / (256 / U); ++i) {
+            const int pnt = tid / U + (256 / U) * i;            // (J, K) of row I
+            const int J = pnt / N, K = pnt % N;
+            const double x = slot2 < D ? f[b2 + ((int64_t(I) * N + J) * N + K) * dof.soasz] : 0.0;
+            st[J * JS + (K / E) * HS + (K % E) * U + u2] = x;
+        }
+    }
+    __syncthreads();
+    const int u = tid / 64, lt = tid % 64;
+    const int slot = sg * U + u;
+    if (slot >= D) return;                           // no barrier follows
+    const int J = lt >> 1, half = lt & 1;
+    cpx<double> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = {st[J * JS + half * HS + e * U + u], 0.0};
+    fq::fft2_dif<N, -1>(v, half);
+    cpx<double> *dst = T0 + (int64_t(slot) * nsrc + s) * G::VOL + int64_t(I) * N + J;
+#pragma unroll
+    for (int e = 0; e < E; ++e) dst[int64_t(half + 2 * e) * G::N2] = v[e];
+}
+
 // ---------------------------------------------------------------------------
 // Shared plane helpers.  A plane unit has LPP lanes; `row` = lt >> 1 is the
 // column J in the column phase and the row (x index) in the row phase.
@@ -2603,7 +2644,11 @@ template <int N, typename T> int run_chunk(fq_handle *h, const Scratch &sc, cons
     int64_t u;
     u = int64_t(D) * nsrc * N;
     pre();
-    hipLaunchKernelGGL(k_fwd_z<N>, dim3(blocks(u)), bt, 0, st, ptr, dof, T0, nsrc, u);
+    if constexpr (N == 32)
+        hipLaunchKernelGGL(k_fwd_z4<N>, dim3(unsigned((D + 3) / 4 * int64_t(nsrc) * N)), bt, 0, st, ptr, dof, T0,
+                           nsrc, D);
+    else
+        hipLaunchKernelGGL(k_fwd_z<N>, dim3(blocks(u)), bt, 0, st, ptr, dof, T0, nsrc, u);
     post(0);
     // the fused FP32 path reads an FP32 copy of F, written by k_fwd_xy itself
     const bool want32 = h->fused && std::is_same<T, float>::value;
