@@ -990,158 +990,158 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
     cpx<T> acc[N];
     T re0[HP ? N : 1];
     for (int m = 0; m < (HP ? 2 : 1); ++m) {
-    const int g = HP ? 2 * gv + m : gv;
+        const int g = HP ? 2 * gv + m : gv;
 #pragma unroll
-    for (int e = 0; e < N; ++e) acc[e] = {0, 0};
-    if (HP && g >= ngroups) break;                 // odd group count: the pair's second member is zero
-    const int p0 = g_p0[g], np = g_np[g];
-    for (int pi = p0; pi < p0 + np; ++pi) {
-        const DevProd pr = prods[pi];
-        const cpx<T> *src1 = FT + (int64_t(slot) * nsrc + item_src[pr.i1]) * G3::VOL;
-        const cpx<T> *src2 = FT + (int64_t(slot) * nsrc + item_src[pr.i2]) * G3::VOL;
-        const cpx<T> *zc1 = zc + (int64_t(pr.i1) * 4 + G) * ZC, *zc2 = zc + (int64_t(pr.i2) * 4 + G) * ZC;
-        const cpx<T> *ph1 = ph + int64_t(pr.i1) * 3 * N, *ph2 = ph + int64_t(pr.i2) * 3 * N;
-        const T cf = T(pr.coef);
-        if constexpr (PACK) {
-            cpx<T> e[N];
-            __syncthreads();                               // LDS free
-            // conjugate products (one source, opposite phases) take the cheaper
-            // z-stage: every product (CONJ) or those marked in pconj
-            if (CONJ || (pconj && pconj[pi])) {
-                const cpx<T> *zqp = zq + (int64_t(pi) * 4 + G) * (3 + 2 * PG::S);
-                if (G & 1) zstage_conj<N, T, P1::THREADS, true>(src1, zqp, ph1, lds);
-                else zstage_conj<N, T, P1::THREADS, false>(src1, zqp, ph1, lds);
-            } else {
-                zstage_pair<N, T, P1::THREADS>(src1, src2, zc1, zc2, ph1, ph2, lds);
-            }
-            __syncthreads();
-            // the product's corrections: global loads in flight across the
-            // x,y transforms, then staged in LDS for the whole workgroup
-            constexpr int KPT = (N * N + P1::THREADS - 1) / P1::THREADS;
-            constexpr int SN = PG::S * N;
-            static_assert(SN <= P1::THREADS, "one J and one I element per thread");
-            const cpx<T> *ny = NY + (int64_t(slot) * nprods + pi) * 3 * G3::N2;
-            const int tid = int(threadIdx.x);
-            cpx<T> rk[KPT], rj = {0, 0}, ri = {0, 0};
+        for (int e = 0; e < N; ++e) acc[e] = {0, 0};
+        if (HP && g >= ngroups) break;                 // odd group count: the pair's second member is zero
+        const int p0 = g_p0[g], np = g_np[g];
+        for (int pi = p0; pi < p0 + np; ++pi) {
+            const DevProd pr = prods[pi];
+            const cpx<T> *src1 = FT + (int64_t(slot) * nsrc + item_src[pr.i1]) * G3::VOL;
+            const cpx<T> *src2 = FT + (int64_t(slot) * nsrc + item_src[pr.i2]) * G3::VOL;
+            const cpx<T> *zc1 = zc + (int64_t(pr.i1) * 4 + G) * ZC, *zc2 = zc + (int64_t(pr.i2) * 4 + G) * ZC;
+            const cpx<T> *ph1 = ph + int64_t(pr.i1) * 3 * N, *ph2 = ph + int64_t(pr.i2) * 3 * N;
+            const T cf = T(pr.coef);
+            if constexpr (PACK) {
+                cpx<T> e[N];
+                __syncthreads();                               // LDS free
+                // conjugate products (one source, opposite phases) take the cheaper
+                // z-stage: every product (CONJ) or those marked in pconj
+                if (CONJ || (pconj && pconj[pi])) {
+                    const cpx<T> *zqp = zq + (int64_t(pi) * 4 + G) * (3 + 2 * PG::S);
+                    if (G & 1) zstage_conj<N, T, P1::THREADS, true>(src1, zqp, ph1, lds);
+                    else zstage_conj<N, T, P1::THREADS, false>(src1, zqp, ph1, lds);
+                } else {
+                    zstage_pair<N, T, P1::THREADS>(src1, src2, zc1, zc2, ph1, ph2, lds);
+                }
+                __syncthreads();
+                // the product's corrections: global loads in flight across the
+                // x,y transforms, then staged in LDS for the whole workgroup
+                constexpr int KPT = (N * N + P1::THREADS - 1) / P1::THREADS;
+                constexpr int SN = PG::S * N;
+                static_assert(SN <= P1::THREADS, "one J and one I element per thread");
+                const cpx<T> *ny = NY + (int64_t(slot) * nprods + pi) * 3 * G3::N2;
+                const int tid = int(threadIdx.x);
+                cpx<T> rk[KPT], rj = {0, 0}, ri = {0, 0};
 #pragma unroll
-            for (int r = 0; r < KPT; ++r) {
-                const int idx = tid + r * P1::THREADS;
-                rk[r] = (N * N % P1::THREADS == 0 || idx < N * N) ? ny[idx] : cpx<T>{0, 0};
-            }
-            if (tid < SN) {
-                const int kk = tid / N, w = tid % N, z = G + 4 * kk;
-                rj = ny[G3::N2 + z * N + w];                       // (vx = w, vz = z)
-                ri = ny[2 * G3::N2 + w * N + z];                   // (vy = w, vz = z)
-            }
-            plane_inv1<N, T>(L1, e, c, act);               // lane c: row vx = c over vy
-            cpx<T> *nyk = lds + PG::S * PG::PLANE, *nyj = nyk + N * N, *nyi = nyj + SN;
+                for (int r = 0; r < KPT; ++r) {
+                    const int idx = tid + r * P1::THREADS;
+                    rk[r] = (N * N % P1::THREADS == 0 || idx < N * N) ? ny[idx] : cpx<T>{0, 0};
+                }
+                if (tid < SN) {
+                    const int kk = tid / N, w = tid % N, z = G + 4 * kk;
+                    rj = ny[G3::N2 + z * N + w];                       // (vx = w, vz = z)
+                    ri = ny[2 * G3::N2 + w * N + z];                   // (vy = w, vz = z)
+                }
+                plane_inv1<N, T>(L1, e, c, act);               // lane c: row vx = c over vy
+                cpx<T> *nyk = lds + PG::S * PG::PLANE, *nyj = nyk + N * N, *nyi = nyj + SN;
 #pragma unroll
-            for (int r = 0; r < KPT; ++r) {
-                const int idx = tid + r * P1::THREADS;
-                if (N * N % P1::THREADS == 0 || idx < N * N) nyk[idx] = rk[r];
-            }
-            if (tid < SN) { nyj[tid] = rj; nyi[tid] = ri; }
-            __syncthreads();
-            const T sz = (vz & 1) ? T(-1) : T(1);
-            const T sc = (c & 1) ? T(-1) : T(1);
-            const cpx<T> xj = act ? nyj[k * N + c] : cpx<T>{0, 0};
+                for (int r = 0; r < KPT; ++r) {
+                    const int idx = tid + r * P1::THREADS;
+                    if (N * N % P1::THREADS == 0 || idx < N * N) nyk[idx] = rk[r];
+                }
+                if (tid < SN) { nyj[tid] = rj; nyi[tid] = ri; }
+                __syncthreads();
+                const T sz = (vz & 1) ? T(-1) : T(1);
+                const T sc = (c & 1) ? T(-1) : T(1);
+                const cpx<T> xj = act ? nyj[k * N + c] : cpx<T>{0, 0};
 #pragma unroll
-            for (int j = 0; j < N; ++j) {
-                const cpx<T> xk = act ? nyk[j * N + c] : cpx<T>{0, 0};
-                const cpx<T> xi = nyi[k * N + j];
-                const T jx = (j & 1) ? -xj.x : xj.x, jy = (j & 1) ? -xj.y : xj.y;
-                const T ux = fq::fmadd(sc, xi.x, fq::fmadd(sz, xk.x, jx));
-                const T uy = fq::fmadd(sc, xi.y, fq::fmadd(sz, xk.y, jy));
-                const T rx = e[j].x - ux, ry = e[j].y - uy;
-                // (rx + i uy)(ry - i ux)
-                const T px = fq::fmadd(rx, ry, uy * ux);
-                const T py = fq::fmadd(uy, ry, -(rx * ux));
-                if constexpr (MULTI) acc[j] = {fq::fmadd(cf, px, acc[j].x), fq::fmadd(cf, py, acc[j].y)};
-                else acc[j] = {px, py};                      // coefficient applied below
-            }
-        } else {
-            cpx<T> e1[N], e2[N];
-            __syncthreads();                               // LDS free
-            zstage<N, T, P1::THREADS>(src1, zc1, ph1, lds);
-            __syncthreads();
-            plane_inv1<N, T>(L1, e1, c, act);
-            __syncthreads();
-            zstage<N, T, P1::THREADS>(src2, zc2, ph2, lds);
-            __syncthreads();
-            plane_inv1<N, T>(L1, e2, c, act);
-            if constexpr (MULTI) {
-#pragma unroll
-                for (int e = 0; e < N; ++e) {
-                    const cpx<T> pe = fq::cmul(e1[e], e2[e]);
-                    acc[e] = {fq::fmadd(cf, pe.x, acc[e].x), fq::fmadd(cf, pe.y, acc[e].y)};
+                for (int j = 0; j < N; ++j) {
+                    const cpx<T> xk = act ? nyk[j * N + c] : cpx<T>{0, 0};
+                    const cpx<T> xi = nyi[k * N + j];
+                    const T jx = (j & 1) ? -xj.x : xj.x, jy = (j & 1) ? -xj.y : xj.y;
+                    const T ux = fq::fmadd(sc, xi.x, fq::fmadd(sz, xk.x, jx));
+                    const T uy = fq::fmadd(sc, xi.y, fq::fmadd(sz, xk.y, jy));
+                    const T rx = e[j].x - ux, ry = e[j].y - uy;
+                    // (rx + i uy)(ry - i ux)
+                    const T px = fq::fmadd(rx, ry, uy * ux);
+                    const T py = fq::fmadd(uy, ry, -(rx * ux));
+                    if constexpr (MULTI) acc[j] = {fq::fmadd(cf, px, acc[j].x), fq::fmadd(cf, py, acc[j].y)};
+                    else acc[j] = {px, py};                      // coefficient applied below
                 }
             } else {
+                cpx<T> e1[N], e2[N];
+                __syncthreads();                               // LDS free
+                zstage<N, T, P1::THREADS>(src1, zc1, ph1, lds);
+                __syncthreads();
+                plane_inv1<N, T>(L1, e1, c, act);
+                __syncthreads();
+                zstage<N, T, P1::THREADS>(src2, zc2, ph2, lds);
+                __syncthreads();
+                plane_inv1<N, T>(L1, e2, c, act);
+                if constexpr (MULTI) {
 #pragma unroll
-                for (int e = 0; e < N; ++e) acc[e] = fq::cscale(fq::cmul(e1[e], e2[e]), cf);
-            }
-        }
-    }
-    if constexpr (PACK && !MULTI) {
-        // the single product's coefficient (1 for VHS and VSS), applied once
-        const T cf = np > 0 ? T(prods[p0].coef) : T(1);
-        if (cf != T(1)) {
+                    for (int e = 0; e < N; ++e) {
+                        const cpx<T> pe = fq::cmul(e1[e], e2[e]);
+                        acc[e] = {fq::fmadd(cf, pe.x, acc[e].x), fq::fmadd(cf, pe.y, acc[e].y)};
+                    }
+                } else {
 #pragma unroll
-            for (int e = 0; e < N; ++e) acc[e] = fq::cscale(acc[e], cf);
-        }
-    }
-    if constexpr (HP) {
-        if (NB) {
-            // Nyquist-plane sums of b = Im G_g.  The plane region is free: its
-            // last reads (plane_inv1) precede the barrier before the unpack.
-            float *nb = NB + (int64_t(slot) * ngroups + g) * kNbPer32;
-            T sj = 0;
-#pragma unroll
-            for (int j = 0; j < N; ++j) sj += (j & 1) ? -acc[j].y : acc[j].y;
-            if (act) nb[4 * N * N + vz * N + c] = sj;                   // J: (vz, vx = c)
-            // b planes as [k][vy][vx] with rows of BR = N + 4 floats: lane c's
-            // stores for one vy are consecutive, a row is 16-byte aligned, and
-            // 16 rows read by 16 lanes start on 16 different 4-bank groups
-            constexpr int BR = N + 4;
-            T *bl = reinterpret_cast<T *>(lds);                        // [S][N][BR]
-            static_assert(PG::S * N * BR * sizeof(T) <= PG::S * PG::PLANE * sizeof(cpx<T>), "fits the planes");
-            if (act) {
-#pragma unroll
-                for (int j = 0; j < N; ++j) bl[(k * N + j) * BR + c] = acc[j].y;
-            }
-            __syncthreads();
-            const int tid = int(threadIdx.x);
-            const T sgn = (G & 1) ? T(-1) : T(1);                      // (-1)^vz on vz = G + 4k
-            static_assert(N * N == 4 * P1::THREADS && P1::THREADS == PG::S * N, "thread map");
-            {
-                // K: four consecutive vx of one vy per thread, summed over the S planes
-                const int y = tid / (N / 4), x0 = 4 * (tid % (N / 4));
-                float4 sk = {0, 0, 0, 0};
-#pragma unroll
-                for (int kk = 0; kk < PG::S; ++kk) {
-                    const float4 q = *reinterpret_cast<const float4 *>(bl + (kk * N + y) * BR + x0);
-                    sk.x += q.x; sk.y += q.y; sk.z += q.z; sk.w += q.w;
+                    for (int e = 0; e < N; ++e) acc[e] = fq::cscale(fq::cmul(e1[e], e2[e]), cf);
                 }
-                *reinterpret_cast<float4 *>(nb + G * N * N + y * N + x0) =
-                    float4{sgn * sk.x, sgn * sk.y, sgn * sk.z, sgn * sk.w};       // K: [G][vy][vx]
             }
-            {
-                // I: the row (plane kk, vy) summed over vx with the sign (-1)^vx
-                const int kk = tid / N, y = tid % N;
-                T sI = 0;
+        }
+        if constexpr (PACK && !MULTI) {
+            // the single product's coefficient (1 for VHS and VSS), applied once
+            const T cf = np > 0 ? T(prods[p0].coef) : T(1);
+            if (cf != T(1)) {
 #pragma unroll
-                for (int x4 = 0; x4 < N / 4; ++x4) {
-                    const float4 q = *reinterpret_cast<const float4 *>(bl + (kk * N + y) * BR + 4 * x4);
-                    sI += (q.x - q.y) + (q.z - q.w);
+                for (int e = 0; e < N; ++e) acc[e] = fq::cscale(acc[e], cf);
+            }
+        }
+        if constexpr (HP) {
+            if (NB) {
+                // Nyquist-plane sums of b = Im G_g.  The plane region is free: its
+                // last reads (plane_inv1) precede the barrier before the unpack.
+                float *nb = NB + (int64_t(slot) * ngroups + g) * kNbPer32;
+                T sj = 0;
+#pragma unroll
+                for (int j = 0; j < N; ++j) sj += (j & 1) ? -acc[j].y : acc[j].y;
+                if (act) nb[4 * N * N + vz * N + c] = sj;                   // J: (vz, vx = c)
+                // b planes as [k][vy][vx] with rows of BR = N + 4 floats: lane c's
+                // stores for one vy are consecutive, a row is 16-byte aligned, and
+                // 16 rows read by 16 lanes start on 16 different 4-bank groups
+                constexpr int BR = N + 4;
+                T *bl = reinterpret_cast<T *>(lds);                        // [S][N][BR]
+                static_assert(PG::S * N * BR * sizeof(T) <= PG::S * PG::PLANE * sizeof(cpx<T>), "fits the planes");
+                if (act) {
+#pragma unroll
+                    for (int j = 0; j < N; ++j) bl[(k * N + j) * BR + c] = acc[j].y;
                 }
-                nb[5 * N * N + (G + 4 * kk) * N + y] = sI;               // I: (vz, vy)
-            }
-            __syncthreads();                                           // LDS reads done
-        }
-        if (m == 0) {
+                __syncthreads();
+                const int tid = int(threadIdx.x);
+                const T sgn = (G & 1) ? T(-1) : T(1);                      // (-1)^vz on vz = G + 4k
+                static_assert(N * N == 4 * P1::THREADS && P1::THREADS == PG::S * N, "thread map");
+                {
+                    // K: four consecutive vx of one vy per thread, summed over the S planes
+                    const int y = tid / (N / 4), x0 = 4 * (tid % (N / 4));
+                    float4 sk = {0, 0, 0, 0};
 #pragma unroll
-            for (int j = 0; j < N; ++j) re0[j] = acc[j].x;
+                    for (int kk = 0; kk < PG::S; ++kk) {
+                        const float4 q = *reinterpret_cast<const float4 *>(bl + (kk * N + y) * BR + x0);
+                        sk.x += q.x; sk.y += q.y; sk.z += q.z; sk.w += q.w;
+                    }
+                    *reinterpret_cast<float4 *>(nb + G * N * N + y * N + x0) =
+                        float4{sgn * sk.x, sgn * sk.y, sgn * sk.z, sgn * sk.w};       // K: [G][vy][vx]
+                }
+                {
+                    // I: the row (plane kk, vy) summed over vx with the sign (-1)^vx
+                    const int kk = tid / N, y = tid % N;
+                    T sI = 0;
+#pragma unroll
+                    for (int x4 = 0; x4 < N / 4; ++x4) {
+                        const float4 q = *reinterpret_cast<const float4 *>(bl + (kk * N + y) * BR + 4 * x4);
+                        sI += (q.x - q.y) + (q.z - q.w);
+                    }
+                    nb[5 * N * N + (G + 4 * kk) * N + y] = sI;               // I: (vz, vy)
+                }
+                __syncthreads();                                           // LDS reads done
+            }
+            if (m == 0) {
+#pragma unroll
+                for (int j = 0; j < N; ++j) re0[j] = acc[j].x;
+            }
         }
-    }
     }
     if constexpr (HP) {
         // a_2u + i a_2u+1 (acc is zero when the second member is absent)
