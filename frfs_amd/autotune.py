@@ -27,7 +27,7 @@ CANDIDATES = (
     {'fused': 1, 'max_chunk': 0, 'two_lane': 0},
     {'fused': 1, 'max_chunk': 1024, 'two_lane': 0},
     {'fused': 1, 'max_chunk': 128, 'two_lane': 0},
-    {'fused': 1, 'max_chunk': 0, 'two_lane': 0, 'accum_quad': 1},
+    {'fused': 1, 'max_chunk': 1024, 'two_lane': 0, 'accum_quad': 1},
     {'fused': 1, 'max_chunk': 0, 'two_lane': 0, 'no_pack': 1},
     {'fused': 1, 'max_chunk': 0, 'two_lane': 0, 'no_hpair': 1},
     {'fused': 1, 'max_chunk': 512, 'two_lane': 0, 'overlap': 1},
