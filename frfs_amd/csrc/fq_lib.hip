@@ -1135,7 +1135,10 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
                     }
                     nb[5 * N * N + (G + 4 * kk) * N + y] = sI;               // I: (vz, vy)
                 }
-                __syncthreads();                                           // LDS reads done
+                // LDS reads done before the region is written again: by the
+                // forward transforms after the pair's last group, or by the
+                // next group's z stage, which is preceded by its own barrier
+                if (m == 1 || 2 * gv + 1 >= ngroups) __syncthreads();
             }
             if (m == 0) {
 #pragma unroll
