@@ -28,8 +28,9 @@ sys.path.insert(0, ROOT)
 PEAK = {'fp64': 78.6, 'fp32': 157.3}     # MI355X vector TFLOP/s (published)
 HBM_PEAK = 8000.0                          # GB/s
 # HBM bytes per DOF of the dominant (product) kernel at the 32^3 workload
-# (N = 32, M = 12, Nrho = 16, Mpre = 32, 4096 DOFs with 1024-DOF scratch:
-# batches of 840 (mixed, group pairs) and 864 (FP64) DOFs), from rocprofv3
+# (N = 32, M = 12, Nrho = 16, Mpre = 32, 4096 DOFs with 1024-DOF scratch and
+# four DOF slots per accumulation workgroup: batches of 840 (mixed, group
+# pairs) and 864 (FP64) DOFs), from rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE runs of this script, scaled by what the same
 # counters read for a 1 GiB device copy (FETCH x2.0, WRITE x1.0).  The bench
 # line carries this figure with its source, and only for that shape.
@@ -38,7 +39,7 @@ K_PROD_TRAFFIC = {
     'shape': {'Nv': 32, 'M': 12, 'Nrho': 16, 'Mpre': 32},
     'per_dof': {'mixed': 17.67e6, 'fp64': 61.63e6},
     # the accumulation kernel (HBM-bound), same runs
-    'accum_per_dof': {'mixed': 14.27e6, 'fp64': 58.45e6},
+    'accum_per_dof': {'mixed': 14.43e6, 'fp64': 58.43e6},
 }
 
 

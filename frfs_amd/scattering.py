@@ -67,6 +67,8 @@ def _flags(cfg):
         fl |= _lib.FLAG_NO_HPAIR
     if cfg.getint('scattering-model', 'overlap', 0):
         fl |= _lib.FLAG_OVERLAP
+    if cfg.getint('scattering-model', 'accum-quad', 0):
+        fl |= _lib.FLAG_ACCUM_QUAD
     return fl
 
 

@@ -122,3 +122,16 @@ def test_tuned_schedule_matches_default(backend):
     again = DGFSBiVSSGLLScatteringModel(backend, cfg, vm)
     assert again.autotune(fin, q1)['cached']
     autotune.clear_cache()
+
+
+def test_schedule_keys_map_to_flags():
+    """Every [scattering-model] schedule key reaches its library flag, so a
+    schedule the autotuner picks can be given explicitly."""
+    from frfs_amd import _lib
+    from frfs_amd.scattering import _flags
+    from tests.cases import vss_cfg
+    assert _flags(vss_cfg()) == 0
+    cfg = vss_cfg(project=1, fused=0, lanes=2, pack=0, hpair=0, overlap=1)
+    cfg.set('scattering-model', 'accum-quad', 1)
+    assert _flags(cfg) == (_lib.FLAG_PROJECT | _lib.FLAG_UNFUSED | _lib.FLAG_TWO_LANE | _lib.FLAG_NO_PACK
+                           | _lib.FLAG_NO_HPAIR | _lib.FLAG_OVERLAP | _lib.FLAG_ACCUM_QUAD)
