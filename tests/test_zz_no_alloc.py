@@ -38,12 +38,15 @@ def _call(h, fn, fin, qout, ndof):
 
 
 # (label, config, vss?, DOFs of the first and of the second call).  The 32^3
-# mixed case takes 9 DOFs per batch of the static scratch, so its 20 DOFs
-# run as three batches.
+# mixed case (Nrho = 2, group pairs) needs 3.85 MiB of scratch per DOF, so 66
+# DOFs fit in a batch of the static scratch and its first call of 150 DOFs
+# runs as three batches of 50.  At Nrho = 16 with the projection (the
+# headline workload's plan) 13 DOFs fit, so 30 DOFs run as three batches of 10.
 CASES = [
     ('vhs16-fp64', lambda: vhs_cfg(16, 8, 6, 5.0), False, 70, 33),
     ('vhs24-mixed-proj', lambda: vhs_cfg(24, 4, 6, 5.0, precision='mixed', project=1), False, 40, 7),
-    ('vss32-mixed', lambda: vss_cfg(32, 2, 12, 6.0, precision='mixed'), True, 20, 5),
+    ('vss32-mixed', lambda: vss_cfg(32, 2, 12, 6.0, precision='mixed'), True, 150, 5),
+    ('vss32-mixed-proj-nrho16', lambda: vss_cfg(32, 16, 12, 6.0, precision='mixed', project=1), True, 30, 7),
 ]
 
 
