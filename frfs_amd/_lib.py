@@ -31,7 +31,9 @@ FLAG_NO_SCRATCH = 16
 FLAG_ACCUM_QUAD = 32
 FLAG_NO_HPAIR = 64
 FLAG_OVERLAP = 128
-PATHS = ((1, 'fused'), (2, 'one_lane'), (4, 'pair_pack'), (8, 'group_pairs'), (16, 'overlap'))
+FLAG_LIBRARY_FFT = 256
+PATHS = ((1, 'fused'), (2, 'one_lane'), (4, 'pair_pack'), (8, 'group_pairs'), (16, 'overlap'),
+         (32, 'library_fft'))
 NOALLOC_SCRATCH_BYTES = 256 << 20
 
 _dp = C.POINTER(C.c_double)

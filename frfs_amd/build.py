@@ -37,7 +37,7 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + '.tmp'
     cmd = [hipcc(), *HIP_FLAGS, *(extra or []), '-I', os.path.join(HERE, '..', 'include'),
-           *srcs, '-o', tmp]
+           *srcs, '-lhipfft', '-o', tmp]
     if verbose:
         print(' '.join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -26,6 +26,7 @@
 // puts one pencil on each lane (FP32, N = 24, 32).  Global loads and stores
 // walk J across lanes, so every access is a contiguous row segment.
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 
 #include <algorithm>
 #include <cmath>
@@ -2211,6 +2212,112 @@ __global__ __launch_bounds__(256) void k_project_b(AosoaPtrs ptr, Dof dof, int n
 }
 
 // ---------------------------------------------------------------------------
+// Grids without native kernels (any even N other than 8, 16, 24 and 32).
+// The batched 3D transforms run in the ROCm FFT library (hipFFT plans, the
+// counterpart of the reference's cuFFT plans, frfs/solvers/dgfs/scattering.py:
+// 141-150 and frfs/solvers/dgfs/cufft.py:231-281), and the pointwise stages of
+// the plan in the kernels below:
+//   F = DFT-(f)/n;  per product A = phi_1 F, B = phi_2 F, both DFT+;
+//   G_g = sum coef A B, DFT-;  Y_t[0] += W_c G_g;  Y_t[1] += W_l F;
+//   Y DFT+;  Q_t = Re Y_t[0] - f_t Re Y_t[1].
+// Volumes are [slot][.][K][I][J] (velocity (I, J, K), wavenumbers
+// (l0[I], l0[J], l0[K])) as everywhere in this file.  A batch of D DOFs sits
+// in scratch sized for `chunk` slots; the library plans are made once for
+// `chunk`, and the slots D..chunk-1 they also transform are independent of
+// the others and never read back.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void gen_kij(int64_t idx, int N, int &K, int &I, int &J) {
+    const int64_t n2 = int64_t(N) * N;
+    K = int(idx / n2);
+    I = int((idx / N) % N);
+    J = int(idx % N);
+}
+
+// F[slot][s] = f_s / n (real) for the batch's slots, zero for the others.
+__global__ __launch_bounds__(256) void k_gen_gather(AosoaPtrs ptr, Dof dof, cpx<double> *F, int nsrc, int D,
+                                                    int N, double inv_n, int64_t total) {
+    const int64_t n = int64_t(N) * N * N;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t idx = i % n, ds = i / n;
+        const int s = int(ds % nsrc), slot = int(ds / nsrc);
+        double v = 0.0;
+        if (slot < D) {
+            int K, I, J;
+            gen_kij(idx, N, K, I, J);
+            v = ptr.f[s][dof.base(slot) + ((int64_t(I) * N + J) * N + K) * dof.soasz] * inv_n;
+        }
+        F[i] = {v, 0.0};
+    }
+}
+
+// A = phi(theta_i1) F_s1 and B = phi(theta_i2) F_s2 (B = A + bofs) with the
+// separable phases ph[(i * 3 + d) * N + k] of the axes d = x (I), y (J), z (K).
+template <typename T>
+__global__ __launch_bounds__(256) void k_gen_phase(const cpx<double> *F, const cpx<T> *ph, int i1, int s1,
+                                                   int i2, int s2, int nsrc, cpx<T> *A, int64_t bofs, int N,
+                                                   int64_t total) {
+    const int64_t n = int64_t(N) * N * N;
+    const cpx<T> *p1 = ph + int64_t(i1) * 3 * N, *p2 = ph + int64_t(i2) * 3 * N;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t idx = i % n, slot = i / n;
+        int K, I, J;
+        gen_kij(idx, N, K, I, J);
+        const cpx<T> f1 = cvt<T>(F[(slot * nsrc + s1) * n + idx]), f2 = cvt<T>(F[(slot * nsrc + s2) * n + idx]);
+        A[i] = fq::cmul(f1, fq::cmul(fq::cmul(p1[I], p1[N + J]), p1[2 * N + K]));
+        A[bofs + i] = fq::cmul(f2, fq::cmul(fq::cmul(p2[I], p2[N + J]), p2[2 * N + K]));
+    }
+}
+
+// G = coef A B (first product of the group) or G += coef A B.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gen_prod(const cpx<T> *A, int64_t bofs, cpx<T> *G, T coef, int first,
+                                                  int64_t total) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const cpx<T> p = fq::cscale(fq::cmul(A[i], A[bofs + i]), coef);
+        G[i] = first ? p : G[i] + p;
+    }
+}
+
+// Y[slot * ystride + idx] += W[idx] X[slot * xstride + idx]  (FP64)
+template <typename T>
+__global__ __launch_bounds__(256) void k_gen_wmac(const cpx<T> *X, int64_t xstride, const cpx<double> *W,
+                                                  cpx<double> *Y, int64_t ystride, int N, int64_t total) {
+    const int64_t n = int64_t(N) * N * N;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t idx = i % n, slot = i / n;
+        cpx<double> &y = Y[slot * ystride + idx];
+        y = y + fq::cmul(W[idx], cvt<double>(X[slot * xstride + idx]));
+    }
+}
+
+// zero fill (kept out of hipMemsetAsync so that a captured graph replays it)
+__global__ __launch_bounds__(256) void k_gen_zero(cpx<double> *Y, int64_t total) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += int64_t(gridDim.x) * blockDim.x)
+        Y[i] = {0.0, 0.0};
+}
+
+// Q_t = Re DFT+(H_t) - f_t Re DFT+(L_t) from Y[slot][t][0 = gain, 1 = loss].
+__global__ __launch_bounds__(256) void k_gen_out(const cpx<double> *Y, AosoaPtrs ptr, Dof dof, int ntgt, int N,
+                                                 int64_t total) {
+    const int64_t n = int64_t(N) * N * N;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t idx = i % n, dt = i / n;
+        const int t = int(dt % ntgt), slot = int(dt / ntgt);
+        int K, I, J;
+        gen_kij(idx, N, K, I, J);
+        const int64_t a = dof.base(slot) + ((int64_t(I) * N + J) * N + K) * dof.soasz;
+        const cpx<double> *y = Y + (int64_t(slot) * ntgt + t) * 2 * n + idx;
+        ptr.q[t][a] = y[0].x - ptr.f[t][a] * y[n].x;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 template <typename X> int upload(X **dst, const std::vector<X> &v) {
@@ -2221,7 +2328,10 @@ template <typename X> int upload(X **dst, const std::vector<X> &v) {
     return FQ_SUCCESS;
 }
 
+// grids with native transform kernels; other even N up to kGenMaxN take the
+// library-transform path
 bool n_supported(int N) { return N == 8 || N == 16 || N == 24 || N == 32; }
+constexpr int kGenMaxN = 128;
 
 }  // namespace
 
@@ -2285,6 +2395,13 @@ struct fq_handle {
     float *NB = nullptr;         // scratch [chunk][groups][kNbPer32]
     cpx<double> *Cn = nullptr;   // scratch [chunk][targets][kCPer32]
     double *MOM = nullptr;       // scratch [chunk][targets][N][5]
+    // grids without native kernels: hipFFT plans for `chunk` slots (F,
+    // the items A and B, the group volume G, the gain and loss Y) sharing one
+    // work area, and the contributions of each group
+    bool generic = false;
+    hipfftHandle gp_src = nullptr, gp_items = nullptr, gp_group = nullptr, gp_out = nullptr;
+    void *gen_work = nullptr;
+    std::vector<std::vector<int>> g_contribs;
     // scratch
     cpx<double> *F = nullptr;
     void *Z = nullptr;           // also T0 and Y
@@ -2815,6 +2932,131 @@ template <int N, typename T> int run_chunk(fq_handle *h, const Scratch &sc, cons
     return FQ_SUCCESS;
 }
 
+// One batch of D DOFs on a grid without native kernels (see k_gen_gather):
+// library transforms of `chunk` slots and grid-stride pointwise kernels.
+#define FFTCHK(x)                                                                          \
+    do {                                                                                   \
+        hipfftResult r_ = (x);                                                             \
+        if (r_ != HIPFFT_SUCCESS) return fail(FQ_ERR_HIP, "hipFFT error " + std::to_string(int(r_)) + " in " #x); \
+    } while (0)
+
+template <typename T> hipfftResult gen_exec(hipfftHandle p, cpx<T> *x, int dir) {
+    if constexpr (sizeof(T) == 8)
+        return hipfftExecZ2Z(p, reinterpret_cast<hipfftDoubleComplex *>(x), reinterpret_cast<hipfftDoubleComplex *>(x),
+                             dir);
+    else
+        return hipfftExecC2C(p, reinterpret_cast<hipfftComplex *>(x), reinterpret_cast<hipfftComplex *>(x), dir);
+}
+
+template <typename T>
+int run_chunk_gen(fq_handle *h, const Scratch &sc, const AosoaPtrs &ptr, const Dof &dof, int D, bool prof,
+                  hipStream_t st) {
+    const fq::Plan &P = h->P;
+    const int N = P.N, nsrc = P.nsrc, ntgt = P.ntgt;
+    const int64_t n = int64_t(N) * N * N, chunk = h->chunk;
+    cpx<double> *F = sc.F;
+    cpx<double> *Y = reinterpret_cast<cpx<double> *>(sc.Z);
+    cpx<T> *A = reinterpret_cast<cpx<T> *>(sc.Pg);                  // A, then B at A + chunk n, then G
+    cpx<T> *Gv = A + 2 * chunk * n;
+    const cpx<T> *ph = reinterpret_cast<const cpx<T> *>(h->ph);
+    auto grid = [&](int64_t total) { return dim3(unsigned(std::min<int64_t>((total + 255) / 256, 16384))); };
+    const dim3 bt(256);
+    std::pair<hipEvent_t, hipEvent_t> cur{};
+    auto pre = [&]() {
+        if (!prof) return;
+        auto get = [&]() {
+            hipEvent_t e;
+            if (!h->ev_pool.empty()) { e = h->ev_pool.back(); h->ev_pool.pop_back(); }
+            else (void)hipEventCreate(&e);
+            return e;
+        };
+        cur = {get(), get()};
+        (void)hipEventRecord(cur.first, st);
+    };
+    auto post = [&](int k) {
+        if (prof) { (void)hipEventRecord(cur.second, st); h->ev_rec.push_back({k, cur}); }
+    };
+    for (hipfftHandle p : {h->gp_src, h->gp_items, h->gp_group, h->gp_out}) FFTCHK(hipfftSetStream(p, st));
+    // F = DFT-(f)/n
+    pre();
+    hipLaunchKernelGGL(k_gen_gather, grid(chunk * nsrc * n), bt, 0, st, ptr, dof, F, nsrc, D, N, 1.0 / double(n),
+                       chunk * nsrc * n);
+    FFTCHK(gen_exec<double>(h->gp_src, F, HIPFFT_FORWARD));
+    post(0);
+    // the loss spectra Y_t[1] = sum_l W_l F_src(l); the gain Y_t[0] starts at zero
+    pre();
+    hipLaunchKernelGGL(k_gen_zero, grid(int64_t(D) * ntgt * 2 * n), bt, 0, st, Y, int64_t(D) * ntgt * 2 * n);
+    for (size_t l = 0; l < P.loss.size(); ++l)
+        hipLaunchKernelGGL(k_gen_wmac<double>, grid(D * n), bt, 0, st, F + int64_t(P.loss[l].src) * n, nsrc * n,
+                           h->W + (int64_t(P.contribs.size()) + int64_t(l)) * n, Y + (int64_t(P.loss[l].tgt) * 2 + 1) * n,
+                           int64_t(ntgt) * 2 * n, N, D * n);
+    post(4);
+    for (int g = 0; g < P.ngroups; ++g) {
+        pre();
+        for (int k = 0; k < P.group_np[g]; ++k) {
+            const fq::PProd &pr = P.prods[P.group_p0[g] + k];
+            hipLaunchKernelGGL(k_gen_phase<T>, grid(D * n), bt, 0, st, F, ph, pr.i1, P.items[pr.i1].src, pr.i2,
+                               P.items[pr.i2].src, nsrc, A, chunk * n, N, D * n);
+            FFTCHK(gen_exec<T>(h->gp_items, A, HIPFFT_BACKWARD));
+            hipLaunchKernelGGL(k_gen_prod<T>, grid(D * n), bt, 0, st, A, chunk * n, Gv, T(pr.coef), k == 0 ? 1 : 0,
+                               D * n);
+        }
+        FFTCHK(gen_exec<T>(h->gp_group, Gv, HIPFFT_FORWARD));
+        post(7);
+        pre();
+        for (int c : h->g_contribs[g])
+            hipLaunchKernelGGL(k_gen_wmac<T>, grid(D * n), bt, 0, st, Gv, n, h->W + int64_t(c) * n,
+                               Y + int64_t(P.contribs[c].tgt) * 2 * n, int64_t(ntgt) * 2 * n, N, D * n);
+        post(4);
+    }
+    // Q_t = Re DFT+(H_t) - f_t Re DFT+(L_t)
+    pre();
+    FFTCHK(gen_exec<double>(h->gp_out, Y, HIPFFT_BACKWARD));
+    hipLaunchKernelGGL(k_gen_out, grid(int64_t(D) * ntgt * n), bt, 0, st, Y, ptr, dof, ntgt, N,
+                       int64_t(D) * ntgt * n);
+    post(5);
+    if (h->flags & FQ_FLAG_PROJECT) {
+        pre();
+        hipLaunchKernelGGL(k_project, dim3(unsigned(8 * ((D + 7) / 8))), dim3(256), 0, st, ptr, dof, ntgt, N, P.L,
+                           h->ginv, h->pmass, int(D));
+        post(6);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(FQ_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return FQ_SUCCESS;
+}
+
+// hipFFT plans of a generic-grid handle for its `chunk` slots, with one work
+// area of the largest size any of them asks for.
+int make_gen_plans(fq_handle *h) {
+    const fq::Plan &P = h->P;
+    const int N = P.N;
+    const int n = N * N * N;
+    int dims[3] = {N, N, N};
+    const hipfftType tt = h->precision == FQ_PREC_MIXED ? HIPFFT_C2C : HIPFFT_Z2Z;
+    const int64_t c = h->chunk;
+    struct { hipfftHandle *p; hipfftType t; int64_t batch; } pl[4] = {
+        {&h->gp_src, HIPFFT_Z2Z, c * P.nsrc}, {&h->gp_items, tt, 2 * c}, {&h->gp_group, tt, c},
+        {&h->gp_out, HIPFFT_Z2Z, c * P.ntgt * 2}};
+    size_t wmax = 0;
+    for (auto &q : pl) {
+        if (q.batch > INT32_MAX / 2) return fail(FQ_ERR_UNSUPPORTED, "batch too large for the FFT library plans");
+        size_t ws = 0;
+        FFTCHK(hipfftCreate(q.p));
+        FFTCHK(hipfftSetAutoAllocation(*q.p, 0));
+        FFTCHK(hipfftMakePlanMany(*q.p, 3, dims, nullptr, 1, n, nullptr, 1, n, q.t, int(q.batch), &ws));
+        wmax = std::max(wmax, ws);
+    }
+    if (wmax > 0) {
+        hipError_t e = hipMalloc(&h->gen_work, wmax);
+        if (e != hipSuccess) return fail(FQ_ERR_ALLOC, "hipMalloc FFT work area failed (reduce max_chunk)");
+        for (auto &q : pl) FFTCHK(hipfftSetWorkArea(*q.p, h->gen_work));
+    }
+    h->g_contribs.assign(P.ngroups, {});
+    for (size_t ci = 0; ci < P.contribs.size(); ++ci) h->g_contribs[P.contribs[ci].group].push_back(int(ci));
+    return FQ_SUCCESS;
+}
+
 template <int N>
 int dispatch_prec(fq_handle *h, const Scratch &sc, const AosoaPtrs &ptr, const Dof &dof, int D, bool prof,
                   hipStream_t st) {
@@ -2851,7 +3093,7 @@ int eval_impl(fq_handle *h, const Scratch *sc, int nsets, int64_t chunk, bool pr
     // whose k_accum_lds workgroups (NS DOFs x N rows each) fill whole rounds
     // of the device's workgroup slots when the scratch holds at least that
     // many: the last batch is then the short one
-    if (h->P.N == 32 && h->fused && nbatch > 1) {
+    if (h->P.N == 32 && h->fused && !h->generic && nbatch > 1) {
         // FP32: 8 slots per workgroup and one workgroup per CU, or 4 slots and
         // two; FP64: 4 slots, three workgroups per CU
         // group pairs: 8 slots x 17 row pairs per workgroup, one workgroup per CU
@@ -2875,6 +3117,12 @@ int eval_impl(fq_handle *h, const Scratch *sc, int nsets, int64_t chunk, bool pr
         const Scratch &s = sc[two ? i % 2 : 0];
         const hipStream_t bs = two && (i % 2) ? h->st2 : st;
         int rc;
+        if (h->generic) {
+            rc = h->precision == FQ_PREC_MIXED ? run_chunk_gen<float>(h, s, ptr, dof, D, prof, bs)
+                                               : run_chunk_gen<double>(h, s, ptr, dof, D, prof, bs);
+            if (rc != FQ_SUCCESS) return rc;
+            continue;
+        }
         switch (h->P.N) {
         case 8: rc = dispatch_prec<8>(h, s, ptr, dof, D, prof, bs); break;
         case 16: rc = dispatch_prec<16>(h, s, ptr, dof, D, prof, bs); break;
@@ -3031,7 +3279,12 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     *out = nullptr;
     if (prm->precision != FQ_PREC_FP64 && prm->precision != FQ_PREC_MIXED)
         return fail(FQ_ERR_INVALID, "unknown precision");
-    if (!n_supported(prm->Nv)) return fail(FQ_ERR_UNSUPPORTED, "Nv must be one of 8, 16, 24, 32");
+    const bool generic = !n_supported(prm->Nv) || (prm->flags & FQ_FLAG_LIBRARY_FFT);
+    if (generic && (prm->Nv < 4 || prm->Nv > kGenMaxN || prm->Nv % 2))
+        return fail(FQ_ERR_UNSUPPORTED, "Nv must be even, from 4 to " + std::to_string(kGenMaxN));
+    if (generic && (prm->flags & FQ_FLAG_NO_SCRATCH))
+        return fail(FQ_ERR_UNSUPPORTED, "FQ_FLAG_NO_SCRATCH needs the native kernels (Nv 8, 16, 24, 32, "
+                                         "without FQ_FLAG_LIBRARY_FFT)");
     if (prm->soasz != 32) return fail(FQ_ERR_UNSUPPORTED, "soasz must be 32");
     auto *h = new fq_handle();
     std::string err;
@@ -3041,11 +3294,13 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     }
     h->precision = prm->precision;
     h->flags = prm->flags;
-    h->fused = !(prm->flags & FQ_FLAG_UNFUSED);
-    h->two_lane = (prm->flags & FQ_FLAG_TWO_LANE) != 0;
-    h->pack = h->fused && !(h->two_lane && h->precision == FQ_PREC_MIXED) && (prm->Nv == 24 || prm->Nv == 32) &&
-              !(prm->flags & FQ_FLAG_NO_PACK);
-    h->nvol = h->P.ngroups;
+    h->generic = generic;
+    // the schedule flags select native kernels; the library path has one schedule
+    h->fused = generic || !(prm->flags & FQ_FLAG_UNFUSED);
+    h->two_lane = !generic && (prm->flags & FQ_FLAG_TWO_LANE) != 0;
+    h->pack = !generic && h->fused && !(h->two_lane && h->precision == FQ_PREC_MIXED) &&
+              (prm->Nv == 24 || prm->Nv == 32) && !(prm->flags & FQ_FLAG_NO_PACK);
+    h->nvol = generic ? 3 : h->P.ngroups;         // library path: the items A, B and the group G
     if (h->pack && !h->two_lane && prm->Nv == 32 && h->precision == FQ_PREC_MIXED && !(prm->flags & FQ_FLAG_NO_HPAIR)) {
         bool single = true;
         for (int g = 0; g < h->P.ngroups; ++g) single = single && h->P.group_np[g] == 1;
@@ -3117,13 +3372,15 @@ int fq_create(const fq_params *prm, fq_handle **out) {
     if (rc == FQ_SUCCESS && (h->flags & FQ_FLAG_NO_SCRATCH) && !h->noalloc)
         rc = fail(FQ_ERR_HIP, "hipGetSymbolAddress of the static scratch failed");
     // two scratch sets (FQ_FLAG_OVERLAP) share the budget; max_chunk bounds each set's batch
-    h->nsets = (h->flags & FQ_FLAG_OVERLAP) && !(h->flags & FQ_FLAG_NO_SCRATCH) ? 2 : 1;
+    // (the library path's plans share one work area: one set, one stream)
+    h->nsets = (h->flags & FQ_FLAG_OVERLAP) && !(h->flags & FQ_FLAG_NO_SCRATCH) && !generic ? 2 : 1;
     const size_t ns = size_t(h->nsets);
     int64_t chunk = prm->max_chunk;
     if (chunk <= 0) {
         const size_t budget = size_t(8) << 30;    // 8 GiB default scratch budget (of 288 GB HBM)
-        chunk = std::max<int64_t>(1, int64_t(budget / (per_dof * ns)));
-        chunk = std::min<int64_t>(chunk, 4096);
+        // the library path also holds an FFT work area of up to one Y per DOF
+        chunk = std::max<int64_t>(1, int64_t(budget / ((per_dof + (generic ? per_dof_Z : 0)) * ns)));
+        chunk = std::min<int64_t>(chunk, generic ? 512 : 4096);
     }
     h->chunk = chunk;
     if (rc == FQ_SUCCESS && (h->flags & FQ_FLAG_NO_SCRATCH)) {
@@ -3148,6 +3405,11 @@ int fq_create(const fq_params *prm, fq_handle **out) {
         if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess || e5 != hipSuccess)
             rc = fail(FQ_ERR_ALLOC, "hipMalloc scratch failed (reduce max_chunk)");
         h->scratch = per_dof * cs;
+        if (rc == FQ_SUCCESS && generic) {
+            // unused batch slots of A, B and G start as zeros, not stale bits
+            if (hipMemset(h->Pg, 0, per_dof_P * cs) != hipSuccess) rc = fail(FQ_ERR_HIP, "hipMemset scratch");
+            if (rc == FQ_SUCCESS) rc = make_gen_plans(h);
+        }
         if (rc == FQ_SUCCESS && h->nsets == 2) {
             // the second stream keeps the default priority
             if (hipStreamCreateWithFlags(&h->st2, hipStreamNonBlocking) != hipSuccess ||
@@ -3205,6 +3467,7 @@ int fq_get_paths(const fq_handle *h, int32_t *paths) {
     *paths = (h->fused ? FQ_PATH_FUSED : 0) | (one_lane ? FQ_PATH_ONE_LANE : 0) |
              (h->pack ? FQ_PATH_PAIR_PACK : 0) | (h->hpair ? FQ_PATH_GROUP_PAIRS : 0) |
              (h->nsets > 1 ? FQ_PATH_OVERLAP : 0);
+    if (h->generic) *paths = FQ_PATH_LIBRARY_FFT;
     return FQ_SUCCESS;
 }
 
@@ -3220,6 +3483,9 @@ int fq_destroy(fq_handle *h) {
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     if (h->ev_join) (void)hipEventDestroy(h->ev_join);
     if (h->st2) (void)hipStreamDestroy(h->st2);
+    for (hipfftHandle p : {h->gp_src, h->gp_items, h->gp_group, h->gp_out})
+        if (p) (void)hipfftDestroy(p);
+    if (h->gen_work) (void)hipFree(h->gen_work);
     for (void *p : {h->ph, (void *)h->item_src, (void *)h->prods, (void *)h->g_p0, (void *)h->g_np,
                     (void *)h->c_group, (void *)h->t_c0, (void *)h->t_cn, (void *)h->l_src, (void *)h->t_l0,
                     (void *)h->t_ln, (void *)h->W, (void *)h->ginv, (void *)h->pmass, (void *)h->F, h->Z,
@@ -3263,6 +3529,8 @@ int fq_eval_noalloc(fq_handle *h, const double *f, double *q, int32_t nupts, int
         return fail(FQ_ERR_INVALID, "fq_eval_noalloc needs a single-species model");
     if (!f || !q) return fail(FQ_ERR_INVALID, "null f or q");
     if (!h->noalloc) return fail(FQ_ERR_HIP, "static scratch not resolved on this handle's device");
+    if (h->generic)
+        return fail(FQ_ERR_UNSUPPORTED, "fq_eval_noalloc needs a grid with native kernels (8, 16, 24, 32)");
     // the largest batch whose regions fit the static scratch
     const size_t one = noalloc_bytes(h, 1);
     if (one > size_t(FQ_NOALLOC_SCRATCH_BYTES))

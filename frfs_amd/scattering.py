@@ -69,6 +69,8 @@ def _flags(cfg):
         fl |= _lib.FLAG_OVERLAP
     if cfg.getint('scattering-model', 'accum-quad', 0):
         fl |= _lib.FLAG_ACCUM_QUAD
+    if cfg.getint('scattering-model', 'library-fft', 0):
+        fl |= _lib.FLAG_LIBRARY_FFT
     return fl
 
 

@@ -91,7 +91,12 @@ enum fq_flags {
                                  stream of the handle (forked from and joined
                                  back to the caller's stream with events), so
                                  one batch's memory-bound stages can run beside
-                                 the next batch's product kernel */
+                                 the next batch's product kernel */,
+    FQ_FLAG_LIBRARY_FFT = 256 /* every 3D transform in the ROCm FFT library
+                                 (hipFFT) with pointwise kernels between them,
+                                 as on grids without native kernels (any even
+                                 Nv from 4 to 128 other than 8, 16, 24, 32);
+                                 for comparison on the native grids */
 };
 
 /* Size of the static device scratch of fq_eval_noalloc (one copy per GPU the
@@ -225,7 +230,8 @@ enum fq_path {
     FQ_PATH_ONE_LANE = 2,     /* FP32 product kernel with one lane per pencil */
     FQ_PATH_PAIR_PACK = 4,    /* the two items of a product as one complex volume */
     FQ_PATH_GROUP_PAIRS = 8,  /* two groups' real parts as one product volume */
-    FQ_PATH_OVERLAP = 16      /* two scratch sets and two streams */
+    FQ_PATH_OVERLAP = 16,     /* two scratch sets and two streams */
+    FQ_PATH_LIBRARY_FFT = 32  /* grid without native kernels: hipFFT transforms */
 };
 int fq_get_paths(const fq_handle *h, int32_t *paths);
 

@@ -133,5 +133,7 @@ def test_schedule_keys_map_to_flags():
     assert _flags(vss_cfg()) == 0
     cfg = vss_cfg(project=1, fused=0, lanes=2, pack=0, hpair=0, overlap=1)
     cfg.set('scattering-model', 'accum-quad', 1)
+    cfg.set('scattering-model', 'library-fft', 1)
     assert _flags(cfg) == (_lib.FLAG_PROJECT | _lib.FLAG_UNFUSED | _lib.FLAG_TWO_LANE | _lib.FLAG_NO_PACK
-                           | _lib.FLAG_NO_HPAIR | _lib.FLAG_OVERLAP | _lib.FLAG_ACCUM_QUAD)
+                           | _lib.FLAG_NO_HPAIR | _lib.FLAG_OVERLAP | _lib.FLAG_ACCUM_QUAD
+                           | _lib.FLAG_LIBRARY_FFT)
