@@ -82,6 +82,9 @@ def parse():
     ap.add_argument('--fno-cons-weight', type=float, default=10.0,
                     help='fno16/32: weight of the conservation penalty in training')
     ap.add_argument('--fno-batch', type=int, default=64, help='training batch (DOFs)')
+    ap.add_argument('--library-steps', type=int, default=2,
+                    help='timed steps of the library-FFT comparison (the same plan with every 3D '
+                         'transform in hipFFT); 0 skips it')
     ap.add_argument('--calib-copy-mib', type=int, default=0,
                     help='run one device-to-device copy of this size first (counter calibration)')
     return ap.parse_args()
@@ -95,7 +98,7 @@ def sm_opts(a, sm):
     return sm
 
 
-def vss_cfg(a, precision, project=0):
+def vss_cfg(a, precision, project=0, extra=None):
     from frfs_amd.inifile import Inifile
     return Inifile.from_dict({
         'non-dim': dict(T0=273.0, H0=1e-3, rho0=0.00011140244750324529, molarMass0=39.96e-3),
@@ -105,7 +108,7 @@ def vss_cfg(a, precision, project=0):
         'scattering-model': sm_opts(a, {'type': 'vss-gll', 'ssrulepre': 'womersley', 'Mpre': a.Mpre,
                                         'alpha00': 1.4, 'omega00': 0.74, 'dRef00': 4.17e-10,
                                         'Tref00': 273, 'precision': precision,
-                                        'conservation-projection': project}),
+                                        'conservation-projection': project, **(extra or {})}),
     })
 
 
@@ -701,6 +704,33 @@ def main():
         del sm, fin, qout
         torch.cuda.empty_cache()
 
+    # The same plan with every 3D transform in the ROCm FFT library (hipFFT)
+    # and pointwise kernels between them: the reference's structure (cuFFT
+    # plans plus pointwise kernels) on this GPU.  Untuned, wall clock over
+    # a few steps after one warmup step.
+    lib = {}
+    for mode in (m for m in ('fp64', 'mixedproj') if m in res and a.library_steps > 0):
+        cfg = vss_cfg(a, 'mixed' if mode.startswith('mixed') else 'double', int(mode == 'mixedproj'),
+                      {'library-fft': 1, 'max-chunk': 512})
+        vm = DGFSBiVelocityMesh(be, cfg, 1)
+        sm = DGFSBiVSSGLLScatteringModel(be, cfg, vm)
+        fin = be.matrix((1, vm.vsize(), a.ndof))
+        qout = be.matrix((1, vm.vsize(), a.ndof))
+        to_aosoa(mixture_device(vm.cv(), fpar), fin)
+        sm.fs_batch(fin, qout)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.library_steps):
+            sm.fs_batch(fin, qout)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / a.library_steps
+        q = from_aosoa(qout).cpu().numpy()
+        lib[mode] = {'value': a.ndof / dt, 'ms_per_step': dt * 1e3, 'paths': sm._h.paths(),
+                     'native_speedup': res[mode]['rate'] * dt / a.ndof,
+                     'rel_l2_vs_native': float(np.linalg.norm(q - qs[mode]) / np.linalg.norm(qs[mode]))}
+        del sm, fin, qout
+        torch.cuda.empty_cache()
+
     N = a.Nv
     n = N ** 3
     info = res[modes[0]]['info']
@@ -854,6 +884,11 @@ def main():
                                   'instructions (dense DFT stages as GEMMs cost more than the radix form)',
                      'kernel_ms_per_step': {k: v[0] / a.steps for k, v in tot.items()}},
         'cpu_baseline': cpu,
+        'library_fft_path': ({**lib, 'steps': a.library_steps, 'chunk_dofs': 512,
+                              'note': 'the same plan with every 3D transform in hipFFT (batched plans) and '
+                                      'pointwise kernels between them, as for grids without native kernels; '
+                                      'native_speedup = native rate / library rate of the same mode'}
+                             if lib else None),
     }
     if 'fp64' in res and head != 'fp64':
         rp = {k: v for k, v in res['fp64']['prof'].items() if v[1] > 0}
