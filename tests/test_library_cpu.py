@@ -418,3 +418,23 @@ def test_inifile_defaults():
     assert ini.getfloat('a', 'y', 2.5) == 2.5
     with pytest.raises(KeyError):
         ini.get('a', 'z')
+
+
+def test_grid_range_checked_before_the_device():
+    """fq_create refuses grids outside the library path's range (odd, below 4,
+    above 128) and FQ_FLAG_NO_SCRATCH off the native grids, before any HIP
+    call; the native grids and the even grids in range get past the check."""
+    for nv, msg in ((130, 'from 4 to 128'), (2, 'from 4 to 128')):
+        p, _ = _vhs_params(vhs_cfg(16, 8, 6, 5.0))
+        p.c.Nv = nv
+        with pytest.raises(_lib.FQError, match=msg):
+            _lib.Handle(p)
+    p, _ = _vhs_params(vhs_cfg(16, 8, 6, 5.0))
+    p.c.Nv = 20
+    p.c.flags |= _lib.FLAG_NO_SCRATCH
+    with pytest.raises(_lib.FQError, match='native kernels'):
+        _lib.Handle(p)
+    p, _ = _vhs_params(vhs_cfg(16, 8, 6, 5.0))
+    p.c.flags |= _lib.FLAG_NO_SCRATCH | _lib.FLAG_LIBRARY_FFT
+    with pytest.raises(_lib.FQError, match='native kernels'):
+        _lib.Handle(p)
