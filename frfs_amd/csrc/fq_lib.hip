@@ -3051,6 +3051,7 @@ int make_gen_plans(fq_handle *h) {
         hipError_t e = hipMalloc(&h->gen_work, wmax);
         if (e != hipSuccess) return fail(FQ_ERR_ALLOC, "hipMalloc FFT work area failed (reduce max_chunk)");
         for (auto &q : pl) FFTCHK(hipfftSetWorkArea(*q.p, h->gen_work));
+        h->scratch += wmax;                   // reported with the batch scratch
     }
     h->g_contribs.assign(P.ngroups, {});
     for (size_t ci = 0; ci < P.contribs.size(); ++ci) h->g_contribs[P.contribs[ci].group].push_back(int(ci));
