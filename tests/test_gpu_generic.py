@@ -14,7 +14,7 @@ from tests.test_gpu_parity import FP64_TOL, MIXED_TOL, _io, _run_vhs, _run_vss
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize('N', [12, 20])
+@pytest.mark.parametrize('N', [4, 6, 12, 20])
 @pytest.mark.parametrize('prec,tol', [('double', FP64_TOL), ('mixed', MIXED_TOL)])
 def test_vhs_generic_grid(backend, N, prec, tol):
     """VHS omega=0.5, M=6 half sphere, Nrho=4 on an N^3 grid without native kernels."""
@@ -28,6 +28,43 @@ def test_vhs_generic_grid(backend, N, prec, tol):
     errs = [rel_l2(Q[d], fsp.vhs_fs(f[d], vm, tab, prm['prefac'])) for d in range(len(f))]
     print(prec, 'max rel-L2', max(errs))
     assert max(errs) <= tol
+
+
+def test_vhs_48_against_oracle(backend):
+    """48^3 (a common grid above the native ones), one DOF of each precision
+    against the oracle."""
+    from oracle import fastspectral as fsp
+    base = vhs_cfg(48, 2, 6, 5.0)
+    vm, prm, tab = oracle_vhs(base)
+    f = fsp.two_maxwellian_batch(vm['cv'], 1, seed=48)
+    ref = fsp.vhs_fs(f[0], vm, tab, prm['prefac'])
+    for prec, tol in (('double', FP64_TOL), ('mixed', MIXED_TOL)):
+        print(f'[generic vhs N=48 {prec}]', flush=True)
+        Q, sm = _run_vhs(backend, vhs_cfg(48, 2, 6, 5.0, precision=prec), f, 1, 1)
+        assert sm._h.paths() == ['library_fft']
+        e = rel_l2(Q[0], ref)
+        print(prec, 'rel-L2', e)
+        assert e <= tol
+
+
+def test_vhs_largest_generic_grid(backend):
+    """The largest grid the library path takes (128^3, 2M points; 32 DOFs per
+    batch in the 8 GiB budget): Q(2f) = 4 Q(f) in FP64 (Q is quadratic) and
+    the mixed mode within 1e-5 of FP64, on 2 DOFs (the oracle takes minutes
+    here; 48^3 above is checked against it)."""
+    from oracle import fastspectral as fsp
+    cfg = vhs_cfg(128, 2, 6, 5.0)
+    vm, _, _ = oracle_vhs(cfg)
+    f = fsp.two_maxwellian_batch(vm['cv'], 2, seed=128)
+    print('[generic vhs N=128]', flush=True)
+    Q, sm = _run_vhs(backend, cfg, f, 1, 2)
+    assert sm._h.paths() == ['library_fft'] and sm._h.chunk_dofs() == 32
+    Q2, _ = _run_vhs(backend, cfg, 2 * f, 1, 2)
+    Qm, _ = _run_vhs(backend, vhs_cfg(128, 2, 6, 5.0, precision='mixed'), f, 1, 2)
+    for d in range(2):
+        assert np.all(np.isfinite(Q[d])) and np.abs(Q[d]).max() > 0
+        assert rel_l2(Q2[d], 4 * Q[d]) <= 1e-13
+        assert rel_l2(Qm[d], Q[d]) <= MIXED_TOL
 
 
 @pytest.mark.parametrize('prec,tol', [('double', FP64_TOL), ('mixed', MIXED_TOL)])
