@@ -9,7 +9,7 @@ as one packed volume or separately; on the 32^3 grid, eight or four DOFs per
 accumulation workgroup sharing each weight row, and one product volume per
 group pair or per group; batches on one stream, or alternating between two
 scratch sets and two streams; and the batch the scratch
-budget allows, a larger or a smaller one.
+budget allows, a larger or a smaller one (the largest for FP32 transforms only).
 ``tune`` builds one handle per candidate on the current device, times each on
 the caller's own inputs (one untimed call, then ``reps`` timed calls between
 device synchronisations) and releases it before the next is built, so at most
@@ -28,6 +28,9 @@ CANDIDATES = (
     {'fused': 1, 'max_chunk': 1024, 'two_lane': 0},
     {'fused': 1, 'max_chunk': 128, 'two_lane': 0},
     {'fused': 1, 'max_chunk': 1024, 'two_lane': 0, 'accum_quad': 1},
+    # FP32 transforms only: twice the batch (twice the scratch, 37 GiB at 32^3)
+    # fills whole rounds of the device in the Nyquist and projection kernels
+    {'fused': 1, 'max_chunk': 2048, 'two_lane': 0, 'accum_quad': 1, 'mixed_only': 1},
     {'fused': 1, 'max_chunk': 0, 'two_lane': 0, 'no_pack': 1},
     {'fused': 1, 'max_chunk': 0, 'two_lane': 0, 'no_hpair': 1},
     {'fused': 1, 'max_chunk': 512, 'two_lane': 0, 'overlap': 1},
@@ -87,8 +90,11 @@ def tune(params, run, key, reps=2, candidates=CANDIDATES):
         _apply(params, cand)
         return _lib.Handle(params), {'cached': True, 'choice': dict(cand)}
     best, times, seen = None, [], []
+    mixed = getattr(params.c, 'precision', _lib.PREC_MIXED) == _lib.PREC_MIXED
     for cand in candidates:
-        cand = _bounded(cand, limit)
+        if cand.get('mixed_only') and not mixed:
+            continue
+        cand = _bounded({k: v for k, v in cand.items() if k != 'mixed_only'}, limit)
         if cand in seen:                           # two candidates the limit made equal
             continue
         seen.append(cand)

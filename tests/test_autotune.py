@@ -84,9 +84,10 @@ def test_user_max_chunk_bounds_every_candidate(monkeypatch):
     p.c.max_chunk = 256
     h, rep = autotune.tune(p, lambda h: None, key=('bounded',), reps=1)
     assert all(mc == min(256, c['max_chunk'] or 256)
-               for (_, mc), c in zip(seen, [c for c in autotune.CANDIDATES if c['max_chunk'] != 1024]))
-    # fused/auto and fused/1024 collapse into one candidate under the bound
-    assert len(rep['timings']) == len(autotune.CANDIDATES) - 1
+               for (_, mc), c in zip(seen, [c for c in autotune.CANDIDATES if c['max_chunk'] not in (1024, 2048)]))
+    # fused/auto and fused/1024 collapse into one candidate under the bound, and
+    # so do the two four-slot candidates
+    assert len(rep['timings']) == len(autotune.CANDIDATES) - 2
     assert p.c.max_chunk == 256
     autotune.clear_cache()
 
@@ -137,3 +138,25 @@ def test_schedule_keys_map_to_flags():
     assert _flags(cfg) == (_lib.FLAG_PROJECT | _lib.FLAG_UNFUSED | _lib.FLAG_TWO_LANE | _lib.FLAG_NO_PACK
                            | _lib.FLAG_NO_HPAIR | _lib.FLAG_OVERLAP | _lib.FLAG_ACCUM_QUAD
                            | _lib.FLAG_LIBRARY_FFT)
+
+
+def test_mixed_only_candidates_skip_fp64(monkeypatch):
+    """Candidates marked for FP32 transforms are not built for an FP64 plan."""
+    from frfs_amd import _lib, autotune
+    built = []
+
+    class H(_FakeHandle):
+        def __init__(self, params):
+            built.append(params.c.max_chunk)
+            self.cost, self.closed = 0.0, False
+
+    monkeypatch.setattr(_lib, 'Handle', H)
+    monkeypatch.setattr(autotune, '_sync', lambda: None)
+    for prec, want in ((_lib.PREC_FP64, False), (_lib.PREC_MIXED, True)):
+        autotune.clear_cache()
+        built.clear()
+        p = _P()
+        p.c.precision = prec
+        autotune.tune(p, lambda h: None, key=('prec', prec), reps=1)
+        assert (2048 in built) == want
+    autotune.clear_cache()
