@@ -28,18 +28,18 @@ sys.path.insert(0, ROOT)
 PEAK = {'fp64': 78.6, 'fp32': 157.3}     # MI355X vector TFLOP/s (published)
 HBM_PEAK = 8000.0                          # GB/s
 # HBM bytes per DOF of the dominant (product) kernel at the 32^3 workload
-# (N = 32, M = 12, Nrho = 16, Mpre = 32, 4096 DOFs with 1024-DOF scratch and
-# four DOF slots per accumulation workgroup: batches of 840 (mixed, group
-# pairs) and 864 (FP64) DOFs), from rocprofv3
+# (N = 32, M = 12, Nrho = 16, Mpre = 32, 4096 DOFs with 2048-DOF scratch and
+# four DOF slots per accumulation workgroup: batches of 1440 DOFs; the
+# per-DOF bytes do not depend on the batch), from rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE runs of this script, scaled by what the same
 # counters read for a 1 GiB device copy (FETCH x2.0, WRITE x1.0).  The bench
 # line carries this figure with its source, and only for that shape.
 K_PROD_TRAFFIC = {
     'source': 'profiles/k_prod_vss32_traffic.txt',
     'shape': {'Nv': 32, 'M': 12, 'Nrho': 16, 'Mpre': 32},
-    'per_dof': {'mixed': 17.67e6, 'fp64': 61.63e6},
+    'per_dof': {'mixed': 17.67e6, 'fp64': 61.62e6},
     # the accumulation kernel (HBM-bound), same runs
-    'accum_per_dof': {'mixed': 14.53e6, 'fp64': 58.39e6},
+    'accum_per_dof': {'mixed': 14.44e6, 'fp64': 58.46e6},
 }
 
 
@@ -773,7 +773,7 @@ def main():
     if (dom == 'k_prod' and {'Nv': N, 'M': a.M, 'Nrho': a.Nrho, 'Mpre': a.Mpre} == K_PROD_TRAFFIC['shape']):
         traffic = K_PROD_TRAFFIC['per_dof']['mixed' if head.startswith('mixed') else 'fp64'] * dofs_per_launch
         traffic_src = (K_PROD_TRAFFIC['source'] + ': FETCH_SIZE + WRITE_SIZE per dispatch of this kernel at '
-                       'this shape (counter runs with 840-DOF mixed and 864-DOF FP64 batches), scaled by a '
+                       'this shape (counter runs with 1440-DOF batches), scaled by a '
                        '1 GiB copy, per DOF times the DOFs of one launch')
     # live HBM rate of the accumulation kernel (the operator's HBM-bound kernel):
     # its counter bytes per DOF x DOFs per launch / its HIP-event launch time
