@@ -79,9 +79,11 @@ def _apply(params, cand):
     c.max_chunk = cand['max_chunk']
 
 
-def tune(params, run, key, reps=2, candidates=CANDIDATES):
+def tune(params, run, key, reps=2, candidates=CANDIDATES, ndof=0):
     """Return ``(handle, report)``; ``run(handle)`` evaluates the operator on
-    the caller's inputs.  ``params`` is left with the chosen schedule."""
+    the caller's inputs (``ndof`` DOFs when known: no candidate's explicit
+    batch then exceeds it, so a small problem never holds scratch for more
+    DOFs than it has).  ``params`` is left with the chosen schedule."""
     orig = (params.c.flags, params.c.max_chunk)
     limit = int(orig[1])
     key = (key, limit)
@@ -95,6 +97,8 @@ def tune(params, run, key, reps=2, candidates=CANDIDATES):
         if cand.get('mixed_only') and not mixed:
             continue
         cand = _bounded({k: v for k, v in cand.items() if k != 'mixed_only'}, limit)
+        if ndof > 0 and cand['max_chunk'] > ndof:
+            cand = {**cand, 'max_chunk': int(ndof)}
         if cand in seen:                           # two candidates the limit made equal
             continue
         seen.append(cand)

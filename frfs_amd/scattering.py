@@ -126,7 +126,7 @@ class DGFSScatteringModel(metaclass=ABCMeta):
                self._params.c.Nv, info['n_items'], info['n_groups'], info['n_targets'], int(ndof),
                tuple(tuple(sorted(c.items())) for c in cands))
         self._h.close()
-        self._h, self.tune_report = _autotune.tune(self._params, run, key, candidates=cands)
+        self._h, self.tune_report = _autotune.tune(self._params, run, key, candidates=cands, ndof=int(ndof))
         self._tuned = True
 
     def _maybe_tune(self, run, ndof):

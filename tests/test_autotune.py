@@ -160,3 +160,26 @@ def test_mixed_only_candidates_skip_fp64(monkeypatch):
         autotune.tune(p, lambda h: None, key=('prec', prec), reps=1)
         assert (2048 in built) == want
     autotune.clear_cache()
+
+
+def test_explicit_batches_capped_at_the_dof_count(monkeypatch):
+    """With the DOF count known, no candidate's explicit batch exceeds it
+    (the automatic one stays automatic), and candidates the cap makes equal
+    run once."""
+    from frfs_amd import _lib, autotune
+    built = []
+
+    class H(_FakeHandle):
+        def __init__(self, params):
+            built.append(params.c.max_chunk)
+            self.cost, self.closed = 0.0, False
+
+    monkeypatch.setattr(_lib, 'Handle', H)
+    monkeypatch.setattr(autotune, '_sync', lambda: None)
+    autotune.clear_cache()
+    p = _P()
+    h, rep = autotune.tune(p, lambda h: None, key=('ndof',), reps=1, ndof=100)
+    assert max(built) == 100 and 0 in built
+    chunks = [(t['max_chunk'], t.get('accum_quad', 0)) for t in rep['timings']]
+    assert chunks.count((100, 1)) == 1           # the 1024 and 2048 four-slot candidates merge
+    autotune.clear_cache()
