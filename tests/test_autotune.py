@@ -114,7 +114,17 @@ def test_tuned_schedule_matches_default(backend):
     tuned = DGFSBiVSSGLLScatteringModel(backend, cfg, vm)
     rep = tuned.autotune(fin, q1)
     print('autotune report', rep)
-    assert not rep['cached'] and len(rep['timings']) == len(autotune.CANDIDATES)
+    # every candidate runs once: FP64 skips the FP32-only one, and batches
+    # capped at the 40 DOFs of the call merge
+    want = []
+    for c in autotune.CANDIDATES:
+        if c.get('mixed_only'):
+            continue
+        c = {k: v for k, v in c.items() if k != 'mixed_only'}
+        c = {**c, 'max_chunk': min(c['max_chunk'], ndof)}
+        if c not in want:
+            want.append(c)
+    assert not rep['cached'] and len(rep['timings']) == len(want)
     q1.set(np.zeros(q1.ioshape))
     tuned.fs_batch(fin, q1)
     torch.cuda.synchronize()

@@ -30,6 +30,32 @@ def test_vhs_generic_grid(backend, N, prec, tol):
     assert max(errs) <= tol
 
 
+def test_vhs_nodal_generic_grid(backend):
+    """The bilinear vhs-gll-nodal form (two sources) on 12^3, FP64."""
+    import torch
+    from oracle import fastspectral as fsp
+    from frfs_amd.scattering import DGFSVHSGLLNodalScatteringModel
+    from frfs_amd.velocitymesh import DGFSVelocityMesh
+    from tests.test_gpu_parity import _from_io
+    cfg = vhs_cfg(12, 3, 6, 5.0, model='vhs-gll-nodal')
+    vm, prm, tab = oracle_vhs(cfg)
+    f = fsp.two_maxwellian_batch(vm['cv'], 2, seed=91)
+    g = fsp.two_maxwellian_batch(vm['cv'], 2, seed=92)
+    hvm = DGFSVelocityMesh(backend, cfg)
+    sm = DGFSVHSGLLNodalScatteringModel(backend, cfg, hvm)
+    assert sm._h.paths() == ['library_fft']
+    n = hvm.vsize()
+    fi = backend.matrix((1, n, 2), _io(f, 1, 2))
+    gi = backend.matrix((1, n, 2), _io(g, 1, 2))
+    qo = backend.matrix((1, n, 2))
+    print('[generic nodal N=12]', flush=True)
+    sm.fs_batch(fi, gi, qo)
+    torch.cuda.synchronize()
+    Q = _from_io(qo.get())
+    errs = [rel_l2(Q[d], fsp.vhs_nodal_fs(f[d], g[d], vm, tab, prm['prefac'])) for d in range(2)]
+    assert max(errs) <= FP64_TOL
+
+
 def test_vhs_48_against_oracle(backend):
     """48^3 (a common grid above the native ones), one DOF of each precision
     against the oracle."""
