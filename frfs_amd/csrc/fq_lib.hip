@@ -112,7 +112,7 @@ template <> __device__ __forceinline__ cpx<double> BufView::ld<double>(uint32_t 
     return __builtin_bit_cast(cpx<double>, __builtin_amdgcn_raw_buffer_load_b128(r, int(voff), int(soff), 0));
 }
 
-// clang vector types for __builtin_nontemporal_load
+// clang vector types for __builtin_nontemporal_load / _store
 typedef float nt_f2 __attribute__((ext_vector_type(2)));
 typedef float nt_f4 __attribute__((ext_vector_type(4)));
 
@@ -1170,13 +1170,16 @@ __global__ __launch_bounds__(Prod1Geo<N>::THREADS, 2) void k_prod1(
     cpx<T> *dst = P + (int64_t(slot) * nvol + gv) * G3::VOL + int64_t(vz) * G3::N2;
     // 16-byte stores: lanes c and c ^ 1 swap one value per row pair, the
     // even lane then writes row 2m and the odd lane row 2m + 1 of columns
-    // (c & ~1, c | 1)
+    // (c & ~1, c | 1).  Non-temporal: the volumes are read once, by the
+    // accumulation kernel after this kernel has finished (1% faster)
     const bool odd = c & 1;
 #pragma unroll
     for (int m = 0; m < N / 2; ++m) {
         const cpx<T> recv = fq::swap_adj(odd ? acc[2 * m] : acc[2 * m + 1]);
         const cpx2f w = odd ? cpx2f{recv, acc[2 * m + 1]} : cpx2f{acc[2 * m], recv};
-        if (act) *reinterpret_cast<cpx2f *>(dst + (2 * m + (odd ? 1 : 0)) * N + (c & ~1)) = w;
+        if (act)
+            __builtin_nontemporal_store(nt_f4{w.a.x, w.a.y, w.b.x, w.b.y},
+                                        reinterpret_cast<nt_f4 *>(dst + (2 * m + (odd ? 1 : 0)) * N + (c & ~1)));
     }
 }
 
@@ -1279,13 +1282,16 @@ __global__ __launch_bounds__(96, (sizeof(T) == 4 ? 4 : 2)) void k_nyq(const cpx<
     }
     cpx<T> *out = NY + ((int64_t(slot) * nprods + pi) * 3 + piece) * G3::N2;
     if constexpr (sizeof(T) == 4) {
-        // 16-byte stores after a swap with lane c ^ 1 (as in k_prod1)
+        // 16-byte non-temporal stores after a swap with lane c ^ 1 (as in
+        // k_prod1): the planes are read by the next kernel
         const bool odd = c & 1;
 #pragma unroll
         for (int m = 0; m < N / 2; ++m) {
             const cpx<T> recv = fq::swap_adj(odd ? v[2 * m] : v[2 * m + 1]);
             const cpx2f w = odd ? cpx2f{recv, v[2 * m + 1]} : cpx2f{v[2 * m], recv};
-            if (act) *reinterpret_cast<cpx2f *>(out + (2 * m + (odd ? 1 : 0)) * N + (c & ~1)) = w;
+            if (act)
+                __builtin_nontemporal_store(nt_f4{w.a.x, w.a.y, w.b.x, w.b.y},
+                                            reinterpret_cast<nt_f4 *>(out + (2 * m + (odd ? 1 : 0)) * N + (c & ~1)));
         }
     } else {
         if (act) {
