@@ -281,7 +281,8 @@ __global__ __launch_bounds__(256) void k_fwd_z4(AosoaPtrs ptr, Dof dof, cpx<doub
     fq::fft2_dif<N, -1>(v, half);
     cpx<double> *dst = T0 + (int64_t(slot) * nsrc + s) * G::VOL + int64_t(I) * N + J;
 #pragma unroll
-    for (int e = 0; e < E; ++e) dst[int64_t(half + 2 * e) * G::N2] = v[e];
+    for (int e = 0; e < E; ++e)        // non-temporal: read once, by k_fwd_xy
+        __builtin_nontemporal_store(nt_d2{v[e].x, v[e].y}, reinterpret_cast<nt_d2 *>(dst + int64_t(half + 2 * e) * G::N2));
 }
 
 // ---------------------------------------------------------------------------
@@ -310,7 +311,14 @@ __global__ __launch_bounds__(kPlaneThreads) void k_fwd_xy(const cpx<double> *T0,
     const cpx<double> *src = T0 + unit * G::N2;
     cpx<double> v[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = act ? src[(half * E + e) * N + c] : cpx<double>{0, 0};
+    for (int e = 0; e < E; ++e) {                // T0 is read once: non-temporal
+        cpx<double> t = {0, 0};
+        if (act) {
+            const nt_d2 x = __builtin_nontemporal_load(reinterpret_cast<const nt_d2 *>(src + (half * E + e) * N + c));
+            t = {x.x, x.y};
+        }
+        v[e] = t;
+    }
     fq::fft2_dif<N, -1>(v, half);                       // over I -> I' cyclic
     cpx<double> *L = lds[u];
     if (act) {
