@@ -912,8 +912,16 @@ __global__ __launch_bounds__(ProdGeo<N>::THREADS, (sizeof(T) == 4 ? 4 : 2)) void
     fq::fft2_dit<N, -1>(acc, half);                       // -> I' = half*E + e
     cpx<T> *dst = P + (int64_t(slot) * ngroups + g) * G3::VOL + int64_t(vz) * G3::N2;
     if (act) {
+        // FP64: non-temporal stores, the volumes are read once by the
+        // accumulation kernel (6% faster)
 #pragma unroll
-        for (int e = 0; e < E; ++e) dst[(half * E + e) * N + c] = acc[e];
+        for (int e = 0; e < E; ++e) {
+            if constexpr (sizeof(T) == 8)
+                __builtin_nontemporal_store(nt_d2{acc[e].x, acc[e].y},
+                                            reinterpret_cast<nt_d2 *>(dst + (half * E + e) * N + c));
+            else
+                dst[(half * E + e) * N + c] = acc[e];
+        }
     }
 }
 
