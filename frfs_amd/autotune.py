@@ -27,9 +27,9 @@ CANDIDATES = (
     {'fused': 1, 'max_chunk': 0, 'two_lane': 0},
     {'fused': 1, 'max_chunk': 1024, 'two_lane': 0},
     {'fused': 1, 'max_chunk': 128, 'two_lane': 0},
-    {'fused': 1, 'max_chunk': 1024, 'two_lane': 0, 'accum_quad': 1},
-    # FP32 transforms only: twice the batch (twice the scratch, 37 GiB at 32^3)
-    # fills whole rounds of the device in the Nyquist and projection kernels
+    # FP32 transforms only: four DOF slots per accumulation workgroup and a
+    # 2048-DOF batch (37 GiB of scratch at 32^3), which fills whole rounds of
+    # the device in the Nyquist and projection kernels
     {'fused': 1, 'max_chunk': 2048, 'two_lane': 0, 'accum_quad': 1, 'mixed_only': 1},
     {'fused': 1, 'max_chunk': 0, 'two_lane': 0, 'no_pack': 1},
     {'fused': 1, 'max_chunk': 0, 'two_lane': 0, 'no_hpair': 1},
@@ -79,7 +79,7 @@ def _apply(params, cand):
     c.max_chunk = cand['max_chunk']
 
 
-def tune(params, run, key, reps=2, candidates=CANDIDATES, ndof=0):
+def tune(params, run, key, reps=3, candidates=CANDIDATES, ndof=0):
     """Return ``(handle, report)``; ``run(handle)`` evaluates the operator on
     the caller's inputs (``ndof`` DOFs when known: no candidate's explicit
     batch then exceeds it, so a small problem never holds scratch for more

@@ -85,9 +85,8 @@ def test_user_max_chunk_bounds_every_candidate(monkeypatch):
     h, rep = autotune.tune(p, lambda h: None, key=('bounded',), reps=1)
     assert all(mc == min(256, c['max_chunk'] or 256)
                for (_, mc), c in zip(seen, [c for c in autotune.CANDIDATES if c['max_chunk'] not in (1024, 2048)]))
-    # fused/auto and fused/1024 collapse into one candidate under the bound, and
-    # so do the two four-slot candidates
-    assert len(rep['timings']) == len(autotune.CANDIDATES) - 2
+    # fused/auto and fused/1024 collapse into one candidate under the bound
+    assert len(rep['timings']) == len(autotune.CANDIDATES) - 1
     assert p.c.max_chunk == 256
     autotune.clear_cache()
 
@@ -191,5 +190,5 @@ def test_explicit_batches_capped_at_the_dof_count(monkeypatch):
     h, rep = autotune.tune(p, lambda h: None, key=('ndof',), reps=1, ndof=100)
     assert max(built) == 100 and 0 in built
     chunks = [(t['max_chunk'], t.get('accum_quad', 0)) for t in rep['timings']]
-    assert chunks.count((100, 1)) == 1           # the 1024 and 2048 four-slot candidates merge
+    assert chunks.count((100, 1)) == 1           # the four-slot candidate, capped
     autotune.clear_cache()
