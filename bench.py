@@ -37,9 +37,9 @@ HBM_PEAK = 8000.0                          # GB/s
 K_PROD_TRAFFIC = {
     'source': 'profiles/k_prod_vss32_traffic.txt',
     'shape': {'Nv': 32, 'M': 12, 'Nrho': 16, 'Mpre': 32},
-    'per_dof': {'mixed': 17.66e6, 'fp64': 61.63e6},
+    'per_dof': {'mixed': 17.66e6, 'fp64': 55.86e6},
     # the accumulation kernel (HBM-bound), same runs
-    'accum_per_dof': {'mixed': 14.40e6, 'fp64': 58.44e6},
+    'accum_per_dof': {'mixed': 14.41e6, 'fp64': 53.29e6},
 }
 
 
